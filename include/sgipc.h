@@ -1,0 +1,173 @@
+/*
+ * sgipc.h -- C ABI of libsgipc.so, the MI355X (gfx950) implementation of the
+ * Stiff-GIPC Newton-step hot path.
+ *
+ * Conventions
+ *   - every function returns 0 on success, a negative SGIPC_E* code on error;
+ *     sgipc_last_error() returns a thread-local message for the last failure
+ *     (the reference aborts instead: cuda_tools.h:172-206).
+ *   - all pointers are HOST pointers; the library owns its device memory.
+ *     Hot-path work runs on device-resident state; inputs are copied once.
+ *   - 3x3 blocks are ROW-MAJOR doubles (reference: column-major Eigen 3x3,
+ *     linear_system/linear_system/global_matrix.h:18-203).
+ *   - BSR matrices at this boundary are full (both triangles) CSR-of-blocks:
+ *     rowptr[nb+1], cols[nnzb], vals[9*nnzb].  Columns sorted within a row.
+ *   - one context per host thread; calls on one context are not thread safe.
+ */
+#ifndef SGIPC_H
+#define SGIPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "sgipc_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGIPC_OK          0
+#define SGIPC_EINVAL     -1 /* bad argument / shape                 */
+#define SGIPC_EHIP       -2 /* HIP runtime error                    */
+#define SGIPC_ECAPACITY  -3 /* a device buffer capacity was exceeded */
+#define SGIPC_ENUMERIC   -4 /* non-finite value / solver breakdown   */
+#define SGIPC_ESTATE     -5 /* call out of order                     */
+
+typedef struct sgipc_context* sgipc_t;
+typedef struct sgipc_linear_system* sgipc_ls_t;
+
+/* message for the last error on this thread ("" if none) */
+const char* sgipc_last_error(void);
+/* "gfx950 <name> CUs=<n>" of the current device, into buf */
+int sgipc_device_info(char* buf, int32_t buflen);
+
+/* ------------------------------------------------------------------------ */
+/* PCG configuration and statistics                                          */
+/* replaces gipc::PCGSolverConfig (linear_system/solver/pcg_solver.h:6-15)  */
+/* ------------------------------------------------------------------------ */
+typedef struct sgipc_pcg_cfg {
+    double  rel_tol;          /* stop when rz <= rel_tol^2 * rz0 (or rr)       */
+    double  abs_tol;          /* ... or rz <= abs_tol^2                        */
+    int32_t max_iter;         /* 0 -> number of DOFs                           */
+    int32_t use_preconditioned_norm; /* 1: rz test (reference default), 0: rr */
+    int32_t check_every;      /* convergence test period (reference: 10)       */
+    int32_t solver_precision; /* enum sgipc_solver_precision                   */
+} sgipc_pcg_cfg;
+
+typedef struct sgipc_pcg_stats {
+    int32_t iterations;       /* PCG iterations (reference count semantics)    */
+    int32_t outer_iterations; /* refinement sweeps (mixed mode), 1 otherwise   */
+    int32_t converged;
+    int32_t pad;
+    double  rel_residual;     /* ||b - A x||_2 / ||b||_2 recomputed in FP64    */
+    double  rz0, rz;          /* preconditioned residual norms (last sweep)    */
+    double  ms;               /* device time of the solve                      */
+} sgipc_pcg_stats;
+
+/* ------------------------------------------------------------------------ */
+/* Stand-alone linear system: BSR + preconditioner + PCG                     */
+/* replaces gipc::GlobalLinearSystem::solve_loaded_system                    */
+/*   (global_linear_system.cu:407-490), PCGSolver::solve (pcg_solver.cu:155),*/
+/*   MASPreconditioner::{setPreconditioner_bcoo, preconditioning}            */
+/*   (MASPreconditioner.cu:2169-2251)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Create from a full symmetric BSR.  part_id (may be NULL) gives the MAS
+ * 16-vertex bank of each block row (non-decreasing); MAS connectivity is the
+ * off-diagonal block pattern.  precond: enum sgipc_precond. */
+int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr,
+                    const int32_t* cols, const double* vals,
+                    const int32_t* part_id, int32_t precond,
+                    int32_t mas_precision, sgipc_ls_t* out);
+int sgipc_ls_destroy(sgipc_ls_t ls);
+/* y = A x  (x, y: 3*nb doubles) */
+int sgipc_ls_spmv(sgipc_ls_t ls, const double* x, double* y);
+/* z = M^-1 r */
+int sgipc_ls_precond_apply(sgipc_ls_t ls, const double* r, double* z);
+/* solve A x = b from x0 = 0 */
+int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x,
+                 const sgipc_pcg_cfg* cfg, sgipc_pcg_stats* stats);
+/* MAS hierarchy: levels, and level_sizes[2*(levels+1)] = (count, offset)
+ * pairs in the reference's d_levelSize layout (MASPreconditioner.cu:1768). */
+int sgipc_ls_mas_info(sgipc_ls_t ls, int32_t* levels, int32_t* level_sizes,
+                      int32_t* total_clusters);
+/* coarse node chain per vertex: chain[v*(levels-1) + l-1] = global id of
+ * vertex v's node at level l (reference itable, _aggregationKernel:592) */
+int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain);
+
+/* ------------------------------------------------------------------------ */
+/* Simulation context (GIPC counterpart)                                     */
+/* ------------------------------------------------------------------------ */
+int sgipc_create(const sgipc_scene_desc* desc, sgipc_t* out);
+int sgipc_destroy(sgipc_t ctx);
+
+int sgipc_set_positions(sgipc_t ctx, const double* x);   /* [3N] */
+int sgipc_get_positions(sgipc_t ctx, double* x);         /* [3N] */
+int sgipc_get_velocities(sgipc_t ctx, double* v);        /* [3N] */
+/* x_tilde = x_prev + dt v + dt^2 g on free vertices (GIPC.cu:8016-8038) */
+int sgipc_set_xtilde(sgipc_t ctx, const double* xt);     /* [3N] */
+int sgipc_get_xtilde(sgipc_t ctx, double* xt);           /* [3N] */
+int sgipc_get_kappa(sgipc_t ctx, double* kappa);
+int sgipc_set_kappa(sgipc_t ctx, double kappa);
+
+/* LBVH over the surface faces (which = 0) or edges (which = 1) at the current
+ * positions (lbvh_f/lbvh_e::Construct, mlbvh.cu:2074-2157).
+ *   keys  [n]        sorted 64-bit (morton30 << 32 | index) keys
+ *   nodes [4*(2n-1)] {parent, left, right, element} (0xFFFFFFFF = none)
+ *   aabbs [6*(2n-1)] {upper xyz, lower xyz} per node
+ * any output may be NULL. */
+int sgipc_lbvh_build(sgipc_t ctx, int32_t which, uint64_t* keys,
+                     uint32_t* nodes, double* aabbs);
+
+/* Proximity contact set at the current positions (buildCP):
+ *   pairs[4*cap] signed int4 codes (mlbvh.cu:563-1057 encoding),
+ *   counts[5] = {total, unused, PP, PE, PT+EE} (reference h_cpNum),
+ *   ground[gcap] vertex ids with (n.x - o)^2 <= dhat^2 (GIPC.cu:6435). */
+int sgipc_contact_query(sgipc_t ctx, int32_t* pairs, int32_t cap,
+                        int32_t* counts, uint32_t* ground, int32_t gcap,
+                        int32_t* n_ground);
+
+/* Gradient and Hessian at the current positions (computeGradientAndHessian,
+ * GIPC.cu:10012-10228).  grad[3N] is the assembled right-hand side (zero on
+ * boundary vertices).  The Hessian is returned as a full BSR; pass cap = 0
+ * to query nnzb only. */
+int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
+                   int32_t* cols, double* vals, int32_t cap);
+
+/* Incremental potential at the current positions, FP64 (computeEnergy,
+ * GIPC.cu:10392-10451).  terms[8] = {total, kinetic, elastic, barrier,
+ * ground, friction, reserved, reserved}. */
+int sgipc_energy(sgipc_t ctx, double* terms);
+
+typedef struct sgipc_iter_stats {
+    int32_t pcg_iterations;
+    int32_t n_pairs;          /* self contact records                  */
+    int32_t n_ground;         /* ground contacts                       */
+    int32_t line_search_steps;
+    int32_t converged;        /* Newton stop test met before solving   */
+    int32_t pad;
+    double  pcg_rel_residual; /* FP64 ||b-Ax||/||b||                   */
+    double  alpha;            /* accepted step                         */
+    double  alpha_ccd;        /* CCD / ground bound before line search */
+    double  xdelta;           /* ||dx||_inf of the previous direction  */
+    double  energy0, energy1;
+    double  ms_assembly, ms_solve, ms_ccd, ms_line_search, ms_total;
+} sgipc_iter_stats;
+
+/* One Newton iteration of solve_subIP (GIPC.cu:10746-10862): assemble,
+ * stop test, MAS-PCG solve, ground/self CCD bound, line search, kappa. */
+int sgipc_newton_iteration(sgipc_t ctx, sgipc_iter_stats* stats);
+
+typedef struct sgipc_step_stats {
+    int32_t newton_iterations;
+    int32_t pcg_iterations;   /* summed over the step */
+    double  ms_total;
+} sgipc_step_stats;
+
+/* One time step (IPC_Solver, GIPC.cu:10940-11131): kappa setup, Newton loop,
+ * velocity update, next x_tilde. */
+int sgipc_time_step(sgipc_t ctx, sgipc_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGIPC_H */

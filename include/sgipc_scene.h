@@ -1,0 +1,94 @@
+/*
+ * sgipc_scene.h -- plain-data scene descriptor shared by the HIP library
+ * (libsgipc.so) and the CPU oracle.  Host arrays only; the library copies
+ * everything it needs to device memory at sgipc_create().
+ *
+ * Mirrors the inputs the reference gathers in StiffGIPC/main.cu:263-356
+ * (apply_settings, initFEM) and StiffGIPC/main.cu:47-75 (setMAS_partition),
+ * flattened to C arrays:
+ *   - positions are interleaved xyz doubles (reference: double3 AoS),
+ *   - tets are 4 x uint32 (reference: uint4), faces 3 x uint32, edges 2 x uint32,
+ *   - dhat is an absolute DISTANCE in metres; the squared threshold used by
+ *     every contact test is dhat*dhat (reference: dHat = rel^2 * bboxDiag^2,
+ *     GIPC.cu:8543, already squared).
+ */
+#ifndef SGIPC_SCENE_H
+#define SGIPC_SCENE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* preconditioner choice (reference: pcg_data.P_type, gipc/gipc.cu:60-72) */
+enum sgipc_precond {
+    SGIPC_PRECOND_DIAG = 0, /* 3x3 block Jacobi (DiagPreconditioner)          */
+    SGIPC_PRECOND_MAS  = 1  /* multilevel additive Schwarz (MAS_Preconditioner) */
+};
+
+/* arithmetic of the MAS block inverses and multilevel R/Z vectors */
+enum sgipc_mas_precision {
+    SGIPC_MAS_FP32 = 0, /* reference-faithful: FP64 assembly, FP32 inverse/apply */
+    SGIPC_MAS_FP64 = 1  /* all-FP64 MAS                                           */
+};
+
+/* linear-solver precision mode */
+enum sgipc_solver_precision {
+    SGIPC_SOLVE_FP64  = 0, /* FP64 SpMV / PCG vectors (reference default)          */
+    SGIPC_SOLVE_MIXED = 1  /* FP32 SpMV+MAS inner PCG, FP64 residual refinement    */
+};
+
+typedef struct sgipc_scene_desc {
+    /* ---- vertices ---- */
+    int32_t         n_verts;
+    const double*   x;       /* [3*n_verts] positions at the start              */
+    const double*   x_rest;  /* [3*n_verts] rest positions (NULL -> x)          */
+    const double*   v;       /* [3*n_verts] velocities (NULL -> zero)           */
+    const int32_t*  btype;   /* [n_verts] 0 free, !=0 fixed/obstacle (NULL -> 0)*/
+    const int32_t*  body_id; /* [n_verts] body id, -1 = self contact allowed    */
+    const int32_t*  part_id; /* [n_verts] 16-vertex bank id, non-decreasing in
+                                vertex order (NULL -> runs of 16 vertices)       */
+    /* ---- volumetric elements ---- */
+    int32_t         n_tets;
+    const uint32_t* tets;    /* [4*n_tets]                                      */
+    const double*   tet_E;   /* [n_tets] Young's modulus                        */
+    const double*   tet_nu;  /* [n_tets] Poisson ratio                          */
+    const double*   tet_rho; /* [n_tets] density                                */
+    /* ---- collision surface ---- */
+    int32_t         n_faces;
+    const uint32_t* faces;      /* [3*n_faces] */
+    int32_t         n_edges;
+    const uint32_t* edges;      /* [2*n_edges] */
+    int32_t         n_surf_verts;
+    const uint32_t* surf_verts; /* [n_surf_verts] */
+    /* ---- physics ---- */
+    double  dt;
+    double  dhat;           /* contact distance threshold (m), squared internally */
+    double  gravity[3];
+    int32_t use_ground;
+    double  ground_normal[3];
+    double  ground_offset;
+    double  kappa;          /* barrier stiffness; <= 0 -> adaptive (suggestKappa) */
+    double  friction_mu;    /* reserved: lagged friction                          */
+    /* ---- Newton / line search (reference JSON "settings") ---- */
+    double  abs_xdelta_tol; /* stop when |dx|_inf < scene_diag * abs_xdelta_tol  */
+    double  rel_xdelta_tol; /* ... or < rel_xdelta_tol * |dx0|_inf               */
+    double  armijo_c1;
+    double  armijo_beta;
+    double  armijo_alpha_min;
+    int32_t newton_max_iter;
+    /* ---- linear solver ---- */
+    double  pcg_rel_tol;
+    double  pcg_abs_tol;
+    int32_t pcg_max_iter;   /* 0 -> number of DOFs                              */
+    int32_t pcg_use_preconditioned_norm;
+    int32_t precond;        /* enum sgipc_precond                               */
+    int32_t mas_precision;  /* enum sgipc_mas_precision                         */
+    int32_t solver_precision; /* enum sgipc_solver_precision                    */
+} sgipc_scene_desc;
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGIPC_SCENE_H */

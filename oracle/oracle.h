@@ -1,0 +1,165 @@
+/*
+ * oracle.h -- CPU FP64 restatement of the Stiff-GIPC Newton-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for libsgipc.so:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  Nothing in the product path links or calls it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates.
+ * Matrices are row-major; 3x3 blocks are row-major 9-double arrays.
+ * Built with -ffp-contract=off so FP64 results do not depend on FMA fusion.
+ */
+#ifndef SGIPC_ORACLE_H
+#define SGIPC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/sgipc_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- dense helpers (orc_math.c) ---------------- */
+/* cyclic Jacobi eigen-decomposition of a symmetric n x n matrix A (row-major).
+ * w[n] ascending eigenvalues, V[n*n] with eigenvector k in column k. */
+void orc_sym_eig(int n, const double* A, double* w, double* V);
+/* A <- V max(w,0) V^T  (PSD projection by eigenvalue clamping) */
+void orc_psd_project(int n, double* A);
+/* in-place inverse by Gauss-Jordan with partial pivoting; returns 0 if ok */
+int orc_inverse(int n, double* A);
+
+/* ---------------- FEM (orc_fem.c) ---------------- */
+double orc_snk1_psi(const double F[9], double mu, double lam);
+void   orc_snk1_pk1(const double F[9], double mu, double lam, double P[9]);
+/* 9x9 Hessian d2psi/dvec(F)^2, vec column-major (index i + 3 j); project=1
+ * clamps negative eigenvalues */
+void   orc_snk1_hess(const double F[9], double mu, double lam, int project,
+                     double H[81]);
+/* Dm^-1, |volume|, Lame mu, lambda, lumped mass (initFEM, main.cu:310-356) */
+void   orc_fem_precompute(int nv, const double* x_rest, int nt,
+                          const uint32_t* tets, const double* E,
+                          const double* nu, const double* rho, double* dminv,
+                          double* vol, double* mu, double* lam, double* mass);
+/* per-tet dt^2 vol-scaled gradient (12) and projected Hessian (12x12)
+ * (_calculate_fem_gradient_hessian, femEnergy.cu:1789-1926) */
+void   orc_tet_grad_hess(const double* x, const uint32_t t[4],
+                         const double dminv[9], double vol, double mu,
+                         double lam, double dt, double g[12], double H[144]);
+double orc_tet_energy(const double* x, const uint32_t t[4],
+                      const double dminv[9], double vol, double mu, double lam);
+
+/* ground IPC barrier b(d^2) = -(d^2 - dhat2)^2 ln(d^2/dhat2) per vertex,
+ * unscaled by kappa (GIPC.cu:6465-6516, 6839-6860) */
+double orc_ground_energy(double dist2, double dhat2);
+/* gradient w.r.t. x (3) and 3x3 block, scaled by kappa */
+void   orc_ground_grad_hess(const double x[3], const double n[3], double off,
+                            double dhat2, double kappa, double g[3],
+                            double H[9]);
+
+/* ---------------- LBVH (orc_lbvh.c) ---------------- */
+uint32_t orc_expand_bits(uint32_t v);
+uint32_t orc_morton_code(double x, double y, double z);
+/* leaf AABBs {upper xyz, lower xyz} of n primitives of arity 2 or 3 */
+void orc_leaf_aabbs(const double* x, const uint32_t* prims, int arity, int n,
+                    double* aabb);
+/* full construction: keys[n] sorted, nodes[4*(2n-1)], aabbs[6*(2n-1)]
+ * (lbvh_f::Construct, mlbvh.cu:2074-2091; Karras 2012) */
+void orc_lbvh_build(const double* x, const uint32_t* prims, int arity, int n,
+                    uint64_t* keys, uint32_t* nodes, double* aabbs);
+/* same from precomputed leaf boxes (used for swept CCD boxes) */
+void orc_lbvh_build_boxes(const double* leaf, int n, uint64_t* keys,
+                          uint32_t* nodes, double* aabbs);
+
+/* ---------------- contacts (orc_contact.c) ---------------- */
+double orc_d_pp(const double* v0, const double* v1);
+double orc_d_pe(const double* v0, const double* v1, const double* v2);
+double orc_d_pt(const double* v0, const double* v1, const double* v2,
+                const double* v3);
+double orc_d_ee(const double* v0, const double* v1, const double* v2,
+                const double* v3);
+int    orc_dtype_pt(const double* v0, const double* v1, const double* v2,
+                    const double* v3);
+int    orc_dtype_ee(const double* v0, const double* v1, const double* v2,
+                    const double* v3);
+/* proximity pairs by brute force over (surface vertex, face) and
+ * (edge, edge); returns the number written (<= cap); counts[5] as the
+ * reference's h_cpNum.  pairs are the signed int4 codes. */
+int orc_contact_pairs_bruteforce(const sgipc_scene_desc* s, const double* x,
+                                 double dhat2, int32_t* pairs, int cap,
+                                 int32_t counts[5]);
+/* same set via the LBVH traversal (mlbvh.cu:1308-1634) */
+int orc_contact_pairs_bvh(const sgipc_scene_desc* s, const double* x,
+                          double dhat2, int32_t* pairs, int cap,
+                          int32_t counts[5]);
+/* barrier energy of one contact code (kappa included, GIPC.cu:703-959) */
+double orc_barrier_energy(const double* x, const double* x_rest,
+                          const int32_t code[4], double dhat2, double kappa);
+/* gradient (on up to 4 vertices) and GIPC Gauss-Newton Hessian; returns the
+ * number of vertices nv (2, 3 or 4); ids[nv]; g[3*nv]; H[(3nv)^2]
+ * (_calBarrierGradientAndHessian, GIPC.cu:3108-4947) */
+int orc_barrier_grad_hess(const double* x, const double* x_rest,
+                          const int32_t code[4], double dhat2, double kappa,
+                          int32_t ids[4], double* g, double* H);
+/* second derivative of kappa dhat2^2 (I-1)^2 ln^2 I in s = sqrt(I), with the
+ * small-distance clamp (GIPC.cu:3272-3289) */
+double orc_barrier_lambda0(double I5, double dhat2, double kappa);
+
+/* ---------------- linear system (orc_linsys.c) ---------------- */
+typedef struct orc_bsr {
+    int nb, nnzb;
+    int* rowptr; /* nb+1 */
+    int* cols;   /* nnzb */
+    double* vals;/* 9*nnzb row-major blocks */
+} orc_bsr;
+
+/* merge upper triplets (row <= col) into a full symmetric BSR
+ * (Converter::convert, converter.cu:23-189) */
+int  orc_bsr_from_upper_triplets(int nb, int ntrip, const int* rows,
+                                 const int* cols, const double* vals,
+                                 orc_bsr* out);
+void orc_bsr_free(orc_bsr* A);
+void orc_bsr_spmv(const orc_bsr* A, const double* x, double* y);
+
+typedef struct orc_mas orc_mas;
+/* build the MAS hierarchy + block inverses for A (FEM range = all rows).
+ * part_id: bank ids (non-decreasing, NULL -> runs of 16).  fp32 = 1 rounds
+ * inverse blocks and R/Z to float as the reference (eigen_data.h:125). */
+orc_mas* orc_mas_create(const orc_bsr* A, const int32_t* part_id, int fp32);
+void     orc_mas_free(orc_mas* m);
+void     orc_mas_apply(const orc_mas* m, const double* r, double* z);
+int      orc_mas_levels(const orc_mas* m);
+void     orc_mas_level_sizes(const orc_mas* m, int32_t* sizes);
+int      orc_mas_total_clusters(const orc_mas* m);
+void     orc_mas_chain(const orc_mas* m, int32_t* chain);
+
+/* PCG (pcg_solver.cu:171-304).  precond: 0 block-Jacobi, 1 MAS, -1 none */
+int orc_pcg(const orc_bsr* A, const orc_mas* mas, int precond,
+            const double* b, double* x, double rel_tol, double abs_tol,
+            int max_iter, int use_rz, int check_every, double* rz0_out,
+            double* rz_out);
+
+/* ---------------- world / Newton (orc_world.c) ---------------- */
+typedef struct orc_world orc_world;
+orc_world* orc_world_create(const sgipc_scene_desc* s);
+void       orc_world_free(orc_world* w);
+void       orc_world_set_positions(orc_world* w, const double* x);
+void       orc_world_get_positions(const orc_world* w, double* x);
+void       orc_world_set_xtilde(orc_world* w, const double* xt);
+void       orc_world_get_xtilde(const orc_world* w, double* xt);
+double     orc_world_get_kappa(const orc_world* w);
+void       orc_world_set_kappa(orc_world* w, double k);
+/* assembled RHS gradient (3N) and full BSR Hessian (caller frees) */
+int  orc_world_assemble(orc_world* w, double* grad, orc_bsr* H);
+/* terms[8] as sgipc_energy */
+void orc_world_energy(orc_world* w, double* terms);
+/* one Newton iteration; stats fields mirror sgipc_iter_stats */
+int  orc_world_newton_iteration(orc_world* w, double* alpha_out,
+                                int* pcg_iters_out, double* xdelta_out,
+                                int* converged_out);
+int  orc_world_time_step(orc_world* w, int* newton_iters_out,
+                         int* pcg_iters_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1190 @@
+/*
+ * orc_world.c -- one Newton iteration / time step of Stiff-GIPC on the CPU.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restates StiffGIPC/GIPC.cu:
+ *   computeGradientAndHessian        :10012-10228 (term order, boundary zeroing)
+ *   computeEnergy                    :10392-10451
+ *   lineSearch                       :10573-10670
+ *   solve_subIP (loop body)          :10746-10862
+ *   IPC_Solver                       :10940-11131
+ *   suggestKappa/upperBoundKappa/initKappa :9746-9831
+ *   ground/self/CFL step bounds      :6904-6980, 7054-7217, 8821-8968
+ *   _edgeTriIntersectionQuery, segTriIntersect :8188-8302, :961-1013
+ *   _stepForward/_updateVelocities/_computeXTilta :7880-7920, :8016-8038
+ * and StiffGIPC/ACCD.cu point_triangle_ccd / edge_edge_ccd :373-524.
+ * Friction is not part of this restatement (friction_mu must be 0).
+ * postLineSearch's kappa doubling is not restated: in the reference its host
+ * counters h_close_gpNum / h_close_cpNum are never read back from the device
+ * (GIPC.cuh:69-70, GIPC.cu:8737, 9176), so the doubling never fires.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+struct orc_world
+{
+    sgipc_scene_desc s;
+    int              nv, nt;
+    double *x, *xr, *xprev, *v, *xt, *mass, *dminv, *vol, *mu, *lam, *dx, *xsave;
+    int32_t *btype, *body, *part;
+    uint32_t *tets, *faces, *edges, *surf;
+    double    dhat2, kappa, bbox_diag2, scene_diag, mean_mass;
+    int       cap, npairs, nccd, ngp;
+    int32_t*  pairs;
+    int32_t*  ccd;
+    int32_t   counts[5];
+    uint32_t* gp;
+    double *  gcontact, *gshape;
+    int       newton_k;
+    double    xdelta0;
+};
+
+static void* dup(const void* p, size_t bytes)
+{
+    if(!p || bytes == 0)
+        return NULL;
+    void* q = malloc(bytes);
+    memcpy(q, p, bytes);
+    return q;
+}
+
+static int bt(const orc_world* w, int v) { return w->btype ? w->btype[v] : 0; }
+
+/* ---------------------------------------------------------------- contacts */
+static void build_cp(orc_world* w)
+{
+    w->npairs = orc_contact_pairs_bvh(&w->s, w->x, w->dhat2, w->pairs, w->cap, w->counts);
+    if(w->npairs > w->cap)
+        w->npairs = w->cap;
+    w->ngp = 0;
+    if(w->s.use_ground)
+    {
+        const double* n = w->s.ground_normal;
+        for(int i = 0; i < w->s.n_surf_verts; i++)
+        {
+            int           v = (int)w->surf[i];
+            const double* p = w->x + 3 * v;
+            double        d = n[0] * p[0] + n[1] * p[1] + n[2] * p[2] - w->s.ground_offset;
+            if(d * d > w->dhat2)
+                continue;
+            w->gp[w->ngp++] = (uint32_t)v;
+        }
+    }
+}
+
+/* The reference stores, next to each proximity code, the raw candidate that
+ * produced it (_ccd_collisionPair): (-v-1, f0, f1, f2) for a vertex-face test
+ * and (e0, e1, e2, e3) for an edge-edge test.  The oracle recomputes them by
+ * re-running the candidate enumeration with a flag that records candidates. */
+typedef struct
+{
+    int32_t* out;
+    int      n, cap;
+} cand_t;
+
+static void cand_push(cand_t* c, int32_t a, int32_t b, int32_t d, int32_t e)
+{
+    if(c->n < c->cap)
+    {
+        c->out[4 * c->n + 0] = a;
+        c->out[4 * c->n + 1] = b;
+        c->out[4 * c->n + 2] = d;
+        c->out[4 * c->n + 3] = e;
+    }
+    c->n++;
+}
+
+static int ovl(const double* l, const double* r, double gap)
+{
+    for(int c = 0; c < 3; c++)
+        if((r[3 + c] - l[c]) >= gap || (l[3 + c] - r[c]) >= gap)
+            return 0;
+    return 1;
+}
+
+static int body_of(const orc_world* w, int v) { return w->body ? w->body[v] : -1; }
+
+/* swept-box candidates (ConstructFullCCD + _selfQuery_*_ccd, mlbvh.cu:1410-1740) */
+static int full_ccd_candidates(orc_world* w, double alpha, int32_t* out, int cap)
+{
+    cand_t          cd  = {out, 0, cap};
+    double          gap = sqrt(w->dhat2);
+    const double*   X   = w->x;
+    const double*   D   = w->dx;
+    uint32_t        stack[64];
+    const sgipc_scene_desc* s = &w->s;
+    if(s->n_faces > 0)
+    {
+        int       n    = s->n_faces;
+        double*   leaf = (double*)malloc(sizeof(double) * 6 * n);
+        uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * n);
+        uint32_t* nd   = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (2 * n - 1));
+        double*   bv   = (double*)malloc(sizeof(double) * 6 * (2 * n - 1));
+        for(int e = 0; e < n; e++)
+        {
+            double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
+            for(int k = 0; k < 3; k++)
+            {
+                int v = (int)s->faces[3 * e + k];
+                for(int c = 0; c < 3; c++)
+                {
+                    double a = X[3 * v + c], b = X[3 * v + c] - D[3 * v + c] * alpha;
+                    lo[c]    = (a < lo[c]) ? a : lo[c];
+                    up[c]    = (up[c] < a) ? a : up[c];
+                    lo[c]    = (b < lo[c]) ? b : lo[c];
+                    up[c]    = (up[c] < b) ? b : up[c];
+                }
+            }
+            for(int c = 0; c < 3; c++)
+            {
+                leaf[6 * e + c]     = up[c];
+                leaf[6 * e + 3 + c] = lo[c];
+            }
+        }
+        orc_lbvh_build_boxes(leaf, n, keys, nd, bv);
+        for(int i = 0; i < s->n_surf_verts; i++)
+        {
+            int    v = (int)s->surf_verts[i];
+            double q[6];
+            for(int c = 0; c < 3; c++)
+            {
+                double a = X[3 * v + c], b = X[3 * v + c] - D[3 * v + c] * alpha;
+                q[c]     = a;
+                q[3 + c] = a;
+                q[3 + c] = (b < q[3 + c]) ? b : q[3 + c];
+                q[c]     = (q[c] < b) ? b : q[c];
+            }
+            int sp      = 0;
+            stack[sp++] = 0;
+            if(n == 1)
+                sp = 0;
+            do
+            {
+                if(n == 1)
+                {
+                    if(ovl(q, bv, gap))
+                    {
+                        const uint32_t* F = s->faces;
+                        if(((body_of(w, v) != body_of(w, F[0])) || body_of(w, v) == -1)
+                           && !(bt(w, v) >= 2 && bt(w, F[0]) >= 2 && bt(w, F[1]) >= 2
+                                && bt(w, F[2]) >= 2)
+                           && v != (int)F[0] && v != (int)F[1] && v != (int)F[2])
+                            cand_push(&cd, -v - 1, F[0], F[1], F[2]);
+                    }
+                    break;
+                }
+                uint32_t node  = stack[--sp];
+                uint32_t ch[2] = {nd[4 * node + 1], nd[4 * node + 2]};
+                for(int c = 0; c < 2; c++)
+                {
+                    uint32_t C = ch[c];
+                    if(!ovl(q, bv + 6 * C, gap))
+                        continue;
+                    uint32_t obj = nd[4 * C + 3];
+                    if(obj != 0xFFFFFFFFu)
+                    {
+                        const uint32_t* F = s->faces + 3 * obj;
+                        if(((body_of(w, v) != body_of(w, F[0])) || body_of(w, v) == -1)
+                           && !(bt(w, v) >= 2 && bt(w, F[0]) >= 2 && bt(w, F[1]) >= 2
+                                && bt(w, F[2]) >= 2)
+                           && v != (int)F[0] && v != (int)F[1] && v != (int)F[2])
+                            cand_push(&cd, -v - 1, F[0], F[1], F[2]);
+                    }
+                    else if(sp < 64)
+                        stack[sp++] = C;
+                }
+            } while(sp > 0);
+        }
+        free(leaf);
+        free(keys);
+        free(nd);
+        free(bv);
+    }
+    if(s->n_edges > 1)
+    {
+        int       n    = s->n_edges;
+        double*   leaf = (double*)malloc(sizeof(double) * 6 * n);
+        uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * n);
+        uint32_t* nd   = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (2 * n - 1));
+        double*   bv   = (double*)malloc(sizeof(double) * 6 * (2 * n - 1));
+        for(int e = 0; e < n; e++)
+        {
+            double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
+            for(int k = 0; k < 2; k++)
+            {
+                int v = (int)s->edges[2 * e + k];
+                for(int c = 0; c < 3; c++)
+                {
+                    double a = X[3 * v + c], b = X[3 * v + c] - D[3 * v + c] * alpha;
+                    lo[c]    = (a < lo[c]) ? a : lo[c];
+                    up[c]    = (up[c] < a) ? a : up[c];
+                    lo[c]    = (b < lo[c]) ? b : lo[c];
+                    up[c]    = (up[c] < b) ? b : up[c];
+                }
+            }
+            for(int c = 0; c < 3; c++)
+            {
+                leaf[6 * e + c]     = up[c];
+                leaf[6 * e + 3 + c] = lo[c];
+            }
+        }
+        orc_lbvh_build_boxes(leaf, n, keys, nd, bv);
+        for(int i = 0; i < n; i++)
+        {
+            int           lf   = i + n - 1;
+            const double* q    = bv + 6 * lf;
+            int           self = (int)nd[4 * lf + 3];
+            const uint32_t* A  = s->edges + 2 * self;
+            int           sp   = 0;
+            stack[sp++]        = 0;
+            do
+            {
+                uint32_t node  = stack[--sp];
+                uint32_t ch[2] = {nd[4 * node + 1], nd[4 * node + 2]};
+                for(int c = 0; c < 2; c++)
+                {
+                    uint32_t C = ch[c];
+                    if(!ovl(q, bv + 6 * C, gap))
+                        continue;
+                    uint32_t obj = nd[4 * C + 3];
+                    if(obj != 0xFFFFFFFFu)
+                    {
+                        const uint32_t* B = s->edges + 2 * obj;
+                        if((int)obj != self
+                           && ((body_of(w, A[0]) != body_of(w, B[0])) || body_of(w, A[0]) == -1)
+                           && !(A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]
+                                || (int)obj < self)
+                           && !(bt(w, A[0]) >= 2 && bt(w, A[1]) >= 2 && bt(w, B[0]) >= 2
+                                && bt(w, B[1]) >= 2))
+                            cand_push(&cd, A[0], A[1], B[0], B[1]);
+                    }
+                    else if(sp < 64)
+                        stack[sp++] = C;
+                }
+            } while(sp > 0);
+        }
+        free(leaf);
+        free(keys);
+        free(nd);
+        free(bv);
+    }
+    return cd.n;
+}
+
+/* -------------------------------------------------------------------- ACCD */
+typedef struct
+{
+    double x, y, z;
+} v3;
+static v3 V(const double* p)
+{
+    v3 r = {p[0], p[1], p[2]};
+    return r;
+}
+static v3 vadd(v3 a, v3 b)
+{
+    v3 r = {a.x + b.x, a.y + b.y, a.z + b.z};
+    return r;
+}
+static v3 vsub(v3 a, v3 b)
+{
+    v3 r = {a.x - b.x, a.y - b.y, a.z - b.z};
+    return r;
+}
+static v3 vmul(v3 a, double s)
+{
+    v3 r = {a.x * s, a.y * s, a.z * s};
+    return r;
+}
+static double vsq(v3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static double dmax(double a, double b) { return (a < b) ? b : a; }
+static double dmin(double a, double b) { return (b < a) ? b : a; }
+
+static double pt_dist_unclassified(v3 p, v3 t0, v3 t1, v3 t2)
+{
+    double P[3] = {p.x, p.y, p.z}, A[3] = {t0.x, t0.y, t0.z}, B[3] = {t1.x, t1.y, t1.z},
+           C[3] = {t2.x, t2.y, t2.z};
+    switch(orc_dtype_pt(P, A, B, C))
+    {
+        case 0: return orc_d_pp(P, A);
+        case 1: return orc_d_pp(P, B);
+        case 2: return orc_d_pp(P, C);
+        case 3: return orc_d_pe(P, A, B);
+        case 4: return orc_d_pe(P, B, C);
+        case 5: return orc_d_pe(P, C, A);
+        case 6: return orc_d_pt(P, A, B, C);
+        default: return 1e32;
+    }
+}
+
+static double ee_dist_unclassified(v3 a0, v3 a1, v3 b0, v3 b1)
+{
+    double A0[3] = {a0.x, a0.y, a0.z}, A1[3] = {a1.x, a1.y, a1.z}, B0[3] = {b0.x, b0.y, b0.z},
+           B1[3] = {b1.x, b1.y, b1.z};
+    switch(orc_dtype_ee(A0, A1, B0, B1))
+    {
+        case 0: return orc_d_pp(A0, B0);
+        case 1: return orc_d_pp(A0, B1);
+        case 2: return orc_d_pe(A0, B0, B1);
+        case 3: return orc_d_pp(A1, B0);
+        case 4: return orc_d_pp(A1, B1);
+        case 5: return orc_d_pe(A1, B0, B1);
+        case 6: return orc_d_pe(B0, A0, A1);
+        case 7: return orc_d_pe(B1, A0, A1);
+        case 8: return orc_d_ee(A0, A1, B0, B1);
+        default: return 1e32;
+    }
+}
+
+static double pt_ccd(v3 p, v3 t0, v3 t1, v3 t2, v3 dp, v3 dt0, v3 dt1, v3 dt2, double eta,
+                     double th)
+{
+    v3 mov = vmul(vadd(vadd(dt0, dt1), vadd(dt2, dp)), -0.25);
+    dt0 = vadd(dt0, mov);
+    dt1 = vadd(dt1, mov);
+    dt2 = vadd(dt2, mov);
+    dp  = vadd(dp, mov);
+    double m0 = vsq(dt0), m1 = vsq(dt1), m2 = vsq(dt2);
+    double maxd = sqrt(vsq(dp)) + sqrt(dmax(m0, dmax(m1, m2)));
+    if(maxd == 0)
+        return 1.0;
+    double d2  = pt_dist_unclassified(p, t0, t1, t2);
+    double d   = sqrt(d2);
+    double gap = eta * (d2 - th * th) / (d + th);
+    double toc = 0.0;
+    int    cnt = 0;
+    while(1)
+    {
+        cnt++;
+        if(cnt > 50000)
+            return toc;
+        double lb = (1 - eta) * (d2 - th * th) / ((d + th) * maxd);
+        p         = vadd(p, vmul(dp, lb));
+        t0        = vadd(t0, vmul(dt0, lb));
+        t1        = vadd(t1, vmul(dt1, lb));
+        t2        = vadd(t2, vmul(dt2, lb));
+        d2        = pt_dist_unclassified(p, t0, t1, t2);
+        d         = sqrt(d2);
+        if(toc && ((d2 - th * th) / (d + th) < gap))
+            break;
+        toc += lb;
+        if(toc > 1.0)
+            return 1.0;
+    }
+    return toc;
+}
+
+static double ee_ccd(v3 a0, v3 a1, v3 b0, v3 b1, v3 da0, v3 da1, v3 db0, v3 db1, double eta,
+                     double th)
+{
+    v3 mov = vmul(vadd(vadd(da0, da1), vadd(db0, db1)), -0.25);
+    da0 = vadd(da0, mov);
+    da1 = vadd(da1, mov);
+    db0 = vadd(db0, mov);
+    db1 = vadd(db1, mov);
+    double maxd = sqrt(dmax(vsq(da0), vsq(da1))) + sqrt(dmax(vsq(db0), vsq(db1)));
+    if(maxd == 0)
+        return 1.0;
+    double d2 = ee_dist_unclassified(a0, a1, b0, b1);
+    double dF = d2 - th * th;
+    if(dF <= 0)
+    {
+        d2 = dmin(dmin(vsq(vsub(a0, b0)), vsq(vsub(a0, b1))),
+                  dmin(vsq(vsub(a1, b0)), vsq(vsub(a1, b1))));
+        dF = d2 - th * th;
+    }
+    double d   = sqrt(d2);
+    double gap = eta * dF / (d + th);
+    double toc = 0.0;
+    int    cnt = 0;
+    while(1)
+    {
+        cnt++;
+        if(cnt > 50000)
+            return toc;
+        double lb = (1 - eta) * dF / ((d + th) * maxd);
+        a0        = vadd(a0, vmul(da0, lb));
+        a1        = vadd(a1, vmul(da1, lb));
+        b0        = vadd(b0, vmul(db0, lb));
+        b1        = vadd(b1, vmul(db1, lb));
+        d2        = ee_dist_unclassified(a0, a1, b0, b1);
+        dF        = d2 - th * th;
+        if(dF <= 0)
+        {
+            d2 = dmin(dmin(vsq(vsub(a0, b0)), vsq(vsub(a0, b1))),
+                      dmin(vsq(vsub(a1, b0)), vsq(vsub(a1, b1))));
+            dF = d2 - th * th;
+        }
+        d = sqrt(d2);
+        if(toc && (dF / (d + th) < gap))
+            break;
+        toc += lb;
+        if(toc > 1.0)
+            return 1.0;
+    }
+    return toc;
+}
+
+static double self_bound(orc_world* w, double slackness, const int32_t* cand, int n)
+{
+    if(n < 1)
+        return 1.0;
+    double mx  = 0.0;
+    double eta = 1.0 - slackness;
+    int    first = 1;
+    for(int i = 0; i < n; i++)
+    {
+        const int32_t* c = cand + 4 * i;
+        double         t;
+        const double*  X = w->x;
+        const double*  D = w->dx;
+        if(c[0] < 0)
+        {
+            int p = -c[0] - 1;
+            t     = 1.0
+                / pt_ccd(V(X + 3 * p), V(X + 3 * c[1]), V(X + 3 * c[2]), V(X + 3 * c[3]),
+                         vmul(V(D + 3 * p), -1), vmul(V(D + 3 * c[1]), -1),
+                         vmul(V(D + 3 * c[2]), -1), vmul(V(D + 3 * c[3]), -1), eta, 0);
+        }
+        else
+        {
+            t = 1.0
+                / ee_ccd(V(X + 3 * c[0]), V(X + 3 * c[1]), V(X + 3 * c[2]), V(X + 3 * c[3]),
+                         vmul(V(D + 3 * c[0]), -1), vmul(V(D + 3 * c[1]), -1),
+                         vmul(V(D + 3 * c[2]), -1), vmul(V(D + 3 * c[3]), -1), eta, 0);
+        }
+        if(first || mx < t)
+            mx = t;
+        first = 0;
+    }
+    return 1.0 / mx;
+}
+
+static double ground_bound(orc_world* w, double slackness)
+{
+    int n = w->s.n_surf_verts;
+    if(n < 1)
+        return 1.0;
+    const double* nn = w->s.ground_normal;
+    double        mx = 0.0;
+    for(int i = 0; i < n; i++)
+    {
+        int           v    = (int)w->surf[i];
+        double        temp = 1.0;
+        const double* d    = w->dx + 3 * v;
+        double        coef = nn[0] * d[0] + nn[1] * d[1] + nn[2] * d[2];
+        if(coef > 0.0)
+        {
+            const double* p    = w->x + 3 * v;
+            double        dist = nn[0] * p[0] + nn[1] * p[1] + nn[2] * p[2] - w->s.ground_offset;
+            temp               = coef / (dist * slackness);
+        }
+        if(i == 0 || mx < temp)
+            mx = temp;
+    }
+    return 1.0 / mx;
+}
+
+static double cfl_speed(orc_world* w)
+{
+    double mx = 0.0;
+    for(int i = 0; i < w->s.n_surf_verts; i++)
+    {
+        const double* d = w->dx + 3 * w->surf[i];
+        double        s = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if(i == 0 || mx < s)
+            mx = s;
+    }
+    return mx;
+}
+
+/* ----------------------------------------------------------- intersection */
+static int seg_tri(v3 e0, v3 e1, v3 t0, v3 t1, v3 t2)
+{
+    v3 c0 = vsub(t1, t0), c1 = vsub(t2, t0), c2 = vsub(e0, e1);
+    v3 n  = {c0.y * c1.z - c0.z * c1.y, c0.z * c1.x - c0.x * c1.z, c0.x * c1.y - c0.y * c1.x};
+    v3 a = vsub(e0, t0), b2 = vsub(e1, t0);
+    if((n.x * a.x + n.y * a.y + n.z * a.z) * (n.x * b2.x + n.y * b2.y + n.z * b2.z) > 0)
+        return 0;
+#define DETC(A, B, C)                                                                         \
+    ((A).x * (B).y * (C).z + (A).y * (B).z * (C).x + (A).z * (B).x * (C).y                   \
+     - (A).z * (B).y * (C).x - (A).x * (B).z * (C).y - (B).x * (A).y * (C).z)
+    double det = DETC(c0, c1, c2);
+    if(fabs(det) < 1e-20)
+        return 0;
+    v3     b = vsub(e0, t0);
+    double u = DETC(b, c1, c2) / det, vv = DETC(c0, b, c2) / det, t = DETC(c0, c1, b) / det;
+#undef DETC
+    return (u >= 0.0 && vv >= 0.0 && u + vv <= 1.0 && t >= 0.0 && t <= 1.0);
+}
+
+static int edge_tri_intersect(orc_world* w)
+{
+    const sgipc_scene_desc* s = &w->s;
+    int                     n = s->n_edges;
+    if(n < 1 || s->n_faces < 1)
+        return 0;
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    uint32_t* nd   = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (2 * n - 1));
+    double*   bv   = (double*)malloc(sizeof(double) * 6 * (2 * n - 1));
+    orc_lbvh_build(w->x, s->edges, 2, n, keys, nd, bv);
+    int      hit = 0;
+    uint32_t stack[64];
+    for(int f = 0; f < s->n_faces && !hit; f++)
+    {
+        const uint32_t* F = s->faces + 3 * f;
+        double          q[6];
+        orc_leaf_aabbs(w->x, F, 3, 1, q);
+        int sp      = 0;
+        stack[sp++] = 0;
+        if(n == 1)
+            sp = 0;
+        while(sp > 0 && !hit)
+        {
+            uint32_t node  = stack[--sp];
+            uint32_t ch[2] = {nd[4 * node + 1], nd[4 * node + 2]};
+            for(int c = 0; c < 2 && !hit; c++)
+            {
+                uint32_t C = ch[c];
+                if(!ovl(q, bv + 6 * C, 0.0))
+                    continue;
+                uint32_t obj = nd[4 * C + 3];
+                if(obj != 0xFFFFFFFFu)
+                {
+                    const uint32_t* E = s->edges + 2 * obj;
+                    if(F[0] == E[0] || F[0] == E[1] || F[1] == E[0] || F[1] == E[1]
+                       || F[2] == E[0] || F[2] == E[1])
+                        continue;
+                    if(bt(w, F[0]) >= 2 && bt(w, F[1]) >= 2 && bt(w, F[2]) >= 2
+                       && bt(w, E[0]) >= 2 && bt(w, E[1]) >= 2)
+                        continue;
+                    if(seg_tri(V(w->x + 3 * E[0]), V(w->x + 3 * E[1]), V(w->x + 3 * F[0]),
+                               V(w->x + 3 * F[1]), V(w->x + 3 * F[2])))
+                        hit = 1;
+                }
+                else if(sp < 64)
+                    stack[sp++] = C;
+            }
+        }
+    }
+    free(keys);
+    free(nd);
+    free(bv);
+    return hit;
+}
+
+static int is_intersected(orc_world* w)
+{
+    if(w->s.use_ground)
+    {
+        const double* n = w->s.ground_normal;
+        for(int i = 0; i < w->ngp; i++)
+        {
+            const double* p = w->x + 3 * w->gp[i];
+            if(n[0] * p[0] + n[1] * p[1] + n[2] * p[2] - w->s.ground_offset < 0)
+                return 1;
+        }
+    }
+    return edge_tri_intersect(w);
+}
+
+/* ------------------------------------------------------------------ world */
+orc_world* orc_world_create(const sgipc_scene_desc* s)
+{
+    orc_world* w = (orc_world*)calloc(1, sizeof(orc_world));
+    w->s         = *s;
+    int nv = s->n_verts, nt = s->n_tets;
+    w->nv = nv;
+    w->nt = nt;
+    size_t b3 = sizeof(double) * 3 * nv;
+    w->x      = (double*)dup(s->x, b3);
+    w->xr     = (double*)dup(s->x_rest ? s->x_rest : s->x, b3);
+    w->xprev  = (double*)dup(s->x, b3);
+    w->v      = (double*)calloc(3 * nv, sizeof(double));
+    if(s->v)
+        memcpy(w->v, s->v, b3);
+    w->xt     = (double*)calloc(3 * nv, sizeof(double));
+    w->dx     = (double*)calloc(3 * nv, sizeof(double));
+    w->xsave  = (double*)calloc(3 * nv, sizeof(double));
+    w->mass   = (double*)calloc(nv, sizeof(double));
+    w->btype  = (int32_t*)dup(s->btype, sizeof(int32_t) * nv);
+    w->body   = (int32_t*)dup(s->body_id, sizeof(int32_t) * nv);
+    w->part   = (int32_t*)dup(s->part_id, sizeof(int32_t) * nv);
+    w->tets   = (uint32_t*)dup(s->tets, sizeof(uint32_t) * 4 * nt);
+    w->faces  = (uint32_t*)dup(s->faces, sizeof(uint32_t) * 3 * s->n_faces);
+    w->edges  = (uint32_t*)dup(s->edges, sizeof(uint32_t) * 2 * s->n_edges);
+    w->surf   = (uint32_t*)dup(s->surf_verts, sizeof(uint32_t) * s->n_surf_verts);
+    w->s.x = w->x;
+    w->s.x_rest = w->xr;
+    w->s.btype = w->btype;
+    w->s.body_id = w->body;
+    w->s.part_id = w->part;
+    w->s.tets = w->tets;
+    w->s.faces = w->faces;
+    w->s.edges = w->edges;
+    w->s.surf_verts = w->surf;
+    w->dminv  = (double*)malloc(sizeof(double) * 9 * (nt ? nt : 1));
+    w->vol    = (double*)malloc(sizeof(double) * (nt ? nt : 1));
+    w->mu     = (double*)malloc(sizeof(double) * (nt ? nt : 1));
+    w->lam    = (double*)malloc(sizeof(double) * (nt ? nt : 1));
+    orc_fem_precompute(nv, w->xr, nt, w->tets, s->tet_E, s->tet_nu, s->tet_rho, w->dminv,
+                       w->vol, w->mu, w->lam, w->mass);
+    double msum = 0.0;
+    for(int v = 0; v < nv; v++)
+        msum += w->mass[v];
+    w->mean_mass = msum / nv;
+    w->dhat2     = s->dhat * s->dhat;
+    /* bboxDiagSize2 from the face-BVH scene box (GIPC.cu:8536-8540) */
+    double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
+    for(int f = 0; f < s->n_faces; f++)
+        for(int k = 0; k < 3; k++)
+            for(int c = 0; c < 3; c++)
+            {
+                double p = w->x[3 * w->faces[3 * f + k] + c];
+                lo[c]    = p < lo[c] ? p : lo[c];
+                up[c]    = up[c] < p ? p : up[c];
+            }
+    w->bbox_diag2 = (up[0] - lo[0]) * (up[0] - lo[0]) + (up[1] - lo[1]) * (up[1] - lo[1])
+                    + (up[2] - lo[2]) * (up[2] - lo[2]);
+    /* scene_diag over non-fixed vertices (main.cu:853-863) */
+    double u2[3] = {-1e32, -1e32, -1e32}, l2[3] = {1e32, 1e32, 1e32};
+    int    any   = 0;
+    for(int v = 0; v < nv; v++)
+        if(bt(w, v) == 0)
+        {
+            any = 1;
+            for(int c = 0; c < 3; c++)
+            {
+                double p = w->x[3 * v + c];
+                l2[c]    = p < l2[c] ? p : l2[c];
+                u2[c]    = u2[c] < p ? p : u2[c];
+            }
+        }
+    w->scene_diag = any ? sqrt((u2[0] - l2[0]) * (u2[0] - l2[0]) + (u2[1] - l2[1]) * (u2[1] - l2[1])
+                               + (u2[2] - l2[2]) * (u2[2] - l2[2]))
+                        : 0.0;
+    w->kappa = s->kappa > 0 ? s->kappa : 0.0;
+    w->cap   = 2 * (s->n_surf_verts + s->n_edges) + 200000;
+    w->pairs = (int32_t*)malloc(sizeof(int32_t) * 4 * w->cap);
+    w->ccd   = (int32_t*)malloc(sizeof(int32_t) * 4 * w->cap);
+    w->gp    = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf_verts + 1));
+    w->gcontact = (double*)calloc(3 * nv, sizeof(double));
+    w->gshape   = (double*)calloc(3 * nv, sizeof(double));
+    /* x_tilde for the first step (main.cu:1037-1045 computeXTilta) */
+    for(int v = 0; v < nv; v++)
+        for(int c = 0; c < 3; c++)
+            w->xt[3 * v + c] = w->xprev[3 * v + c] + s->dt * w->v[3 * v + c]
+                               + (bt(w, v) == 0 ? s->gravity[c] * s->dt * s->dt : 0.0);
+    build_cp(w);
+    return w;
+}
+
+void orc_world_free(orc_world* w)
+{
+    if(!w)
+        return;
+    free(w->x); free(w->xr); free(w->xprev); free(w->v); free(w->xt); free(w->dx);
+    free(w->xsave); free(w->mass); free(w->btype); free(w->body); free(w->part);
+    free(w->tets); free(w->faces); free(w->edges); free(w->surf); free(w->dminv);
+    free(w->vol); free(w->mu); free(w->lam); free(w->pairs); free(w->ccd); free(w->gp);
+    free(w->gcontact); free(w->gshape);
+    free(w);
+}
+
+void orc_world_set_positions(orc_world* w, const double* x)
+{
+    memcpy(w->x, x, sizeof(double) * 3 * w->nv);
+    build_cp(w);
+}
+void orc_world_get_positions(const orc_world* w, double* x)
+{
+    memcpy(x, w->x, sizeof(double) * 3 * w->nv);
+}
+void orc_world_set_xtilde(orc_world* w, const double* xt)
+{
+    memcpy(w->xt, xt, sizeof(double) * 3 * w->nv);
+}
+void orc_world_get_xtilde(const orc_world* w, double* xt)
+{
+    memcpy(xt, w->xt, sizeof(double) * 3 * w->nv);
+}
+double orc_world_get_kappa(const orc_world* w) { return w->kappa; }
+void   orc_world_set_kappa(orc_world* w, double k) { w->kappa = k; }
+
+/* ---------------------------------------------------------------- assembly */
+typedef struct
+{
+    int*    r;
+    int*    c;
+    double* v;
+    int     n, cap;
+} trip_t;
+
+static void trip_push(trip_t* t, int r, int c, const double* blk)
+{
+    if(t->n == t->cap)
+    {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->r   = (int*)realloc(t->r, sizeof(int) * t->cap);
+        t->c   = (int*)realloc(t->c, sizeof(int) * t->cap);
+        t->v   = (double*)realloc(t->v, sizeof(double) * 9 * t->cap);
+    }
+    t->r[t->n] = r;
+    t->c[t->n] = c;
+    memcpy(t->v + 9 * t->n, blk, sizeof(double) * 9);
+    t->n++;
+}
+
+/* upper blocks of an (3m)x(3m) element matrix, transposed when the global
+ * row index exceeds the column (write_triplet, GIPC.cu:83-139) */
+static void push_element(trip_t* t, int m, const int* ids, const double* H, int zero_if_bnd,
+                         const orc_world* w)
+{
+    int n = 3 * m;
+    for(int a = 0; a < m; a++)
+        for(int b = a; b < m; b++)
+        {
+            int    r = ids[a], c = ids[b];
+            double blk[9];
+            if(r <= c)
+            {
+                for(int i = 0; i < 3; i++)
+                    for(int j = 0; j < 3; j++)
+                        blk[i * 3 + j] = H[(3 * a + i) * n + 3 * b + j];
+            }
+            else
+            {
+                int tmp = r;
+                r       = c;
+                c       = tmp;
+                for(int i = 0; i < 3; i++)
+                    for(int j = 0; j < 3; j++)
+                        blk[i * 3 + j] = H[(3 * b + i) * n + 3 * a + j];
+            }
+            if(zero_if_bnd && (bt(w, r) != 0 || bt(w, c) != 0))
+                memset(blk, 0, sizeof(blk));
+            trip_push(t, r, c, blk);
+        }
+}
+
+static void compute_grad_hess(orc_world* w, trip_t* T)
+{
+    int nv = w->nv;
+    memset(w->gcontact, 0, sizeof(double) * 3 * nv);
+    for(int v = 0; v < nv; v++)
+        for(int c = 0; c < 3; c++)
+            w->gshape[3 * v + c] = (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v];
+    /* barrier */
+    for(int i = 0; i < w->npairs; i++)
+    {
+        int32_t ids[4];
+        double  g[12], H[144];
+        int     m = orc_barrier_grad_hess(w->x, w->xr, w->pairs + 4 * i, w->dhat2, w->kappa, ids,
+                                          g, H);
+        for(int a = 0; a < m; a++)
+            for(int c = 0; c < 3; c++)
+                w->gcontact[3 * ids[a] + c] += g[3 * a + c];
+        if(T)
+            push_element(T, m, ids, H, 1, w);
+    }
+    /* ground */
+    for(int i = 0; i < w->ngp; i++)
+    {
+        int    v = (int)w->gp[i];
+        double g[3], H[9];
+        orc_ground_grad_hess(w->x + 3 * v, w->s.ground_normal, w->s.ground_offset, w->dhat2,
+                             w->kappa, g, H);
+        for(int c = 0; c < 3; c++)
+            w->gcontact[3 * v + c] += g[c];
+        if(T)
+        {
+            int ids[1] = {v};
+            push_element(T, 1, ids, H, 1, w);
+        }
+    }
+    /* FEM */
+    for(int e = 0; e < w->nt; e++)
+    {
+        const uint32_t* t = w->tets + 4 * e;
+        double          g[12], H[144];
+        orc_tet_grad_hess(w->x, t, w->dminv + 9 * e, w->vol[e], w->mu[e], w->lam[e], w->s.dt, g,
+                          T ? H : H);
+        for(int a = 0; a < 4; a++)
+            for(int c = 0; c < 3; c++)
+                w->gshape[3 * t[a] + c] += g[3 * a + c];
+        if(T)
+        {
+            int ids[4] = {(int)t[0], (int)t[1], (int)t[2], (int)t[3]};
+            push_element(T, 4, ids, H, 1, w);
+        }
+    }
+    /* mass diagonal (never zeroed) */
+    if(T)
+        for(int v = 0; v < nv; v++)
+        {
+            double blk[9] = {w->mass[v], 0, 0, 0, w->mass[v], 0, 0, 0, w->mass[v]};
+            trip_push(T, v, v, blk);
+        }
+}
+
+int orc_world_assemble(orc_world* w, double* grad, orc_bsr* H)
+{
+    trip_t T = {0};
+    compute_grad_hess(w, H ? &T : NULL);
+    for(int v = 0; v < w->nv; v++)
+        for(int c = 0; c < 3; c++)
+            grad[3 * v + c] =
+                bt(w, v) != 0 ? 0.0 : w->gcontact[3 * v + c] + w->gshape[3 * v + c];
+    int rc = 0;
+    if(H)
+        rc = orc_bsr_from_upper_triplets(w->nv, T.n, T.r, T.c, T.v, H);
+    free(T.r);
+    free(T.c);
+    free(T.v);
+    return rc;
+}
+
+void orc_world_energy(orc_world* w, double* terms)
+{
+    double kin = 0.0, ela = 0.0, bar = 0.0, gnd = 0.0;
+    for(int v = 0; v < w->nv; v++)
+    {
+        double d2 = 0.0;
+        for(int c = 0; c < 3; c++)
+        {
+            double d = w->x[3 * v + c] - w->xt[3 * v + c];
+            d2 += d * d;
+        }
+        kin += d2 * w->mass[v] * 0.5;
+    }
+    for(int e = 0; e < w->nt; e++)
+        ela += orc_tet_energy(w->x, w->tets + 4 * e, w->dminv + 9 * e, w->vol[e], w->mu[e],
+                              w->lam[e]);
+    ela *= w->s.dt * w->s.dt;
+    for(int i = 0; i < w->npairs; i++)
+        bar += orc_barrier_energy(w->x, w->xr, w->pairs + 4 * i, w->dhat2, w->kappa);
+    const double* n = w->s.ground_normal;
+    for(int i = 0; i < w->ngp; i++)
+    {
+        const double* p = w->x + 3 * w->gp[i];
+        double        d = n[0] * p[0] + n[1] * p[1] + n[2] * p[2] - w->s.ground_offset;
+        gnd += orc_ground_energy(d * d, w->dhat2);
+    }
+    gnd *= w->kappa;
+    terms[0] = kin + ela + bar + gnd;
+    terms[1] = kin;
+    terms[2] = ela;
+    terms[3] = bar;
+    terms[4] = gnd;
+    terms[5] = 0.0;
+    terms[6] = 0.0;
+    terms[7] = 0.0;
+}
+
+static double energy(orc_world* w)
+{
+    double t[8];
+    orc_world_energy(w, t);
+    return t[0];
+}
+
+/* ------------------------------------------------------------------- kappa */
+static double compute_H_b(double d, double dHat)
+{
+    double t = d - dHat;
+    return (log(d / dHat) * -2.0 - t * 4.0 / d) + 1.0 / (d * d) * (t * t);
+}
+
+static void suggest_kappa(orc_world* w, double* kappa)
+{
+    double H_b = compute_H_b(1.0e-16 * w->bbox_diag2, w->dhat2);
+    if(w->mean_mass == 0.0)
+        *kappa = 1e11 / (4.0e-16 * w->bbox_diag2 * H_b);
+    else
+        *kappa = 1e11 * w->mean_mass / (4.0e-16 * w->bbox_diag2 * H_b);
+}
+
+static void upper_bound_kappa(orc_world* w, double* kappa)
+{
+    double H_b  = compute_H_b(1.0e-16 * w->bbox_diag2, w->dhat2);
+    double kmax = 100 * 1e11 * w->mean_mass / (4.0e-16 * w->bbox_diag2 * H_b);
+    if(w->mean_mass == 0.0)
+        kmax = 100 * 1e11 / (4.0e-16 * w->bbox_diag2 * H_b);
+    if(*kappa > kmax)
+        *kappa = kmax;
+}
+
+static void init_kappa(orc_world* w)
+{
+    if(w->npairs <= 0)
+        return;
+    int     nv = w->nv;
+    double* GE = (double*)calloc(3 * nv, sizeof(double));
+    double* gc = (double*)calloc(3 * nv, sizeof(double));
+    for(int v = 0; v < nv; v++)
+        for(int c = 0; c < 3; c++)
+            GE[3 * v + c] = (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v];
+    for(int e = 0; e < w->nt; e++)
+    {
+        const uint32_t* t = w->tets + 4 * e;
+        double          g[12], H[144];
+        orc_tet_grad_hess(w->x, t, w->dminv + 9 * e, w->vol[e], w->mu[e], w->lam[e], w->s.dt, g,
+                          H);
+        for(int a = 0; a < 4; a++)
+            for(int c = 0; c < 3; c++)
+                GE[3 * t[a] + c] += g[3 * a + c];
+    }
+    for(int i = 0; i < w->ngp; i++)
+    {
+        int    v = (int)w->gp[i];
+        double g[3], H[9];
+        orc_ground_grad_hess(w->x + 3 * v, w->s.ground_normal, w->s.ground_offset, w->dhat2, 1.0,
+                             g, H);
+        for(int c = 0; c < 3; c++)
+            gc[3 * v + c] += g[c];
+    }
+    for(int i = 0; i < w->npairs; i++)
+    {
+        int32_t ids[4];
+        double  g[12], H[144];
+        int m = orc_barrier_grad_hess(w->x, w->xr, w->pairs + 4 * i, w->dhat2, 1.0, ids, g, H);
+        for(int a = 0; a < m; a++)
+            for(int c = 0; c < 3; c++)
+                gc[3 * ids[a] + c] += g[3 * a + c];
+    }
+    double gsum = 0.0, gsnorm = 0.0;
+    for(int i = 0; i < 3 * nv; i++)
+    {
+        gsum += gc[i] * GE[i];
+        gsnorm += gc[i] * gc[i];
+    }
+    double minKappa = -gsum / gsnorm;
+    if(minKappa > 0.0)
+        w->kappa = minKappa;
+    suggest_kappa(w, &minKappa);
+    if(w->kappa < minKappa)
+        w->kappa = minKappa;
+    upper_bound_kappa(w, &w->kappa);
+    free(GE);
+    free(gc);
+}
+
+/* ------------------------------------------------------------------ Newton */
+static void step_forward(orc_world* w, double alpha)
+{
+    for(int v = 0; v < w->nv; v++)
+        if(bt(w, v) == 0)
+            for(int c = 0; c < 3; c++)
+                w->x[3 * v + c] = w->xsave[3 * v + c] - w->dx[3 * v + c] * alpha;
+}
+
+static void line_search(orc_world* w, double* alpha, double cfl_alpha, int* nls)
+{
+    double E0  = energy(w);
+    double gdx = 0.0;
+    for(int i = 0; i < 3 * w->nv; i++)
+        gdx += w->gcontact[i] * w->dx[i];
+    double c1m = w->s.armijo_c1 * gdx;
+    memcpy(w->xsave, w->x, sizeof(double) * 3 * w->nv);
+    step_forward(w, *alpha);
+    while(is_intersected(w))
+    {
+        *alpha /= 2.0;
+        *alpha = dmin(cfl_alpha, *alpha);
+        step_forward(w, *alpha);
+    }
+    build_cp(w);
+    double E  = energy(w);
+    double LF = *alpha;
+    int    k  = 0;
+    for(int i = 0; i < 30; i++)
+    {
+        if(E <= E0 + c1m * *alpha)
+            break;
+        *alpha *= w->s.armijo_beta;
+        k++;
+        step_forward(w, *alpha);
+        build_cp(w);
+        E = energy(w);
+    }
+    if(*alpha < LF)
+    {
+        int recompute = 0;
+        while(is_intersected(w))
+        {
+            *alpha *= w->s.armijo_beta;
+            double lo = w->s.armijo_alpha_min, hi = cfl_alpha;
+            *alpha = (*alpha < lo) ? lo : (hi < *alpha ? hi : *alpha);
+            step_forward(w, *alpha);
+            recompute = 1;
+        }
+        if(recompute)
+            build_cp(w);
+    }
+    if(nls)
+        *nls = k;
+}
+
+/* raw candidates of the current proximity codes: the reference keeps the
+ * candidate that produced each code (_ccd_collisionPair); since every code
+ * determines its candidate only up to the face/edge that produced it, the
+ * oracle re-enumerates candidates with the distance test applied. */
+static int proximity_candidates(orc_world* w, int32_t* out, int cap)
+{
+    /* reuse the swept query with alpha = 0 and keep only candidates that
+     * produce a proximity code */
+    int    n  = full_ccd_candidates(w, 0.0, out, cap);
+    int    m  = 0;
+    for(int i = 0; i < n && i < cap; i++)
+    {
+        int32_t* c  = out + 4 * i;
+        double   d2 = 1e300;
+        if(c[0] < 0)
+        {
+            int           p = -c[0] - 1;
+            const double* X = w->x;
+            switch(orc_dtype_pt(X + 3 * p, X + 3 * c[1], X + 3 * c[2], X + 3 * c[3]))
+            {
+                case 0: d2 = orc_d_pp(X + 3 * p, X + 3 * c[1]); break;
+                case 1: d2 = orc_d_pp(X + 3 * p, X + 3 * c[2]); break;
+                case 2: d2 = orc_d_pp(X + 3 * p, X + 3 * c[3]); break;
+                case 3: d2 = orc_d_pe(X + 3 * p, X + 3 * c[1], X + 3 * c[2]); break;
+                case 4: d2 = orc_d_pe(X + 3 * p, X + 3 * c[2], X + 3 * c[3]); break;
+                case 5: d2 = orc_d_pe(X + 3 * p, X + 3 * c[3], X + 3 * c[1]); break;
+                case 6: d2 = orc_d_pt(X + 3 * p, X + 3 * c[1], X + 3 * c[2], X + 3 * c[3]); break;
+            }
+        }
+        else
+        {
+            const double* X = w->x;
+            switch(orc_dtype_ee(X + 3 * c[0], X + 3 * c[1], X + 3 * c[2], X + 3 * c[3]))
+            {
+                case 0: d2 = orc_d_pp(X + 3 * c[0], X + 3 * c[2]); break;
+                case 1: d2 = orc_d_pp(X + 3 * c[0], X + 3 * c[3]); break;
+                case 2: d2 = orc_d_pe(X + 3 * c[0], X + 3 * c[2], X + 3 * c[3]); break;
+                case 3: d2 = orc_d_pp(X + 3 * c[1], X + 3 * c[2]); break;
+                case 4: d2 = orc_d_pp(X + 3 * c[1], X + 3 * c[3]); break;
+                case 5: d2 = orc_d_pe(X + 3 * c[1], X + 3 * c[2], X + 3 * c[3]); break;
+                case 6: d2 = orc_d_pe(X + 3 * c[2], X + 3 * c[0], X + 3 * c[1]); break;
+                case 7: d2 = orc_d_pe(X + 3 * c[3], X + 3 * c[0], X + 3 * c[1]); break;
+                case 8: d2 = orc_d_ee(X + 3 * c[0], X + 3 * c[1], X + 3 * c[2], X + 3 * c[3]); break;
+            }
+        }
+        if(d2 < w->dhat2)
+        {
+            memmove(out + 4 * m, c, sizeof(int32_t) * 4);
+            m++;
+        }
+    }
+    return m;
+}
+
+int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_out,
+                               double* xdelta_out, int* converged_out)
+{
+    int      nv   = w->nv;
+    double*  grad = (double*)malloc(sizeof(double) * 3 * nv);
+    orc_bsr  H    = {0};
+    orc_world_assemble(w, grad, &H);
+    double xdelta = 0.0;
+    for(int i = 0; i < 3 * nv; i++)
+        xdelta = dmax(xdelta, fabs(w->dx[i]));
+    if(xdelta_out)
+        *xdelta_out = xdelta;
+    int k = w->newton_k;
+    if(k == 1)
+        w->xdelta0 = xdelta;
+    if(k > 0
+       && (xdelta < w->scene_diag * w->s.abs_xdelta_tol || xdelta < w->s.rel_xdelta_tol * w->xdelta0))
+    {
+        if(converged_out)
+            *converged_out = 1;
+        orc_bsr_free(&H);
+        free(grad);
+        return 0;
+    }
+    if(converged_out)
+        *converged_out = 0;
+    orc_mas* mas = NULL;
+    if(w->s.precond == SGIPC_PRECOND_MAS)
+        mas = orc_mas_create(&H, w->part, w->s.mas_precision == SGIPC_MAS_FP32);
+    double rz0, rz;
+    int    it = orc_pcg(&H, mas, w->s.precond == SGIPC_PRECOND_MAS ? 1 : 0, grad, w->dx,
+                        w->s.pcg_rel_tol, w->s.pcg_abs_tol, w->s.pcg_max_iter,
+                        w->s.pcg_use_preconditioned_norm, 10, &rz0, &rz);
+    if(pcg_iters_out)
+        *pcg_iters_out = it;
+    orc_mas_free(mas);
+    orc_bsr_free(&H);
+    free(grad);
+
+    double alpha = 1.0;
+    if(w->s.use_ground)
+        alpha = dmin(alpha, ground_bound(w, 0.8));
+    int32_t* cand = w->ccd;
+    int      np   = proximity_candidates(w, cand, w->cap);
+    alpha         = dmin(alpha, self_bound(w, 0.8, cand, np));
+    double temp_alpha = alpha, alpha_cfl = alpha;
+    int    nccd = full_ccd_candidates(w, temp_alpha, cand, w->cap);
+    if(nccd > w->cap)
+        nccd = w->cap;
+    if(nccd > 0)
+    {
+        double maxSpeed = cfl_speed(w);
+        alpha_cfl       = sqrt(w->dhat2) / maxSpeed * 0.5;
+        alpha           = dmin(alpha, alpha_cfl);
+        if(temp_alpha > 2 * alpha_cfl)
+        {
+            alpha = dmin(temp_alpha, self_bound(w, 0.8, cand, nccd));
+            alpha = dmax(alpha, alpha_cfl);
+        }
+    }
+    line_search(w, &alpha, alpha_cfl, NULL);
+    if(w->kappa == 0.0)
+        init_kappa(w);
+    w->newton_k++;
+    if(alpha_out)
+        *alpha_out = alpha;
+    return 0;
+}
+
+int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
+{
+    upper_bound_kappa(w, &w->kappa);
+    if(w->kappa < 1e-16)
+        suggest_kappa(w, &w->kappa);
+    init_kappa(w);
+    memset(w->dx, 0, sizeof(double) * 3 * w->nv);
+    w->newton_k = 0;
+    w->xdelta0  = 0.0;
+    int total_pcg = 0, k = 0;
+    int cap = w->s.newton_max_iter > 0 ? w->s.newton_max_iter : 10000;
+    for(k = 0; k < cap; k++)
+    {
+        int it = 0, conv = 0;
+        orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
+        if(conv)
+            break;
+        total_pcg += it;
+    }
+    for(int v = 0; v < w->nv; v++)
+        for(int c = 0; c < 3; c++)
+        {
+            if(bt(w, v) == 0)
+                w->v[3 * v + c] = (w->x[3 * v + c] - w->xprev[3 * v + c]) / w->s.dt;
+            else
+                w->v[3 * v + c] = 0.0;
+            w->xprev[3 * v + c] = w->x[3 * v + c];
+        }
+    for(int v = 0; v < w->nv; v++)
+        for(int c = 0; c < 3; c++)
+            w->xt[3 * v + c] = w->xprev[3 * v + c] + w->s.dt * w->v[3 * v + c]
+                               + (bt(w, v) == 0 ? w->s.gravity[c] * w->s.dt * w->s.dt : 0.0);
+    if(newton_iters_out)
+        *newton_iters_out = k;
+    if(pcg_iters_out)
+        *pcg_iters_out = total_pcg;
+    return 0;
+}

@@ -1,0 +1,318 @@
+"""ctypes bindings of oracle/liboracle.so -- the CPU FP64 checker.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+up = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+
+
+class Bsr(C.Structure):
+    _fields_ = [("nb", C.c_int), ("nnzb", C.c_int), ("rowptr", C.POINTER(C.c_int)),
+                ("cols", C.POINTER(C.c_int)), ("vals", dp)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_sym_eig.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_psd_project.argtypes = [C.c_int, dp]
+        L.orc_inverse.argtypes = [C.c_int, dp]
+        L.orc_snk1_psi.restype = C.c_double
+        L.orc_snk1_psi.argtypes = [dp, C.c_double, C.c_double]
+        L.orc_snk1_pk1.argtypes = [dp, C.c_double, C.c_double, dp]
+        L.orc_snk1_hess.argtypes = [dp, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_fem_precompute.argtypes = [C.c_int, dp, C.c_int, up, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_tet_grad_hess.argtypes = [dp, up, dp, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, dp, dp]
+        L.orc_tet_energy.restype = C.c_double
+        L.orc_tet_energy.argtypes = [dp, up, dp, C.c_double, C.c_double, C.c_double]
+        L.orc_ground_energy.restype = C.c_double
+        L.orc_ground_energy.argtypes = [C.c_double, C.c_double]
+        L.orc_ground_grad_hess.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.orc_expand_bits.restype = C.c_uint32
+        L.orc_expand_bits.argtypes = [C.c_uint32]
+        L.orc_morton_code.restype = C.c_uint32
+        L.orc_morton_code.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_leaf_aabbs.argtypes = [dp, up, C.c_int, C.c_int, dp]
+        L.orc_lbvh_build.argtypes = [dp, up, C.c_int, C.c_int, u64p, up, dp]
+        for f in ("orc_d_pp",):
+            getattr(L, f).restype = C.c_double
+        L.orc_d_pp.argtypes = [dp, dp]
+        L.orc_d_pe.restype = C.c_double
+        L.orc_d_pe.argtypes = [dp, dp, dp]
+        L.orc_d_pt.restype = C.c_double
+        L.orc_d_pt.argtypes = [dp, dp, dp, dp]
+        L.orc_d_ee.restype = C.c_double
+        L.orc_d_ee.argtypes = [dp, dp, dp, dp]
+        L.orc_dtype_pt.argtypes = [dp, dp, dp, dp]
+        L.orc_dtype_ee.argtypes = [dp, dp, dp, dp]
+        L.orc_contact_pairs_bruteforce.argtypes = [C.c_void_p, dp, C.c_double, ip, C.c_int, ip]
+        L.orc_contact_pairs_bvh.argtypes = [C.c_void_p, dp, C.c_double, ip, C.c_int, ip]
+        L.orc_barrier_energy.restype = C.c_double
+        L.orc_barrier_energy.argtypes = [dp, dp, ip, C.c_double, C.c_double]
+        L.orc_barrier_grad_hess.argtypes = [dp, dp, ip, C.c_double, C.c_double, ip, dp, dp]
+        L.orc_barrier_lambda0.restype = C.c_double
+        L.orc_barrier_lambda0.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_bsr_from_upper_triplets.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int),
+                                                  C.POINTER(C.c_int), dp, C.POINTER(Bsr)]
+        L.orc_bsr_free.argtypes = [C.POINTER(Bsr)]
+        L.orc_bsr_spmv.argtypes = [C.POINTER(Bsr), dp, dp]
+        L.orc_mas_create.restype = C.c_void_p
+        L.orc_mas_create.argtypes = [C.POINTER(Bsr), ip, C.c_int]
+        L.orc_mas_free.argtypes = [C.c_void_p]
+        L.orc_mas_apply.argtypes = [C.c_void_p, dp, dp]
+        L.orc_mas_levels.argtypes = [C.c_void_p]
+        L.orc_mas_level_sizes.argtypes = [C.c_void_p, ip]
+        L.orc_mas_total_clusters.argtypes = [C.c_void_p]
+        L.orc_mas_chain.argtypes = [C.c_void_p, ip]
+        L.orc_pcg.argtypes = [C.POINTER(Bsr), C.c_void_p, C.c_int, dp, dp, C.c_double,
+                              C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.orc_world_create.restype = C.c_void_p
+        L.orc_world_create.argtypes = [C.c_void_p]
+        L.orc_world_free.argtypes = [C.c_void_p]
+        L.orc_world_set_positions.argtypes = [C.c_void_p, dp]
+        L.orc_world_get_positions.argtypes = [C.c_void_p, dp]
+        L.orc_world_set_xtilde.argtypes = [C.c_void_p, dp]
+        L.orc_world_get_xtilde.argtypes = [C.c_void_p, dp]
+        L.orc_world_get_kappa.restype = C.c_double
+        L.orc_world_get_kappa.argtypes = [C.c_void_p]
+        L.orc_world_set_kappa.argtypes = [C.c_void_p, C.c_double]
+        L.orc_world_assemble.argtypes = [C.c_void_p, dp, C.POINTER(Bsr)]
+        L.orc_world_energy.argtypes = [C.c_void_p, dp]
+        L.orc_world_newton_iteration.argtypes = [C.c_void_p, dp, ip, dp, ip]
+        L.orc_world_time_step.argtypes = [C.c_void_p, ip, ip]
+        _lib = L
+    return _lib
+
+
+def P(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+# ----------------------------------------------------------------- helpers
+
+def snk1(F, mu, lam):
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    Pm = np.zeros(9)
+    H = np.zeros(81)
+    lib().orc_snk1_pk1(P(F), mu, lam, P(Pm))
+    return lib().orc_snk1_psi(P(F), mu, lam), Pm.reshape(3, 3)
+
+
+def snk1_hess(F, mu, lam, project=True):
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    H = np.zeros(81)
+    lib().orc_snk1_hess(P(F), mu, lam, int(project), P(H))
+    return H.reshape(9, 9)
+
+
+def lbvh(x, prims):
+    x = np.ascontiguousarray(x, np.float64)
+    prims = np.ascontiguousarray(prims, np.uint32)
+    n, ar = prims.shape
+    keys = np.zeros(n, np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), np.uint32)
+    aabbs = np.zeros((2 * n - 1, 6), np.float64)
+    lib().orc_lbvh_build(P(x), P(prims, C.c_uint32), ar, n, P(keys, C.c_uint64),
+                         P(nodes, C.c_uint32), P(aabbs))
+    return keys, nodes, aabbs
+
+
+def contact_pairs(scene, x=None, method="bvh", cap=None):
+    d = scene.desc()
+    x = np.ascontiguousarray(scene.x if x is None else x, np.float64)
+    cap = cap or max(1000, 4 * (scene.surf_verts.size + scene.edges.shape[0]))
+    pairs = np.zeros((cap, 4), np.int32)
+    counts = np.zeros(5, np.int32)
+    f = lib().orc_contact_pairs_bvh if method == "bvh" else lib().orc_contact_pairs_bruteforce
+    n = f(C.byref(d), P(x), scene.dhat ** 2, P(pairs, C.c_int32), cap, P(counts, C.c_int32))
+    assert n <= cap, "pair capacity exceeded"
+    return pairs[:n].copy(), counts
+
+
+class OBsr:
+    """Owned full BSR produced by the oracle."""
+
+    def __init__(self, raw: Bsr):
+        self.nb = raw.nb
+        self.nnzb = raw.nnzb
+        self.rowptr = np.ctypeslib.as_array(raw.rowptr, (raw.nb + 1,)).copy()
+        self.cols = np.ctypeslib.as_array(raw.cols, (max(raw.nnzb, 1),))[: raw.nnzb].copy()
+        self.vals = np.ctypeslib.as_array(raw.vals, (max(raw.nnzb, 1) * 9,))[: 9 * raw.nnzb] \
+            .reshape(raw.nnzb, 3, 3).copy()
+        lib().orc_bsr_free(C.byref(raw))
+
+    def raw(self):
+        self._keep = (np.ascontiguousarray(self.rowptr, np.int32),
+                      np.ascontiguousarray(self.cols, np.int32),
+                      np.ascontiguousarray(self.vals, np.float64))
+        r = Bsr()
+        r.nb, r.nnzb = self.nb, self.nnzb
+        r.rowptr = P(self._keep[0], C.c_int)
+        r.cols = P(self._keep[1], C.c_int)
+        r.vals = P(self._keep[2])
+        return r
+
+    def dense(self):
+        n = 3 * self.nb
+        A = np.zeros((n, n))
+        for i in range(self.nb):
+            for k in range(self.rowptr[i], self.rowptr[i + 1]):
+                j = self.cols[k]
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = self.vals[k]
+        return A
+
+    def spmv(self, x):
+        y = np.zeros(3 * self.nb)
+        r = self.raw()
+        lib().orc_bsr_spmv(C.byref(r), P(np.ascontiguousarray(x, np.float64)), P(y))
+        return y
+
+
+def bsr_from_upper_triplets(nb, rows, cols, vals):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    raw = Bsr()
+    rc = lib().orc_bsr_from_upper_triplets(nb, rows.size, P(rows, C.c_int), P(cols, C.c_int),
+                                           P(vals), C.byref(raw))
+    assert rc == 0
+    return OBsr(raw)
+
+
+class Mas:
+    def __init__(self, A: OBsr, part_id=None, fp32=True):
+        self._raw = A.raw()
+        self._part = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
+        self.h = lib().orc_mas_create(C.byref(self._raw),
+                                      None if self._part is None else P(self._part, C.c_int32),
+                                      int(fp32))
+        self.nb = A.nb
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mas_free(self.h)
+            self.h = None
+
+    def apply(self, r):
+        z = np.zeros(3 * self.nb)
+        lib().orc_mas_apply(self.h, P(np.ascontiguousarray(r, np.float64)), P(z))
+        return z
+
+    @property
+    def levels(self):
+        return lib().orc_mas_levels(self.h)
+
+    def level_sizes(self):
+        s = np.zeros(2 * (self.levels + 1), np.int32)
+        lib().orc_mas_level_sizes(self.h, P(s, C.c_int32))
+        return s.reshape(-1, 2)
+
+    def total_clusters(self):
+        return lib().orc_mas_total_clusters(self.h)
+
+    def chain(self):
+        L = self.levels
+        c = np.zeros(self.nb * max(L - 1, 1), np.int32)
+        lib().orc_mas_chain(self.h, P(c, C.c_int32))
+        return c.reshape(self.nb, max(L - 1, 1))
+
+
+def pcg(A: OBsr, b, mas: Mas | None = None, precond=1, rel_tol=1e-4, abs_tol=0.0, max_iter=0,
+        use_rz=1, check_every=10):
+    raw = A.raw()
+    x = np.zeros(3 * A.nb)
+    rz0 = C.c_double()
+    rz = C.c_double()
+    it = lib().orc_pcg(C.byref(raw), mas.h if mas else None, precond,
+                       P(np.ascontiguousarray(b, np.float64)), P(x), rel_tol, abs_tol, max_iter,
+                       use_rz, check_every, C.byref(rz0), C.byref(rz))
+    return x, it, rz0.value, rz.value
+
+
+class World:
+    """Oracle simulation state for one scene."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self._desc = scene.desc()
+        self.h = lib().orc_world_create(C.byref(self._desc))
+        self.nv = scene.n_verts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_world_free(self.h)
+            self.h = None
+
+    def positions(self):
+        x = np.zeros(3 * self.nv)
+        lib().orc_world_get_positions(self.h, P(x))
+        return x.reshape(-1, 3)
+
+    def set_positions(self, x):
+        lib().orc_world_set_positions(self.h, P(np.ascontiguousarray(x, np.float64).ravel()))
+
+    def xtilde(self):
+        x = np.zeros(3 * self.nv)
+        lib().orc_world_get_xtilde(self.h, P(x))
+        return x.reshape(-1, 3)
+
+    def set_xtilde(self, xt):
+        lib().orc_world_set_xtilde(self.h, P(np.ascontiguousarray(xt, np.float64).ravel()))
+
+    @property
+    def kappa(self):
+        return lib().orc_world_get_kappa(self.h)
+
+    @kappa.setter
+    def kappa(self, k):
+        lib().orc_world_set_kappa(self.h, k)
+
+    def assemble(self):
+        g = np.zeros(3 * self.nv)
+        raw = Bsr()
+        lib().orc_world_assemble(self.h, P(g), C.byref(raw))
+        return g, OBsr(raw)
+
+    def energy(self):
+        t = np.zeros(8)
+        lib().orc_world_energy(self.h, P(t))
+        return t
+
+    def newton_iteration(self):
+        a = C.c_double()
+        it = C.c_int32()
+        xd = C.c_double()
+        cv = C.c_int32()
+        lib().orc_world_newton_iteration(self.h, C.byref(a), C.byref(it), C.byref(xd), C.byref(cv))
+        return dict(alpha=a.value, pcg_iterations=it.value, xdelta=xd.value, converged=cv.value)
+
+    def time_step(self):
+        n = C.c_int32()
+        it = C.c_int32()
+        lib().orc_world_time_step(self.h, C.byref(n), C.byref(it))
+        return dict(newton_iterations=n.value, pcg_iterations=it.value)

@@ -1,0 +1,228 @@
+// capi.hip -- extern "C" boundary of libsgipc.so (include/sgipc.h).
+// Every entry point converts C++ exceptions into a status code and a
+// thread-local message.
+#include <chrono>
+#include <cstring>
+
+#include "../../include/sgipc.h"
+#include "linsys.h"
+
+using namespace sgipc;
+
+namespace
+{
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f)
+{
+    try
+    {
+        f();
+        g_err.clear();
+        return SGIPC_OK;
+    }
+    catch(const Error& e)
+    {
+        g_err = e.what();
+        return e.code;
+    }
+    catch(const std::exception& e)
+    {
+        g_err = e.what();
+        return SGIPC_EHIP;
+    }
+    catch(...)
+    {
+        g_err = "unknown error";
+        return SGIPC_EHIP;
+    }
+}
+}  // namespace
+
+struct sgipc_linear_system
+{
+    hipStream_t  s = nullptr;
+    Bsr          A;
+    Mas          mas;
+    Pcg          pcg;
+    int          precond = 1;
+    DBuf<double> b, x, y;
+    ~sgipc_linear_system()
+    {
+        if(s)
+            (void)hipStreamDestroy(s);
+    }
+};
+
+extern "C" {
+
+const char* sgipc_last_error(void) { return g_err.c_str(); }
+
+int sgipc_device_info(char* buf, int32_t n)
+{
+    return guard([&] {
+        int dev = 0;
+        SG_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        SG_HIP(hipGetDeviceProperties(&p, dev));
+        snprintf(buf, n, "%s %s CUs=%d", p.gcnArchName, p.name, p.multiProcessorCount);
+    });
+}
+
+int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr, const int32_t* cols,
+                    const double* vals, const int32_t* part_id, int32_t precond,
+                    int32_t mas_precision, sgipc_ls_t* out)
+{
+    return guard([&] {
+        SG_CHECK(out && rowptr && cols && vals && nb > 0 && nnzb > 0, SGIPC_EINVAL,
+                 "sgipc_ls_create: null argument or empty matrix");
+        SG_CHECK(rowptr[0] == 0 && rowptr[nb] == nnzb, SGIPC_EINVAL,
+                 "sgipc_ls_create: rowptr must start at 0 and end at nnzb");
+        for(int i = 0; i < nb; i++)
+        {
+            SG_CHECK(rowptr[i + 1] >= rowptr[i], SGIPC_EINVAL, "rowptr not monotone");
+            for(int k = rowptr[i]; k < rowptr[i + 1]; k++)
+            {
+                SG_CHECK(cols[k] >= 0 && cols[k] < nb, SGIPC_EINVAL, "column out of range");
+                SG_CHECK(k == rowptr[i] || cols[k] > cols[k - 1], SGIPC_EINVAL,
+                         "columns must be strictly increasing within a row");
+            }
+        }
+        SG_CHECK(precond == SGIPC_PRECOND_DIAG || precond == SGIPC_PRECOND_MAS, SGIPC_EINVAL,
+                 "unknown preconditioner");
+        auto* ls = new sgipc_linear_system();
+        try
+        {
+            SG_HIP(hipStreamCreateWithFlags(&ls->s, hipStreamNonBlocking));
+            std::vector<int> rp(rowptr, rowptr + nb + 1), cl(cols, cols + nnzb);
+            ls->A.set_pattern(nb, rp, cl, ls->s);
+            ls->A.vals.upload(vals, (size_t)9 * nnzb, ls->s);
+            ls->precond = precond;
+            if(precond == SGIPC_PRECOND_MAS)
+            {
+                MasConfig mc;
+                mc.fp32 = (mas_precision == SGIPC_MAS_FP32);
+                ls->mas.init(ls->A, part_id, mc, ls->s);
+                ls->mas.build(SysView{&ls->A, nullptr}, ls->s);
+            }
+            else
+                ls->pcg.build_diag(ls->A, ls->s);
+            SG_HIP(hipStreamSynchronize(ls->s));
+        }
+        catch(...)
+        {
+            delete ls;
+            throw;
+        }
+        *out = ls;
+    });
+}
+
+int sgipc_ls_destroy(sgipc_ls_t ls)
+{
+    return guard([&] { delete ls; });
+}
+
+int sgipc_ls_spmv(sgipc_ls_t ls, const double* x, double* y)
+{
+    return guard([&] {
+        SG_CHECK(ls && x && y, SGIPC_EINVAL, "null argument");
+        size_t n = 3 * (size_t)ls->A.nb;
+        ls->x.upload(x, n, ls->s);
+        ls->y.resize(n);
+        spmv(SysView{&ls->A, nullptr}, ls->x.data(), ls->y.data(), nullptr, ls->s);
+        ls->y.download(y, n, ls->s);
+        SG_HIP(hipStreamSynchronize(ls->s));
+    });
+}
+
+int sgipc_ls_precond_apply(sgipc_ls_t ls, const double* r, double* z)
+{
+    return guard([&] {
+        SG_CHECK(ls && r && z, SGIPC_EINVAL, "null argument");
+        size_t n = 3 * (size_t)ls->A.nb;
+        ls->x.upload(r, n, ls->s);
+        ls->y.resize(n);
+        if(ls->precond == SGIPC_PRECOND_MAS)
+            ls->mas.apply(ls->x.data(), ls->y.data(), ls->s);
+        else
+            ls->pcg.apply_diag(ls->A.nb, ls->x.data(), ls->y.data(), ls->s);
+        ls->y.download(z, n, ls->s);
+        SG_HIP(hipStreamSynchronize(ls->s));
+    });
+}
+
+int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x, const sgipc_pcg_cfg* cfg,
+                 sgipc_pcg_stats* stats)
+{
+    return guard([&] {
+        SG_CHECK(ls && b && x && cfg, SGIPC_EINVAL, "null argument");
+        SG_CHECK(cfg->solver_precision == SGIPC_SOLVE_FP64, SGIPC_EINVAL,
+                 "only the FP64 solver precision is available for stand-alone systems");
+        size_t n = 3 * (size_t)ls->A.nb;
+        ls->b.upload(b, n, ls->s);
+        ls->x.resize(n);
+        PcgConfig pc;
+        pc.rel_tol     = cfg->rel_tol;
+        pc.abs_tol     = cfg->abs_tol;
+        pc.max_iter    = cfg->max_iter;
+        pc.use_rz      = cfg->use_preconditioned_norm != 0;
+        pc.check_every = cfg->check_every;
+        hipEvent_t e0, e1;
+        SG_HIP(hipEventCreate(&e0));
+        SG_HIP(hipEventCreate(&e1));
+        SG_HIP(hipEventRecord(e0, ls->s));
+        PcgStats st = ls->pcg.solve(SysView{&ls->A, nullptr},
+                                    ls->precond == SGIPC_PRECOND_MAS ? Precond::Mas : Precond::Diag,
+                                    &ls->mas, ls->b.data(), ls->x.data(), pc, ls->s);
+        SG_HIP(hipEventRecord(e1, ls->s));
+        ls->x.download(x, n, ls->s);
+        SG_HIP(hipStreamSynchronize(ls->s));
+        float ms = 0;
+        SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if(stats)
+        {
+            stats->iterations       = st.iterations;
+            stats->outer_iterations = 1;
+            stats->converged        = st.converged ? 1 : 0;
+            stats->rel_residual     = st.rel_residual;
+            stats->rz0              = st.rz0;
+            stats->rz               = st.rz;
+            stats->ms               = ms;
+        }
+    });
+}
+
+int sgipc_ls_mas_info(sgipc_ls_t ls, int32_t* levels, int32_t* level_sizes,
+                      int32_t* total_clusters)
+{
+    return guard([&] {
+        SG_CHECK(ls && ls->precond == SGIPC_PRECOND_MAS && ls->mas.ready(), SGIPC_ESTATE,
+                 "no MAS hierarchy on this system");
+        if(levels)
+            *levels = ls->mas.levels();
+        if(total_clusters)
+            *total_clusters = ls->mas.total_clusters();
+        if(level_sizes)
+        {
+            const auto& v = ls->mas.level_sizes();
+            for(size_t i = 0; i < v.size(); i++)
+                level_sizes[i] = v[i];
+        }
+    });
+}
+
+int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain)
+{
+    return guard([&] {
+        SG_CHECK(ls && ls->precond == SGIPC_PRECOND_MAS && ls->mas.ready() && chain,
+                 SGIPC_ESTATE, "no MAS hierarchy on this system");
+        auto v = ls->mas.chain_host(ls->s);
+        std::memcpy(chain, v.data(), v.size() * sizeof(int));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// common.h -- HIP runtime helpers shared by every translation unit of
+// libsgipc.so: error propagation, device buffers, launch shapes and wave64
+// reductions.  Written for gfx950 (CDNA4, 64-lane wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sgipc
+{
+
+// ---------------------------------------------------------------- errors
+// The reference aborts on every CUDA error (cuda_tools.h:172-206); this
+// library throws, and the C ABI converts the exception into a status code
+// plus a thread-local message.
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m)
+        : std::runtime_error(m)
+        , code(c)
+    {
+    }
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define SG_HIP(expr)                                                              \
+    do                                                                            \
+    {                                                                             \
+        hipError_t _e = (expr);                                                   \
+        if(_e != hipSuccess)                                                      \
+            ::sgipc::throw_hip(_e, #expr, __FILE__, __LINE__);                    \
+    } while(0)
+
+#define SG_CHECK(cond, code, msg)                                                 \
+    do                                                                            \
+    {                                                                             \
+        if(!(cond))                                                               \
+            throw ::sgipc::Error((code), std::string(msg));                       \
+    } while(0)
+
+#define SG_LAUNCH_CHECK() SG_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- buffers
+template <typename T>
+class DBuf
+{
+  public:
+    DBuf() = default;
+    explicit DBuf(size_t n) { resize(n); }
+    ~DBuf() { release(); }
+    DBuf(const DBuf&)            = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept
+        : p_(o.p_)
+        , n_(o.n_)
+        , cap_(o.cap_)
+    {
+        o.p_   = nullptr;
+        o.n_   = 0;
+        o.cap_ = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if(this != &o)
+        {
+            release();
+            p_     = o.p_;
+            n_     = o.n_;
+            cap_   = o.cap_;
+            o.p_   = nullptr;
+            o.n_   = 0;
+            o.cap_ = 0;
+        }
+        return *this;
+    }
+    // grow-only reallocation; contents are not preserved
+    void resize(size_t n)
+    {
+        if(n > cap_)
+        {
+            release();
+            size_t bytes = (n ? n : 1) * sizeof(T);
+            SG_HIP(hipMalloc(&p_, bytes));
+            cap_ = n ? n : 1;
+        }
+        n_ = n;
+    }
+    void release()
+    {
+        if(p_)
+            (void)hipFree(p_);
+        p_   = nullptr;
+        n_   = 0;
+        cap_ = 0;
+    }
+    void zero(hipStream_t s = 0)
+    {
+        if(n_)
+            SG_HIP(hipMemsetAsync(p_, 0, n_ * sizeof(T), s));
+    }
+    void upload(const T* h, size_t n, hipStream_t s = 0)
+    {
+        resize(n);
+        if(n)
+            SG_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s = 0) { upload(h.data(), h.size(), s); }
+    void download(T* h, size_t n, hipStream_t s = 0) const
+    {
+        if(n)
+            SG_HIP(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    std::vector<T> to_host(hipStream_t s = 0) const
+    {
+        std::vector<T> h(n_);
+        download(h.data(), n_, s);
+        SG_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+    T*       data() { return p_; }
+    const T* data() const { return p_; }
+    size_t   size() const { return n_; }
+
+  private:
+    T*     p_   = nullptr;
+    size_t n_   = 0;
+    size_t cap_ = 0;
+};
+
+// ---------------------------------------------------------------- launch
+constexpr int kWave = 64;
+
+inline unsigned grid_for(long long n, int block)
+{
+    long long g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------- wave ops
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+    {
+        double u = __shfl_xor(v, o, 64);
+        v        = (v < u) ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+    {
+        double u = __shfl_xor(v, o, 64);
+        v        = (u < v) ? u : v;
+    }
+    return v;
+}
+
+// positive doubles order like their bit patterns: atomic max via u64 max
+__device__ __forceinline__ void atomic_max_pos(double* addr, double v)
+{
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// wave-aggregated append: one atomic per wave, returns this lane's slot
+// (only lanes with pred != 0 receive a meaningful slot)
+__device__ __forceinline__ unsigned wave_append(unsigned* counter, bool pred)
+{
+    unsigned long long m    = __ballot(pred);
+    unsigned           lane = __lane_id();
+    unsigned           base = 0;
+    int                leader = __ffsll((long long)m) - 1;
+    if(m && (int)lane == leader)
+        base = atomicAdd(counter, (unsigned)__popcll(m));
+    base = __shfl(base, leader < 0 ? 0 : leader, 64);
+    unsigned long long lt = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    return base + (unsigned)__popcll(lt);
+}
+
+}  // namespace sgipc

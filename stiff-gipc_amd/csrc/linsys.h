@@ -1,0 +1,166 @@
+// linsys.h -- device linear system: full-storage BSR (CSR of 3x3 blocks) plus a
+// dynamic COO tail for contact blocks outside the static pattern, the MAS
+// preconditioner and PCG.
+//
+// Counterparts in the reference:
+//   GIPCTripletMatrix + Converter (linear_system/linear_system/global_matrix.h,
+//     linear_system/utils/converter.cu) -> static pattern built once from the
+//     mesh, contact blocks appended to a COO tail (no per-iteration sort);
+//   Spmv::warp_reduce_sym_spmv (linear_system/utils/spmv.cu:8-133) -> row-parallel
+//     full-storage SpMV (no atomics) + COO atomics;
+//   PCGSolver (linear_system/solver/pcg_solver.cu:155-304) -> Pcg below, with
+//     device-resident scalars and one host sync per convergence test;
+//   MASPreconditioner (MASPreconditioner.cu) -> Mas below.
+#pragma once
+
+#include "common.h"
+
+namespace sgipc
+{
+
+// --------------------------------------------------------------- matrices
+struct Bsr
+{
+    int          nb   = 0;  // block rows (= vertices)
+    int          nnzb = 0;  // stored blocks, both triangles
+    DBuf<int>    rowptr;    // nb + 1
+    DBuf<int>    cols;      // nnzb, sorted within each row
+    DBuf<int>    rows;      // nnzb, block row of each entry
+    DBuf<int>    diag;      // nb, index of block (i, i)
+    DBuf<double> vals;      // 9 * nnzb, row-major blocks
+    std::vector<int> h_rowptr, h_cols;  // host copy of the pattern
+
+    // build the pattern from host CSR (cols sorted, every row has its diagonal)
+    void set_pattern(int nb, const std::vector<int>& rowptr, const std::vector<int>& cols,
+                     hipStream_t s);
+};
+
+struct Coo
+{
+    int            cap = 0;
+    DBuf<int>      r, c;     // cap
+    DBuf<double>   v;        // 9 * cap
+    DBuf<unsigned> count;    // 1 (device)
+    void reserve(int n)
+    {
+        cap = n;
+        r.resize(n);
+        c.resize(n);
+        v.resize(9 * (size_t)n);
+        count.resize(1);
+    }
+};
+
+// view of A = static BSR + COO tail
+struct SysView
+{
+    const Bsr* A   = nullptr;
+    const Coo* coo = nullptr;  // may be null
+};
+
+// y = A x (FP64); optional accumulation of dot(x, y) into *dot
+void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t s);
+// r = b - A x
+void residual(const SysView& S, const double* b, const double* x, double* r, hipStream_t s);
+
+// ------------------------------------------------------------------ MAS
+struct MasConfig
+{
+    bool fp32 = true;  // FP32 inverses and multilevel R/Z (reference default)
+};
+
+class Mas
+{
+  public:
+    // static set-up: partition (non-decreasing part ids, <= 16 per part) and
+    // the static adjacency of the pattern of A
+    void init(const Bsr& A, const int* part_id_host, const MasConfig& cfg, hipStream_t s);
+    // per Newton iteration: hierarchy (with COO connectivity), assembly and
+    // block inverses (ReorderRealtime + PrepareHessian_bcoo + __inverse6_P96x96)
+    void build(const SysView& S, hipStream_t s);
+    // z = M^-1 r (standalone)
+    void apply(const double* r, double* z, hipStream_t s);
+
+    // fused PCG pieces (see Pcg): level-0 banks with the optional update
+    // x += a p, r -= a Ap (a = rz[k-1] / pAp[k]); coarse levels; collect
+    void apply_fused(double* x, double* r, const double* p, const double* Ap,
+                     const double* rz_prev, const double* pAp, double* z, double* rz_out,
+                     hipStream_t s);
+
+    int levels() const { return L_; }
+    int total_clusters() const { return total_clusters_; }
+    const std::vector<int>& level_sizes() const { return h_lvl_; }  // (count, off) pairs
+    std::vector<int> chain_host(hipStream_t s) const;
+    bool ready() const { return built_; }
+
+  private:
+    void cluster_level(int level, hipStream_t s);
+    void assemble(const SysView& S, hipStream_t s);
+    void invert(hipStream_t s);
+
+    MasConfig cfg_;
+    int       nv_ = 0, nparts_ = 0, total_map_ = 0, L_ = 0, total_clusters_ = 0;
+    bool      built_ = false;
+    DBuf<int>      slot_of_, vert_of_slot_;
+    DBuf<unsigned> mask0_static_;
+    DBuf<int>      xa_, xb_;   // static cross-bank edges (unordered)
+    int            nx_ = 0;
+    DBuf<unsigned> mask_;      // per-level scratch masks
+    DBuf<unsigned> comp_;      // component masks
+    DBuf<int>      bank_cnt_, bank_prefix_, newid_;
+    DBuf<int>      node_;      // L x nv local node ids (levels >= 1)
+    DBuf<int>      chain_;     // nv x (L-1) global ids
+    DBuf<int>      lvl_;       // 2 * (L + 1): count, offset
+    std::vector<int> h_lvl_;
+    DBuf<double>   B_;         // nbanks x 48 x 48 assembled (FP64)
+    DBuf<float>    inv32_;
+    DBuf<double>   inv64_;
+    DBuf<float>    R32_, Z32_;
+    DBuf<double>   R64_, Z64_;
+    DBuf<int>      going_next_; // global id -> global id one level up
+    DBuf<int>      bank_first_; // global bank -> first next-level global id
+    DBuf<int>      bank_ncl_;   // global bank -> number of next-level nodes
+    size_t         nbanks_alloc_ = 0;
+};
+
+// ------------------------------------------------------------------ PCG
+struct PcgConfig
+{
+    double rel_tol     = 1e-4;
+    double abs_tol     = 0.0;
+    int    max_iter    = 0;
+    bool   use_rz      = true;
+    int    check_every = 10;
+};
+
+struct PcgStats
+{
+    int    iterations = 0;
+    bool   converged  = false;
+    double rz0 = 0, rz = 0;
+    double rel_residual = 0;
+};
+
+enum class Precond
+{
+    Diag = 0,
+    Mas  = 1
+};
+
+class Pcg
+{
+  public:
+    // solve A x = b from x = 0 with the reference's iteration structure
+    PcgStats solve(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
+                   const PcgConfig& cfg, hipStream_t s);
+    // block-Jacobi inverse of the diagonal blocks (DiagPreconditioner)
+    void build_diag(const Bsr& A, hipStream_t s);
+    void apply_diag(int nb, const double* r, double* z, hipStream_t s);
+
+  private:
+    void ensure(int n, int maxit);
+    DBuf<double> r_, z_, p_, Ap_, sc_, dinv_, tmp_;
+    int          n_ = 0, maxit_ = 0;
+};
+
+}  // namespace sgipc

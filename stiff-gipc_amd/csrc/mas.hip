@@ -1,0 +1,748 @@
+// mas.hip -- multilevel additive Schwarz (MAS) preconditioner on gfx950.
+//
+// Semantics follow StiffGIPC/MASPreconditioner.cu and main.cu:47-75:
+//   * bank p holds the vertices of part p in slots 16p .. 16p+15 (run order);
+//   * levelnum = computeNumLevels(nv) (MASPreconditioner.cu:1720-1739);
+//   * the nodes of level l+1 are the connected components of the graph induced
+//     on each 16-node bank of level l (mesh adjacency plus contact cliques),
+//     numbered bank by bank in order of each component's lowest lane
+//     (_buildCML0_new :63, _preparePrefixSumL0_new :157, _buildLevel1_new :271,
+//     _buildConnectMaskLx_new :401, _nextLevelCluster :480, _prefixSumLx :530);
+//   * goingNext[g] = global id of node g one level up (_buildLevel1_new,
+//     _prefixSumLx), chain[v] = global ids of vertex v at levels 1..L-1
+//     (_aggregationKernel :592);
+//   * bank matrices = (P_l^T A P_l) restricted to each bank, zero diagonals set
+//     to 1, inverted (PrepareHessian_bcoo :1810-2084, __inverse6_P96x96 :628);
+//   * z = sum_l P_l Binv_l P_l^T r (__buildMultiLevelR_optimized_new :727,
+//     _schwarzLocalXSym6 :954, __collectFinalZ_new :848).
+//
+// MI355X design: a 16-node bank is a quarter of a wave64; closures run on a
+// bank's 16 lanes through LDS, counts come from one 64-bit ballot.  Coarse
+// Galerkin sums use wave-aggregated atomics (one atomic per distinct target
+// per wave).  Each bank matrix is inverted by one wave in LDS (lane = column).
+// The apply is level-synchronous and atomic-free: every node of level l+1 lies
+// inside a single bank of level l, so restriction is a bank-local LDS
+// reduction done by the same wave that applies that bank's inverse.
+#include "linsys.h"
+
+namespace sgipc
+{
+
+constexpr int BANK = 16;
+constexpr int BD   = 48;  // 3 * BANK
+
+static int compute_num_levels(int nv)
+{
+    int nLevel  = 1;
+    int levelSz = (nv + BANK - 1) / BANK * BANK;
+    while(levelSz > BANK)
+    {
+        levelSz /= BANK;
+        nLevel++;
+        levelSz = (levelSz + BANK - 1) / BANK * BANK;
+    }
+    nLevel = nLevel + 1;
+    return nLevel > 6 ? 6 : nLevel;
+}
+
+// ------------------------------------------------------------ hierarchy
+__global__ void k_l0_mask_coo(const unsigned* __restrict__ count, const int* __restrict__ cr,
+                              const int* __restrict__ cc, const int* __restrict__ slot_of,
+                              unsigned* mask)
+{
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= *count)
+        return;
+    int a = slot_of[cr[t]], b = slot_of[cc[t]];
+    if(a != b && (a >> 4) == (b >> 4))
+        atomicOr(mask + a, 1u << (b & 15));
+}
+
+// level l >= 1: mask bits from static cross-bank edges and COO entries
+__global__ void k_lx_mask(int nx, const int* __restrict__ xa, const int* __restrict__ xb,
+                          const unsigned* __restrict__ coo_count, const int* __restrict__ cr,
+                          const int* __restrict__ cc, const int* __restrict__ node_l,
+                          unsigned* mask)
+{
+    int t     = blockIdx.x * blockDim.x + threadIdx.x;
+    int ncoo  = coo_count ? (int)*coo_count : 0;
+    int u, v;
+    if(t < nx)
+    {
+        u = xa[t];
+        v = xb[t];
+    }
+    else if(t < nx + ncoo)
+    {
+        u = cr[t - nx];
+        v = cc[t - nx];
+    }
+    else
+        return;
+    int a = node_l[u], b = node_l[v];
+    if(a != b && (a >> 4) == (b >> 4))
+    {
+        atomicOr(mask + a, 1u << (b & 15));
+        atomicOr(mask + b, 1u << (a & 15));
+    }
+}
+
+__device__ __forceinline__ int level_count(const int* lvl, int level, int total_map)
+{
+    return level == 0 ? total_map : lvl[2 * level];
+}
+
+// closure of the 16-lane bank masks; per-bank cluster counts
+__global__ void __launch_bounds__(256) k_cluster(int level, int total_map,
+                                                 const int* __restrict__ lvl,
+                                                 const int* __restrict__ vert_of_slot,
+                                                 const unsigned* __restrict__ mask,
+                                                 unsigned* __restrict__ comp,
+                                                 int* __restrict__ bank_cnt)
+{
+    __shared__ unsigned sm[256];
+    int  t      = blockIdx.x * blockDim.x + threadIdx.x;
+    int  lane16 = t & 15;
+    int  n      = level_count(lvl, level, total_map);
+    int  nbanks = (n + 15) >> 4;
+    bool inb    = (t >> 4) < nbanks;
+    bool valid  = inb && (level == 0 ? vert_of_slot[t] >= 0 : t < n);
+    unsigned m  = valid ? (mask[t] | (1u << lane16)) : 0u;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    unsigned c = m, visited = valid ? (1u << lane16) : 0u;
+    int      gb = threadIdx.x & ~15;
+    while(true)
+    {
+        unsigned todo = c & ~visited;
+        if(!todo)
+            break;
+        int nx = __ffs(todo) - 1;
+        visited |= 1u << nx;
+        c |= sm[gb + nx];
+    }
+    bool               elected = valid && ((c & ((1u << lane16) - 1u)) == 0u);
+    unsigned long long bal     = __ballot(elected);
+    unsigned           grp     = (unsigned)((bal >> (threadIdx.x & 48)) & 0xFFFFull);
+    if(inb)
+    {
+        comp[t] = c;
+        if(lane16 == 0)
+            bank_cnt[t >> 4] = __popc(grp);
+    }
+}
+
+// exclusive scan of bank counts (one workgroup); writes level l+1 size/offset
+__global__ void __launch_bounds__(1024) k_scan_banks(int level, int total_map, int* lvl,
+                                                     const int* __restrict__ bank_cnt,
+                                                     int* __restrict__ bank_prefix)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    int n      = level_count(lvl, level, total_map);
+    int nbanks = (n + 15) >> 4;
+    if(threadIdx.x == 0)
+        carry_s = 0;
+    __syncthreads();
+    for(int base = 0; base < nbanks; base += 1024)
+    {
+        int i = base + threadIdx.x;
+        int v = (i < nbanks) ? bank_cnt[i] : 0;
+        // inclusive wave scan
+        int s = v;
+        for(int o = 1; o < 64; o <<= 1)
+        {
+            int u = __shfl_up(s, o, 64);
+            if((int)__lane_id() >= o)
+                s += u;
+        }
+        int w = threadIdx.x >> 6;
+        if(__lane_id() == 63)
+            wsum[w] = s;
+        __syncthreads();
+        if(threadIdx.x < 16)
+        {
+            int ws = wsum[threadIdx.x];
+            for(int o = 1; o < 16; o <<= 1)
+            {
+                int u = __shfl_up(ws, o, 16);
+                if((int)threadIdx.x >= o)
+                    ws += u;
+            }
+            wsum[threadIdx.x] = ws;
+        }
+        __syncthreads();
+        int carry = carry_s;
+        int excl  = s - v + (w > 0 ? wsum[w - 1] : 0) + carry;
+        if(i < nbanks)
+            bank_prefix[i] = excl;
+        __syncthreads();
+        if(threadIdx.x == 1023)
+            carry_s = excl + v;
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        int total            = carry_s;
+        int off              = lvl[2 * level + 1];
+        lvl[2 * (level + 1)]     = total;
+        lvl[2 * (level + 1) + 1] = off + ((n + 15) / 16) * 16;
+    }
+}
+
+// new (level l+1) local ids, goingNext and per-bank first/count
+__global__ void __launch_bounds__(256) k_assign(int level, int total_map, const int* __restrict__ lvl,
+                                                const int* __restrict__ vert_of_slot,
+                                                const unsigned* __restrict__ comp,
+                                                const int* __restrict__ bank_prefix,
+                                                const int* __restrict__ bank_cnt,
+                                                int* __restrict__ newid, int* __restrict__ going_next,
+                                                int* __restrict__ bank_first, int* __restrict__ bank_ncl)
+{
+    int  t      = blockIdx.x * blockDim.x + threadIdx.x;
+    int  lane16 = t & 15;
+    int  n      = level_count(lvl, level, total_map);
+    int  nbanks = (n + 15) >> 4;
+    bool inb    = (t >> 4) < nbanks;
+    bool valid  = inb && (level == 0 ? vert_of_slot[t] >= 0 : t < n);
+    unsigned c  = inb ? comp[t] : 0u;
+    bool     elected = valid && ((c & ((1u << lane16) - 1u)) == 0u);
+    unsigned long long bal = __ballot(elected);
+    unsigned grp           = (unsigned)((bal >> (threadIdx.x & 48)) & 0xFFFFull);
+    if(!inb)
+        return;
+    int off_l  = lvl[2 * level + 1];
+    int off_n  = lvl[2 * (level + 1) + 1];
+    int bank   = t >> 4;
+    if(lane16 == 0)
+    {
+        bank_first[(off_l >> 4) + bank] = off_n + bank_prefix[bank];
+        bank_ncl[(off_l >> 4) + bank]   = bank_cnt[bank];
+    }
+    if(!valid)
+    {
+        newid[t] = -1;
+        return;
+    }
+    int e    = __ffs(c) - 1;
+    int id   = bank_prefix[bank] + __popc(grp & ((1u << e) - 1u));
+    newid[t] = id;
+    going_next[off_l + t] = off_n + id;
+}
+
+// per fine vertex: node id at level l+1 and its chain entry
+__global__ void k_chain(int level, int L, int nv, const int* __restrict__ lvl,
+                        const int* __restrict__ slot_of, const int* __restrict__ newid,
+                        int* __restrict__ node, int* __restrict__ chain)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if(v >= nv)
+        return;
+    int src = (level == 0) ? slot_of[v] : node[(size_t)level * nv + v];
+    int nid = newid[src];
+    node[(size_t)(level + 1) * nv + v]   = nid;
+    chain[(size_t)v * (L - 1) + level] = lvl[2 * (level + 1) + 1] + nid;
+}
+
+// ------------------------------------------------------------- assembly
+// Every stored block (i, j) of A (both triangles, plus the COO tail) adds
+// into the bank matrix of each level at which i and j share a bank.
+__global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restrict__ rows,
+                                                    const int* __restrict__ cols,
+                                                    const double* __restrict__ vals,
+                                                    const unsigned* __restrict__ coo_count,
+                                                    const int* __restrict__ cr,
+                                                    const int* __restrict__ cc,
+                                                    const double* __restrict__ cv, int L, int nv,
+                                                    const int* __restrict__ slot_of,
+                                                    const int* __restrict__ chain,
+                                                    double* __restrict__ B)
+{
+    int  t    = blockIdx.x * blockDim.x + threadIdx.x;
+    int  ncoo = coo_count ? (int)*coo_count : 0;
+    int  i = 0, j = 0;
+    const double* blk = nullptr;
+    bool live = true;
+    if(t < nnz)
+    {
+        i   = rows[t];
+        j   = cols[t];
+        blk = vals + 9 * (size_t)t;
+    }
+    else if(t < nnz + ncoo)
+    {
+        i   = cr[t - nnz];
+        j   = cc[t - nnz];
+        blk = cv + 9 * (size_t)(t - nnz);
+    }
+    else
+        live = false;
+    double b[9];
+#pragma unroll
+    for(int e = 0; e < 9; e++)
+        b[e] = live ? blk[e] : 0.0;
+    if(live)
+    {
+        int si = slot_of[i], sj = slot_of[j];
+        if((si >> 4) == (sj >> 4))
+        {
+            double* M = B + (size_t)(si >> 4) * BD * BD + (3 * (si & 15)) * BD + 3 * (sj & 15);
+#pragma unroll
+            for(int a = 0; a < 3; a++)
+#pragma unroll
+                for(int c = 0; c < 3; c++)
+                    atomicAdd(M + a * BD + c, b[3 * a + c]);
+        }
+    }
+    for(int l = 1; l < L; l++)
+    {
+        long long key = -1;
+        int       gi = 0, gj = 0;
+        if(live)
+        {
+            gi = chain[(size_t)i * (L - 1) + l - 1];
+            gj = chain[(size_t)j * (L - 1) + l - 1];
+            if((gi >> 4) == (gj >> 4))
+                key = ((long long)gi << 32) | (unsigned)gj;
+        }
+        unsigned long long pending = __ballot(key != -1);
+        while(pending)
+        {
+            int       leader = __ffsll((long long)pending) - 1;
+            long long lk     = __shfl(key, leader, 64);
+            bool      mine   = (key == lk);
+            unsigned long long mm = __ballot(mine);
+            int       lgi = (int)(lk >> 32), lgj = (int)(lk & 0xffffffffll);
+            double*   M   = B + (size_t)(lgi >> 4) * BD * BD + (3 * (lgi & 15)) * BD + 3 * (lgj & 15);
+#pragma unroll
+            for(int e = 0; e < 9; e++)
+            {
+                double s = wave_sum(mine ? b[e] : 0.0);
+                if((int)__lane_id() == leader)
+                    atomicAdd(M + (e / 3) * BD + (e % 3), s);
+            }
+            pending &= ~mm;
+        }
+    }
+}
+
+// One wave per bank: load 48x48, zero diagonal -> 1, in-place Gauss-Jordan
+// (no pivoting: bank matrices are SPD), store the transpose so the apply's
+// per-lane dot products read contiguous rows.
+template <typename T>
+__global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B, T* __restrict__ inv)
+{
+    __shared__ double M[BD][BD + 1];
+    __shared__ double colk[BD];
+    const int     bank = blockIdx.x;
+    const int     c    = threadIdx.x;
+    const double* src  = B + (size_t)bank * BD * BD;
+    if(c < BD)
+        for(int r = 0; r < BD; r++)
+        {
+            double v = src[r * BD + c];
+            if(r == c && v == 0.0)
+                v = 1.0;
+            M[r][c] = v;
+        }
+    __syncthreads();
+    for(int k = 0; k < BD; k++)
+    {
+        if(c < BD)
+            colk[c] = M[c][k];
+        __syncthreads();
+        double piv = colk[k];
+        double ip  = 1.0 / piv;
+        if(c < BD)
+        {
+            if(c == k)
+            {
+                for(int r = 0; r < BD; r++)
+                    M[r][k] = (r == k) ? ip : -colk[r] * ip;
+            }
+            else
+            {
+                double a = M[k][c] * ip;
+                for(int r = 0; r < BD; r++)
+                    if(r != k)
+                        M[r][c] -= colk[r] * a;
+                M[k][c] = a;
+            }
+        }
+        __syncthreads();
+    }
+    T* dst = inv + (size_t)bank * BD * BD;
+    if(c < BD)
+        for(int r = 0; r < BD; r++)
+            dst[r * BD + c] = (T)M[c][r];  // transposed
+}
+
+// ---------------------------------------------------------------- apply
+// Level-0 banks (one wave each): optional PCG update of x and r, z0 = Binv r,
+// and the level-1 cluster sums of r.
+template <typename T>
+__global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_slot,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               const double* __restrict__ p,
+                                               const double* __restrict__ Ap,
+                                               const double* __restrict__ rz_prev,
+                                               const double* __restrict__ pAp,
+                                               const T* __restrict__ inv,
+                                               const int* __restrict__ going_next,
+                                               const int* __restrict__ bank_first,
+                                               const int* __restrict__ bank_ncl, T* __restrict__ R,
+                                               T* __restrict__ Z, int has_coarse)
+{
+    __shared__ T Rb[BD];
+    __shared__ T Rc[BD];
+    const int b  = blockIdx.x;
+    const int t  = threadIdx.x;
+    const int s  = b * BANK + t / 3;
+    const int cc = t % 3;
+    int       v  = (t < BD) ? vert_of_slot[s] : -1;
+    double    rv = 0.0;
+    if(v >= 0)
+    {
+        size_t d = 3 * (size_t)v + cc;
+        rv       = r[d];
+        if(x)
+        {
+            double a = *rz_prev / *pAp;
+            x[d] += a * p[d];
+            rv -= a * Ap[d];
+            r[d] = rv;
+        }
+    }
+    if(t < BD)
+    {
+        Rb[t] = (T)rv;
+        Rc[t] = (T)0;
+    }
+    __syncthreads();
+    if(t < BD)
+    {
+        const T* P   = inv + (size_t)b * BD * BD;
+        T        acc = 0;
+        for(int u = 0; u < BD; u++)
+            acc += P[u * BD + t] * Rb[u];
+        Z[(size_t)b * BD + t] = acc;
+        if(has_coarse && v >= 0)
+        {
+            int k = going_next[s] - bank_first[b];
+            atomicAdd(&Rc[3 * k + cc], Rb[t]);
+        }
+    }
+    __syncthreads();
+    if(has_coarse && t < 3 * bank_ncl[b])
+        R[3 * (size_t)bank_first[b] + t] = Rc[t];
+}
+
+// Banks of one coarse level: z = Binv R, and the next level's sums.
+template <typename T>
+__global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
+                                               const T* __restrict__ inv,
+                                               const int* __restrict__ going_next,
+                                               const int* __restrict__ bank_first,
+                                               const int* __restrict__ bank_ncl, T* __restrict__ R,
+                                               T* __restrict__ Z, int has_next)
+{
+    __shared__ T Rb[BD];
+    __shared__ T Rc[BD];
+    const int gb = bank0 + blockIdx.x;
+    const int t  = threadIdx.x;
+    const int g  = gb * BANK + t / 3;  // global node id
+    const int cc = t % 3;
+    bool      valid = (t < BD) && (g - node0 < count);
+    if(t < BD)
+    {
+        Rb[t] = valid ? R[3 * (size_t)g + cc] : (T)0;
+        Rc[t] = (T)0;
+    }
+    __syncthreads();
+    if(t < BD)
+    {
+        const T* P   = inv + (size_t)gb * BD * BD;
+        T        acc = 0;
+        for(int u = 0; u < BD; u++)
+            acc += P[u * BD + t] * Rb[u];
+        Z[(size_t)gb * BD + t] = acc;
+        if(has_next && valid)
+        {
+            int k = going_next[g] - bank_first[gb];
+            atomicAdd(&Rc[3 * k + cc], Rb[t]);
+        }
+    }
+    __syncthreads();
+    if(has_next && t < 3 * bank_ncl[gb])
+        R[3 * (size_t)bank_first[gb] + t] = Rc[t];
+}
+
+// z_v = Z[slot(v)] + sum_l Z[chain_l(v)];  rz += r . z
+template <typename T>
+__global__ void k_mas_collect(int nv, int L, const int* __restrict__ slot_of,
+                              const int* __restrict__ chain, const T* __restrict__ Z,
+                              const double* __restrict__ r, double* __restrict__ z,
+                              double* rz_out)
+{
+    int    v = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if(v < nv)
+    {
+        int s  = slot_of[v];
+        T   z0 = Z[3 * (size_t)s], z1 = Z[3 * (size_t)s + 1], z2 = Z[3 * (size_t)s + 2];
+        for(int l = 1; l < L; l++)
+        {
+            size_t g = chain[(size_t)v * (L - 1) + l - 1];
+            z0 += Z[3 * g];
+            z1 += Z[3 * g + 1];
+            z2 += Z[3 * g + 2];
+        }
+        z[3 * (size_t)v]     = (double)z0;
+        z[3 * (size_t)v + 1] = (double)z1;
+        z[3 * (size_t)v + 2] = (double)z2;
+        if(rz_out)
+            d = r[3 * (size_t)v] * (double)z0 + r[3 * (size_t)v + 1] * (double)z1
+                + r[3 * (size_t)v + 2] * (double)z2;
+    }
+    if(rz_out)
+    {
+        d = wave_sum(d);
+        if(__lane_id() == 0 && d != 0.0)
+            atomicAdd(rz_out, d);
+    }
+}
+
+// ----------------------------------------------------------------- host
+void Mas::init(const Bsr& A, const int* part_id, const MasConfig& cfg, hipStream_t s)
+{
+    cfg_ = cfg;
+    nv_  = A.nb;
+    std::vector<int> slot(nv_);
+    int              part = -1, idx = 0, maxpart = -1, prev = -1;
+    for(int v = 0; v < nv_; v++)
+    {
+        int pv = part_id ? part_id[v] : v / BANK;
+        SG_CHECK(pv >= prev, -1, "MAS: part ids must be non-decreasing in vertex order");
+        prev = pv;
+        if(pv != part)
+        {
+            part = pv;
+            idx  = 0;
+        }
+        SG_CHECK(idx < BANK, -1, "MAS: a part holds more than 16 vertices");
+        slot[v] = BANK * pv + idx;
+        idx++;
+        maxpart = pv > maxpart ? pv : maxpart;
+    }
+    nparts_    = maxpart + 1;
+    total_map_ = BANK * nparts_;
+    L_         = compute_num_levels(nv_);
+    std::vector<int>      vos(total_map_, -1);
+    std::vector<unsigned> m0(total_map_, 0u);
+    std::vector<int>      xa, xb;
+    for(int v = 0; v < nv_; v++)
+        vos[slot[v]] = v;
+    for(int i = 0; i < nv_; i++)
+        for(int k = A.h_rowptr[i]; k < A.h_rowptr[i + 1]; k++)
+        {
+            int j = A.h_cols[k];
+            if(j == i)
+                continue;
+            int a = slot[i], b = slot[j];
+            if((a >> 4) == (b >> 4))
+                m0[a] |= 1u << (b & 15);
+            else if(i < j)
+            {
+                xa.push_back(i);
+                xb.push_back(j);
+            }
+        }
+    nx_ = (int)xa.size();
+    slot_of_.upload(slot, s);
+    vert_of_slot_.upload(vos, s);
+    mask0_static_.upload(m0, s);
+    xa_.upload(xa, s);
+    xb_.upload(xb, s);
+    mask_.resize(total_map_);
+    comp_.resize(total_map_);
+    newid_.resize(total_map_);
+    bank_cnt_.resize(total_map_ / BANK + 1);
+    bank_prefix_.resize(total_map_ / BANK + 1);
+    node_.resize((size_t)L_ * nv_);
+    chain_.resize((size_t)nv_ * (L_ > 1 ? L_ - 1 : 1));
+    lvl_.resize(2 * (L_ + 2));
+    size_t bound = (size_t)L_ * (total_map_ + BANK) + BANK;
+    going_next_.resize(bound);
+    bank_first_.resize(bound / BANK + 1);
+    bank_ncl_.resize(bound / BANK + 1);
+    std::vector<int> lvl0(2 * (L_ + 2), 0);
+    lvl0[0] = nv_;
+    lvl_.upload(lvl0, s);
+    SG_HIP(hipStreamSynchronize(s));
+    built_ = false;
+}
+
+void Mas::cluster_level(int level, hipStream_t s)
+{
+    unsigned g = grid_for(total_map_, 256);
+    hipLaunchKernelGGL(k_cluster, dim3(g), dim3(256), 0, s, level, total_map_, lvl_.data(),
+                       vert_of_slot_.data(), mask_.data(), comp_.data(), bank_cnt_.data());
+    SG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_banks, dim3(1), dim3(1024), 0, s, level, total_map_, lvl_.data(),
+                       bank_cnt_.data(), bank_prefix_.data());
+    SG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_assign, dim3(g), dim3(256), 0, s, level, total_map_, lvl_.data(),
+                       vert_of_slot_.data(), comp_.data(), bank_prefix_.data(), bank_cnt_.data(),
+                       newid_.data(), going_next_.data(), bank_first_.data(), bank_ncl_.data());
+    SG_LAUNCH_CHECK();
+    if(level + 1 <= L_ - 1)
+    {
+        hipLaunchKernelGGL(k_chain, dim3(grid_for(nv_, 256)), dim3(256), 0, s, level, L_, nv_,
+                           lvl_.data(), slot_of_.data(), newid_.data(), node_.data(),
+                           chain_.data());
+        SG_LAUNCH_CHECK();
+    }
+}
+
+void Mas::build(const SysView& S, hipStream_t s)
+{
+    const Coo* coo     = (S.coo && S.coo->cap > 0) ? S.coo : nullptr;
+    int        coo_cap = coo ? coo->cap : 0;
+    // ---- hierarchy (ReorderRealtime)
+    SG_HIP(hipMemcpyAsync(mask_.data(), mask0_static_.data(), total_map_ * sizeof(unsigned),
+                          hipMemcpyDeviceToDevice, s));
+    if(coo)
+    {
+        hipLaunchKernelGGL(k_l0_mask_coo, dim3(grid_for(coo_cap, 256)), dim3(256), 0, s,
+                           coo->count.data(), coo->r.data(), coo->c.data(), slot_of_.data(),
+                           mask_.data());
+        SG_LAUNCH_CHECK();
+    }
+    cluster_level(0, s);
+    for(int l = 1; l < L_; l++)
+    {
+        SG_HIP(hipMemsetAsync(mask_.data(), 0, total_map_ * sizeof(unsigned), s));
+        long long ne = (long long)nx_ + coo_cap;
+        if(ne > 0)
+        {
+            hipLaunchKernelGGL(k_lx_mask, dim3(grid_for(ne, 256)), dim3(256), 0, s, nx_,
+                               xa_.data(), xb_.data(), coo ? coo->count.data() : nullptr,
+                               coo ? coo->r.data() : nullptr, coo ? coo->c.data() : nullptr,
+                               node_.data() + (size_t)l * nv_, mask_.data());
+            SG_LAUNCH_CHECK();
+        }
+        cluster_level(l, s);
+    }
+    h_lvl_.resize(2 * (L_ + 1));
+    SG_HIP(hipMemcpyAsync(h_lvl_.data(), lvl_.data(), h_lvl_.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    total_clusters_ = h_lvl_[2 * L_ + 1];
+    size_t nbanks   = total_clusters_ / BANK;
+    if(nbanks > nbanks_alloc_)
+    {
+        B_.resize(nbanks * BD * BD);
+        if(cfg_.fp32)
+        {
+            inv32_.resize(nbanks * BD * BD);
+            R32_.resize(3 * (size_t)total_clusters_);
+            Z32_.resize(3 * (size_t)total_clusters_);
+        }
+        else
+        {
+            inv64_.resize(nbanks * BD * BD);
+            R64_.resize(3 * (size_t)total_clusters_);
+            Z64_.resize(3 * (size_t)total_clusters_);
+        }
+        nbanks_alloc_ = nbanks;
+    }
+    assemble(S, s);
+    invert(s);
+    built_ = true;
+}
+
+void Mas::assemble(const SysView& S, hipStream_t s)
+{
+    const Bsr& A       = *S.A;
+    const Coo* coo     = (S.coo && S.coo->cap > 0) ? S.coo : nullptr;
+    size_t     nbanks  = total_clusters_ / BANK;
+    SG_HIP(hipMemsetAsync(B_.data(), 0, nbanks * BD * BD * sizeof(double), s));
+    long long n = (long long)A.nnzb + (coo ? coo->cap : 0);
+    hipLaunchKernelGGL(k_mas_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, A.nnzb,
+                       A.rows.data(), A.cols.data(), A.vals.data(),
+                       coo ? coo->count.data() : nullptr, coo ? coo->r.data() : nullptr,
+                       coo ? coo->c.data() : nullptr, coo ? coo->v.data() : nullptr, L_, nv_,
+                       slot_of_.data(), chain_.data(), B_.data());
+    SG_LAUNCH_CHECK();
+}
+
+void Mas::invert(hipStream_t s)
+{
+    unsigned nbanks = total_clusters_ / BANK;
+    if(cfg_.fp32)
+        hipLaunchKernelGGL(k_mas_invert<float>, dim3(nbanks), dim3(64), 0, s, B_.data(),
+                           inv32_.data());
+    else
+        hipLaunchKernelGGL(k_mas_invert<double>, dim3(nbanks), dim3(64), 0, s, B_.data(),
+                           inv64_.data());
+    SG_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void mas_apply_t(int nparts, int L, int nv, const std::vector<int>& lvl,
+                        const int* vert_of_slot, const int* slot_of, const int* chain,
+                        const int* going_next, const int* bank_first, const int* bank_ncl,
+                        const T* inv, T* R, T* Z, double* x, double* r, const double* p,
+                        const double* Ap, const double* rz_prev, const double* pAp, double* z,
+                        double* rz_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mas_l0<T>, dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
+                       rz_prev, pAp, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0);
+    SG_LAUNCH_CHECK();
+    for(int l = 1; l < L; l++)
+    {
+        int count = lvl[2 * l], off = lvl[2 * l + 1];
+        if(count <= 0)
+            continue;
+        hipLaunchKernelGGL(k_mas_lx<T>, dim3((count + BANK - 1) / BANK), dim3(64), 0, s,
+                           off / BANK, off, count, inv, going_next, bank_first, bank_ncl, R, Z,
+                           (l + 1 < L) ? 1 : 0);
+        SG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_mas_collect<T>, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L,
+                       slot_of, chain, Z, r, z, rz_out);
+    SG_LAUNCH_CHECK();
+}
+
+void Mas::apply_fused(double* x, double* r, const double* p, const double* Ap,
+                      const double* rz_prev, const double* pAp, double* z, double* rz_out,
+                      hipStream_t s)
+{
+    SG_CHECK(built_, -5, "MAS apply before build");
+    if(cfg_.fp32)
+        mas_apply_t<float>(nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
+                           chain_.data(), going_next_.data(), bank_first_.data(),
+                           bank_ncl_.data(), inv32_.data(), R32_.data(), Z32_.data(), x, r, p,
+                           Ap, rz_prev, pAp, z, rz_out, s);
+    else
+        mas_apply_t<double>(nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
+                            chain_.data(), going_next_.data(), bank_first_.data(),
+                            bank_ncl_.data(), inv64_.data(), R64_.data(), Z64_.data(), x, r, p,
+                            Ap, rz_prev, pAp, z, rz_out, s);
+}
+
+void Mas::apply(const double* r, double* z, hipStream_t s)
+{
+    apply_fused(nullptr, const_cast<double*>(r), nullptr, nullptr, nullptr, nullptr, z, nullptr,
+                s);
+}
+
+std::vector<int> Mas::chain_host(hipStream_t s) const
+{
+    std::vector<int> h((size_t)nv_ * (L_ - 1));
+    chain_.download(h.data(), h.size(), s);
+    SG_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace sgipc

@@ -1,0 +1,95 @@
+// sim_kernels.h -- plain-data view of the device state used by the element,
+// contact and CCD kernels, plus their host launch wrappers.
+#pragma once
+
+#include "common.h"
+#include "dmath.h"
+
+namespace sgipc
+{
+
+struct SimDev
+{
+    int    nv = 0, nt = 0, ns = 0, nf = 0, ne = 0;
+    double dt = 0, dHat = 0, goff = 0;
+    V3     gn{0, 1, 0}, grav{0, 0, 0};
+    int    use_ground = 0;
+    // state
+    double *x = nullptr, *xt = nullptr, *mass = nullptr, *gshape = nullptr, *gcontact = nullptr;
+    double* vals = nullptr;
+    const double *dminv = nullptr, *vol = nullptr, *mu = nullptr, *lam = nullptr;
+    const int *btype = nullptr, *body = nullptr, *diag = nullptr, *tslot = nullptr;
+    const int *rowptr = nullptr, *cols = nullptr;
+    const uint32_t *tets = nullptr, *surf = nullptr, *faces = nullptr, *edges = nullptr;
+    // ground contacts
+    uint32_t* gp     = nullptr;
+    unsigned* gcount = nullptr;
+    // self contacts (int4 codes) and their raw candidates
+    int4*     pairs    = nullptr;
+    int4*     pairs_raw = nullptr;
+    unsigned* pcount   = nullptr;
+    int       pair_cap = 0;
+    // COO tail
+    int *     coo_r = nullptr, *coo_c = nullptr;
+    double*   coo_v     = nullptr;
+    unsigned* coo_count = nullptr;
+    int       coo_cap   = 0;
+    int*      overflow  = nullptr;  // [0] pairs, [1] coo, [2] ccd
+};
+
+struct EnergyLayout
+{
+    int n[4];
+    int off[4];
+};
+
+// fem.hip
+void launch_init_diag(const SimDev& d, hipStream_t s);
+void launch_fem(const SimDev& d, bool hessian, hipStream_t s);
+void launch_ground_detect(const SimDev& d, hipStream_t s);
+void launch_ground(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);
+void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);
+void launch_rhs(const SimDev& d, double* rhs, hipStream_t s);
+EnergyLayout energy_layout(const SimDev& d);
+void launch_energy(const SimDev& d, double kappa, double* partial, double* terms, hipStream_t s);
+void launch_dot(int n, const double* a, const double* b, double* out, hipStream_t s);
+void launch_max_abs(int n, const double* a, double* out, hipStream_t s);
+void launch_step_forward(const SimDev& d, const double* xs, const double* dx, double alpha,
+                         hipStream_t s);
+void launch_update_velocity(const SimDev& d, double* xprev, double* v, hipStream_t s);
+
+// lbvh.hip: linear BVH over faces (arity 3) or edges (arity 2)
+struct Lbvh
+{
+    int               n = 0, arity = 0;
+    const uint32_t*   prims = nullptr;
+    DBuf<double>      leaf;    // 6 n
+    DBuf<double>      bvs;     // 6 (2n - 1) {upper, lower}
+    DBuf<uint32_t>    nodes;   // 4 (2n - 1) {parent, left, right, element}
+    DBuf<uint64_t>    keys, keys_alt;
+    DBuf<unsigned>    flags;
+    DBuf<unsigned>    hist;
+    DBuf<double>      scene, partial;
+    void init(int n, int arity, const uint32_t* prims);
+    // swept boxes when dx != nullptr: box(x) U box(x - alpha dx)
+    void build(const double* x, const double* dx, double alpha, hipStream_t s);
+};
+
+// contact.hip
+void launch_query_proximity(const SimDev& d, const Lbvh& bf, const Lbvh& be, const double* xrest,
+                            hipStream_t s);
+// swept candidates into cand (count in *ccount, capacity cap)
+void launch_query_ccd(const SimDev& d, const Lbvh& bf, const Lbvh& be, const double* dx,
+                      double alpha, int4* cand, unsigned* ccount, int cap, int* overflow,
+                      hipStream_t s);
+void launch_edge_tri(const SimDev& d, const Lbvh& be, int* hit, hipStream_t s);
+void launch_ground_intersect(const SimDev& d, int* hit, hipStream_t s);
+
+// ccd.hip: reductions write max(1/toc) style values into out (positive max)
+void launch_self_ccd(const double* x, const double* dx, const int4* cand, const unsigned* count,
+                     int cap, double slackness, double* out_max, hipStream_t s);
+void launch_ground_ccd(const SimDev& d, const double* dx, double slackness, double* out_max,
+                       hipStream_t s);
+void launch_cfl(const SimDev& d, const double* dx, double* out_max, hipStream_t s);
+
+}  // namespace sgipc

@@ -1,0 +1,183 @@
+"""ctypes binding of libsgipc.so (include/sgipc.h).
+
+The library is the only compute path: if it cannot be loaded every call
+raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .scene import SceneDesc
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libsgipc.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+up = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+SGIPC_PRECOND_DIAG = 0
+SGIPC_PRECOND_MAS = 1
+SGIPC_MAS_FP32 = 0
+SGIPC_MAS_FP64 = 1
+SGIPC_SOLVE_FP64 = 0
+SGIPC_SOLVE_MIXED = 1
+
+
+class PcgCfg(C.Structure):
+    _fields_ = [("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_iter", C.c_int32),
+                ("use_preconditioned_norm", C.c_int32), ("check_every", C.c_int32),
+                ("solver_precision", C.c_int32)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32),
+                ("converged", C.c_int32), ("pad", C.c_int32), ("rel_residual", C.c_double),
+                ("rz0", C.c_double), ("rz", C.c_double), ("ms", C.c_double)]
+
+
+class IterStats(C.Structure):
+    _fields_ = [("pcg_iterations", C.c_int32), ("n_pairs", C.c_int32), ("n_ground", C.c_int32),
+                ("line_search_steps", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
+                ("pcg_rel_residual", C.c_double), ("alpha", C.c_double),
+                ("alpha_ccd", C.c_double), ("xdelta", C.c_double), ("energy0", C.c_double),
+                ("energy1", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
+                ("ms_ccd", C.c_double), ("ms_line_search", C.c_double),
+                ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class StepStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("pcg_iterations", C.c_int32),
+                ("ms_total", C.c_double)]
+
+
+class SgipcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# every symbol include/sgipc.h declares (tests check the export table)
+EXPORTS = [
+    "sgipc_last_error", "sgipc_device_info",
+    "sgipc_ls_create", "sgipc_ls_destroy", "sgipc_ls_spmv", "sgipc_ls_precond_apply",
+    "sgipc_ls_pcg", "sgipc_ls_mas_info", "sgipc_ls_mas_chain",
+    "sgipc_create", "sgipc_destroy", "sgipc_set_positions", "sgipc_get_positions",
+    "sgipc_get_velocities", "sgipc_set_xtilde", "sgipc_get_xtilde", "sgipc_get_kappa",
+    "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
+    "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SgipcError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.sgipc_last_error.restype = C.c_char_p
+        L.sgipc_device_info.argtypes = [C.c_char_p, C.c_int32]
+        L.sgipc_ls_create.argtypes = [C.c_int32, C.c_int32, ip, ip, dp, ip, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_void_p)]
+        L.sgipc_ls_destroy.argtypes = [C.c_void_p]
+        L.sgipc_ls_spmv.argtypes = [C.c_void_p, dp, dp]
+        L.sgipc_ls_precond_apply.argtypes = [C.c_void_p, dp, dp]
+        L.sgipc_ls_pcg.argtypes = [C.c_void_p, dp, dp, C.POINTER(PcgCfg), C.POINTER(PcgStats)]
+        L.sgipc_ls_mas_info.argtypes = [C.c_void_p, ip, ip, ip]
+        L.sgipc_ls_mas_chain.argtypes = [C.c_void_p, ip]
+        if hasattr(L, "sgipc_create"):
+            L.sgipc_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
+            L.sgipc_destroy.argtypes = [C.c_void_p]
+            for f in ("sgipc_set_positions", "sgipc_get_positions", "sgipc_get_velocities",
+                      "sgipc_set_xtilde", "sgipc_get_xtilde"):
+                getattr(L, f).argtypes = [C.c_void_p, dp]
+            L.sgipc_get_kappa.argtypes = [C.c_void_p, dp]
+            L.sgipc_set_kappa.argtypes = [C.c_void_p, C.c_double]
+            L.sgipc_lbvh_build.argtypes = [C.c_void_p, C.c_int32, u64p, up, dp]
+            L.sgipc_contact_query.argtypes = [C.c_void_p, ip, C.c_int32, ip, up, C.c_int32, ip]
+            L.sgipc_assemble.argtypes = [C.c_void_p, dp, ip, ip, ip, dp, C.c_int32]
+            L.sgipc_energy.argtypes = [C.c_void_p, dp]
+            L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
+            L.sgipc_time_step.argtypes = [C.c_void_p, C.POINTER(StepStats)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise SgipcError(f"sgipc error {rc}: {lib().sgipc_last_error().decode()}")
+
+
+def P(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    check(lib().sgipc_device_info(buf, 256))
+    return buf.value.decode()
+
+
+class LinearSystem:
+    """Stand-alone full symmetric BSR system with MAS or block-Jacobi PCG
+    (counterpart of GlobalLinearSystem::solve_loaded_system)."""
+
+    def __init__(self, rowptr, cols, vals, part_id=None, precond=SGIPC_PRECOND_MAS,
+                 mas_precision=SGIPC_MAS_FP32):
+        self.rowptr = np.ascontiguousarray(rowptr, np.int32)
+        self.cols = np.ascontiguousarray(cols, np.int32)
+        self.vals = np.ascontiguousarray(vals, np.float64).reshape(-1)
+        self.nb = self.rowptr.size - 1
+        self.part = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
+        h = C.c_void_p()
+        check(lib().sgipc_ls_create(self.nb, self.cols.size, P(self.rowptr, C.c_int32),
+                                    P(self.cols, C.c_int32), P(self.vals),
+                                    None if self.part is None else P(self.part, C.c_int32),
+                                    precond, mas_precision, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgipc_ls_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def spmv(self, x):
+        y = np.zeros(3 * self.nb)
+        check(lib().sgipc_ls_spmv(self.h, P(np.ascontiguousarray(x, np.float64)), P(y)))
+        return y
+
+    def precond(self, r):
+        z = np.zeros(3 * self.nb)
+        check(lib().sgipc_ls_precond_apply(self.h, P(np.ascontiguousarray(r, np.float64)), P(z)))
+        return z
+
+    def pcg(self, b, rel_tol=1e-4, abs_tol=0.0, max_iter=0, use_rz=True, check_every=10,
+            solver_precision=SGIPC_SOLVE_FP64):
+        x = np.zeros(3 * self.nb)
+        cfg = PcgCfg(rel_tol, abs_tol, max_iter, int(use_rz), check_every, solver_precision)
+        st = PcgStats()
+        check(lib().sgipc_ls_pcg(self.h, P(np.ascontiguousarray(b, np.float64)), P(x),
+                                 C.byref(cfg), C.byref(st)))
+        return x, st
+
+    def mas_info(self):
+        L = C.c_int32()
+        tc = C.c_int32()
+        sizes = np.zeros(16, np.int32)
+        check(lib().sgipc_ls_mas_info(self.h, C.byref(L), P(sizes, C.c_int32), C.byref(tc)))
+        return L.value, sizes[: 2 * (L.value + 1)].reshape(-1, 2), tc.value
+
+    def mas_chain(self):
+        L = self.mas_info()[0]
+        c = np.zeros(self.nb * (L - 1), np.int32)
+        check(lib().sgipc_ls_mas_chain(self.h, P(c, C.c_int32)))
+        return c.reshape(self.nb, L - 1)

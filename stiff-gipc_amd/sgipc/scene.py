@@ -1,0 +1,318 @@
+"""Synthetic scene construction and the ctypes mirror of ``sgipc_scene_desc``.
+
+The reference loads sorted Gmsh meshes and METIS ``.part`` files
+(StiffGIPC/load_mesh.cpp:449-596, MeshProcess/metis_partition) and gathers
+settings from the scene JSON (StiffGIPC/main.cu:263-356).  This module builds
+the same inputs synthetically: structured hex grids split into 6 tets per
+cell, vertices ordered so that each 2x2x4 vertex block is one 16-vertex MAS
+bank (part ids non-decreasing, as the reference's sorted meshes guarantee),
+plus the collision surface (faces, edges, surface vertices).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+class SceneDesc(C.Structure):
+    """ctypes layout of include/sgipc_scene.h::sgipc_scene_desc."""
+
+    _fields_ = [
+        ("n_verts", C.c_int32),
+        ("x", C.POINTER(C.c_double)),
+        ("x_rest", C.POINTER(C.c_double)),
+        ("v", C.POINTER(C.c_double)),
+        ("btype", C.POINTER(C.c_int32)),
+        ("body_id", C.POINTER(C.c_int32)),
+        ("part_id", C.POINTER(C.c_int32)),
+        ("n_tets", C.c_int32),
+        ("tets", C.POINTER(C.c_uint32)),
+        ("tet_E", C.POINTER(C.c_double)),
+        ("tet_nu", C.POINTER(C.c_double)),
+        ("tet_rho", C.POINTER(C.c_double)),
+        ("n_faces", C.c_int32),
+        ("faces", C.POINTER(C.c_uint32)),
+        ("n_edges", C.c_int32),
+        ("edges", C.POINTER(C.c_uint32)),
+        ("n_surf_verts", C.c_int32),
+        ("surf_verts", C.POINTER(C.c_uint32)),
+        ("dt", C.c_double),
+        ("dhat", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("use_ground", C.c_int32),
+        ("ground_normal", C.c_double * 3),
+        ("ground_offset", C.c_double),
+        ("kappa", C.c_double),
+        ("friction_mu", C.c_double),
+        ("abs_xdelta_tol", C.c_double),
+        ("rel_xdelta_tol", C.c_double),
+        ("armijo_c1", C.c_double),
+        ("armijo_beta", C.c_double),
+        ("armijo_alpha_min", C.c_double),
+        ("newton_max_iter", C.c_int32),
+        ("pcg_rel_tol", C.c_double),
+        ("pcg_abs_tol", C.c_double),
+        ("pcg_max_iter", C.c_int32),
+        ("pcg_use_preconditioned_norm", C.c_int32),
+        ("precond", C.c_int32),
+        ("mas_precision", C.c_int32),
+        ("solver_precision", C.c_int32),
+    ]
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return C.POINTER(ctype)()
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class Scene:
+    x: np.ndarray                    # (N,3) float64
+    tets: np.ndarray                 # (T,4) uint32
+    E: np.ndarray                    # (T,) float64
+    nu: np.ndarray
+    rho: np.ndarray
+    part_id: np.ndarray              # (N,) int32 non-decreasing
+    body_id: np.ndarray              # (N,) int32
+    btype: np.ndarray                # (N,) int32
+    faces: np.ndarray                # (F,3) uint32
+    edges: np.ndarray                # (E,2) uint32
+    surf_verts: np.ndarray           # (S,) uint32
+    v: np.ndarray | None = None
+    x_rest: np.ndarray | None = None
+    dt: float = 0.01
+    dhat: float = 1e-3
+    gravity: tuple = (0.0, -9.8, 0.0)
+    use_ground: bool = True
+    ground_normal: tuple = (0.0, 1.0, 0.0)
+    ground_offset: float = 0.0
+    kappa: float = 0.0
+    friction_mu: float = 0.0
+    abs_xdelta_tol: float = 0.0
+    rel_xdelta_tol: float = 0.0
+    armijo_c1: float = 1e-4
+    armijo_beta: float = 0.5
+    armijo_alpha_min: float = 1e-12
+    newton_max_iter: int = 50
+    pcg_rel_tol: float = 1e-4
+    pcg_abs_tol: float = 0.0
+    pcg_max_iter: int = 0
+    pcg_use_preconditioned_norm: int = 1
+    precond: int = 1
+    mas_precision: int = 0
+    solver_precision: int = 0
+    name: str = "scene"
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_verts(self):
+        return int(self.x.shape[0])
+
+    @property
+    def n_tets(self):
+        return int(self.tets.shape[0])
+
+    def desc(self) -> SceneDesc:
+        """Build a SceneDesc whose pointers reference this object's arrays.
+        Keep the Scene alive while the descriptor is used."""
+        self._keep = [
+            np.ascontiguousarray(self.x, np.float64),
+            None if self.x_rest is None else np.ascontiguousarray(self.x_rest, np.float64),
+            None if self.v is None else np.ascontiguousarray(self.v, np.float64),
+            np.ascontiguousarray(self.btype, np.int32),
+            np.ascontiguousarray(self.body_id, np.int32),
+            np.ascontiguousarray(self.part_id, np.int32),
+            np.ascontiguousarray(self.tets, np.uint32),
+            np.ascontiguousarray(self.E, np.float64),
+            np.ascontiguousarray(self.nu, np.float64),
+            np.ascontiguousarray(self.rho, np.float64),
+            np.ascontiguousarray(self.faces, np.uint32),
+            np.ascontiguousarray(self.edges, np.uint32),
+            np.ascontiguousarray(self.surf_verts, np.uint32),
+        ]
+        k = self._keep
+        d = SceneDesc()
+        d.n_verts = self.n_verts
+        d.x = _ptr(k[0], C.c_double)
+        d.x_rest = _ptr(k[1], C.c_double)
+        d.v = _ptr(k[2], C.c_double)
+        d.btype = _ptr(k[3], C.c_int32)
+        d.body_id = _ptr(k[4], C.c_int32)
+        d.part_id = _ptr(k[5], C.c_int32)
+        d.n_tets = self.n_tets
+        d.tets = _ptr(k[6], C.c_uint32)
+        d.tet_E = _ptr(k[7], C.c_double)
+        d.tet_nu = _ptr(k[8], C.c_double)
+        d.tet_rho = _ptr(k[9], C.c_double)
+        d.n_faces = int(self.faces.shape[0])
+        d.faces = _ptr(k[10], C.c_uint32)
+        d.n_edges = int(self.edges.shape[0])
+        d.edges = _ptr(k[11], C.c_uint32)
+        d.n_surf_verts = int(self.surf_verts.shape[0])
+        d.surf_verts = _ptr(k[12], C.c_uint32)
+        d.dt = self.dt
+        d.dhat = self.dhat
+        d.gravity = (C.c_double * 3)(*self.gravity)
+        d.use_ground = int(self.use_ground)
+        d.ground_normal = (C.c_double * 3)(*self.ground_normal)
+        d.ground_offset = self.ground_offset
+        d.kappa = self.kappa
+        d.friction_mu = self.friction_mu
+        d.abs_xdelta_tol = self.abs_xdelta_tol
+        d.rel_xdelta_tol = self.rel_xdelta_tol
+        d.armijo_c1 = self.armijo_c1
+        d.armijo_beta = self.armijo_beta
+        d.armijo_alpha_min = self.armijo_alpha_min
+        d.newton_max_iter = self.newton_max_iter
+        d.pcg_rel_tol = self.pcg_rel_tol
+        d.pcg_abs_tol = self.pcg_abs_tol
+        d.pcg_max_iter = self.pcg_max_iter
+        d.pcg_use_preconditioned_norm = self.pcg_use_preconditioned_norm
+        d.precond = self.precond
+        d.mas_precision = self.mas_precision
+        d.solver_precision = self.solver_precision
+        return d
+
+
+# --------------------------------------------------------------------------
+# mesh construction
+# --------------------------------------------------------------------------
+
+def hex_grid(nc, h, origin=(0.0, 0.0, 0.0), bank=(2, 2, 4)):
+    """Structured grid of nc = (nx, ny, nz) hex cells of size h, each split
+    into 6 tets around the main diagonal (conforming Kuhn split).
+
+    Vertices are ordered bank by bank: the vertex lattice is cut into
+    bank = (bx, by, bz) blocks (16 vertices each, smaller at the far faces);
+    part_id gives the block index.  Returns (x, tets, part_id)."""
+    nx, ny, nz = nc
+    vx, vy, vz = nx + 1, ny + 1, nz + 1
+    ii, jj, kk = np.meshgrid(np.arange(vx), np.arange(vy), np.arange(vz), indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    bx, by, bz = bank
+    nbx, nby, nbz = -(-vx // bx), -(-vy // by), -(-vz // bz)
+    blk = (ii // bx) * (nby * nbz) + (jj // by) * nbz + (kk // bz)
+    local = (ii % bx) * (by * bz) + (jj % by) * bz + (kk % bz)
+    order = np.lexsort((local, blk))           # vertices sorted by (block, local)
+    lattice_to_new = np.empty_like(order)
+    lattice_to_new[order] = np.arange(order.size)
+    blk_sorted = blk[order]
+    # dense part ids in order of appearance
+    _, part_id = np.unique(blk_sorted, return_inverse=True)
+    x = np.stack([ii, jj, kk], 1)[order].astype(np.float64) * h + np.asarray(origin, np.float64)
+
+    def vid(i, j, k):
+        return lattice_to_new[(i * vy + j) * vz + k]
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    e = np.eye(3, dtype=np.int64)
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for p in perms:
+        a = np.zeros(3, np.int64)
+        b = e[p[0]]
+        c = e[p[0]] + e[p[1]]
+        d = np.ones(3, np.int64)
+        tets.append(np.stack([vid(ci + q[0], cj + q[1], ck + q[2]) for q in (a, b, c, d)], 1))
+    tets = np.concatenate(tets, 0).astype(np.uint32)
+    return x, tets, part_id.astype(np.int32)
+
+
+def surface_of(tets):
+    """Boundary triangles (faces seen once), their unique edges and vertices."""
+    t = np.asarray(tets, np.int64)
+    f = np.concatenate([t[:, [0, 1, 2]], t[:, [0, 1, 3]], t[:, [0, 2, 3]], t[:, [1, 2, 3]]], 0)
+    key = np.sort(f, 1)
+    _, idx, cnt = np.unique(key, axis=0, return_index=True, return_counts=True)
+    faces = f[idx[cnt == 1]]
+    faces = faces[np.lexsort((faces[:, 2], faces[:, 1], faces[:, 0]))]
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]], 0)
+    e = np.unique(np.sort(e, 1), axis=0)
+    sv = np.unique(faces.ravel())
+    return faces.astype(np.uint32), e.astype(np.uint32), sv.astype(np.uint32)
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def assemble_bodies(bodies, **kw):
+    """Concatenate bodies = [(x, tets, part_id, E, nu, rho, btype)] into a Scene
+    with per-body ids and offset part ids."""
+    xs, ts, ps, Es, nus, rhos, bts, bids = [], [], [], [], [], [], [], []
+    voff = 0
+    poff = 0
+    for b, (x, t, p, E, nu, rho, bt) in enumerate(bodies):
+        xs.append(x)
+        ts.append(t.astype(np.int64) + voff)
+        ps.append(p + poff)
+        Es.append(np.full(t.shape[0], E))
+        nus.append(np.full(t.shape[0], nu))
+        rhos.append(np.full(t.shape[0], rho))
+        bts.append(np.full(x.shape[0], bt, np.int32))
+        bids.append(np.full(x.shape[0], b, np.int32))
+        voff += x.shape[0]
+        poff = int(ps[-1].max()) + 1
+    tets = np.concatenate(ts).astype(np.uint32)
+    faces, edges, sv = surface_of(tets)
+    return Scene(
+        x=np.concatenate(xs), tets=tets, E=np.concatenate(Es), nu=np.concatenate(nus),
+        rho=np.concatenate(rhos), part_id=np.concatenate(ps).astype(np.int32),
+        body_id=np.concatenate(bids), btype=np.concatenate(bts), faces=faces, edges=edges,
+        surf_verts=sv, **kw)
+
+
+# --------------------------------------------------------------------------
+# the benchmark scenes (sizes of the metric's configurations)
+# --------------------------------------------------------------------------
+
+def bar_scene(cells=(16, 4, 4), h=0.05, drop=0.1, E=1e5, nu=0.49, rho=1000.0, **kw):
+    """One tet bar of 16x4x4 hex cells (h = 0.05 m, 425 vertices) dropped from
+    0.1 m onto the ground plane; SNK1 E = 1e5, nu = 0.49, rho = 1000."""
+    x, t, p = hex_grid(cells, h, origin=(0.0, drop, 0.0))
+    defaults = dict(dt=0.01, dhat=1e-3, name="bar_16x4x4")
+    defaults.update(kw)
+    sc = assemble_bodies([(x, t, p, E, nu, rho, 0)], **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
+
+
+def cubes_scene(n_side=4, cells=8, size=0.2, spacing=0.6, fall=0.5, seed=0, nu=0.4,
+                rho=1000.0, E_range=(1e5, 1e9), **kw):
+    """n_side^3 tet cubes of cells^3 hex cells placed on a grid with seeded
+    +/-15% jitter and uniform random rotations, Young's modulus log-uniform in
+    E_range, lowest cube 'fall' metres above the ground."""
+    rng = np.random.default_rng(seed)
+    x0, t0, p0 = hex_grid((cells,) * 3, size / cells)
+    x0 = x0 - x0.mean(0)
+    half_diag = np.sqrt(3.0) * size / 2
+    bodies = []
+    for a in range(n_side):
+        for b in range(n_side):
+            for c in range(n_side):
+                R = random_rotation(rng)
+                jit = rng.uniform(-0.15, 0.15, 3) * spacing
+                ctr = np.array([a * spacing, fall + half_diag + b * spacing, c * spacing]) + jit
+                ctr[1] = max(ctr[1], fall + half_diag)
+                E = float(np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]))))
+                bodies.append((x0 @ R.T + ctr, t0, p0, E, nu, rho, 0))
+    defaults = dict(dt=0.01, dhat=1e-3, name=f"cubes_{n_side ** 3}x{cells}^3")
+    defaults.update(kw)
+    sc = assemble_bodies(bodies, **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
+
+
+def _diag(x):
+    return float(np.linalg.norm(x.max(0) - x.min(0)))
