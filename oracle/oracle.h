@@ -158,6 +158,15 @@ int  orc_world_newton_iteration(orc_world* w, double* alpha_out,
                                 int* converged_out);
 int  orc_world_time_step(orc_world* w, int* newton_iters_out,
                          int* pcg_iters_out);
+/* additive CCD time of impact (ACCD.cu:373-524), eta = 1 - slackness */
+double orc_pt_ccd(const double* p, const double* t0, const double* t1,
+                  const double* t2, const double* dp, const double* dt0,
+                  const double* dt1, const double* dt2, double eta,
+                  double thickness);
+double orc_ee_ccd(const double* a0, const double* a1, const double* b0,
+                  const double* b1, const double* da0, const double* da1,
+                  const double* db0, const double* db1, double eta,
+                  double thickness);
 
 #ifdef __cplusplus
 }

@@ -675,8 +675,9 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     /* x_tilde for the first step (main.cu:1037-1045 computeXTilta) */
     for(int v = 0; v < nv; v++)
         for(int c = 0; c < 3; c++)
-            w->xt[3 * v + c] = w->xprev[3 * v + c] + s->dt * w->v[3 * v + c]
-                               + (bt(w, v) == 0 ? s->gravity[c] * s->dt * s->dt : 0.0);
+            w->xt[3 * v + c] = w->xprev[3 * v + c]
+                               + (w->v[3 * v + c] * s->dt
+                                  + (bt(w, v) == 0 ? s->gravity[c] * (s->dt * s->dt) : 0.0));
     build_cp(w);
     return w;
 }
@@ -1173,18 +1174,35 @@ int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
         for(int c = 0; c < 3; c++)
         {
             if(bt(w, v) == 0)
-                w->v[3 * v + c] = (w->x[3 * v + c] - w->xprev[3 * v + c]) / w->s.dt;
+                w->v[3 * v + c] = (w->x[3 * v + c] - w->xprev[3 * v + c]) * (1 / w->s.dt);
             else
                 w->v[3 * v + c] = 0.0;
             w->xprev[3 * v + c] = w->x[3 * v + c];
         }
     for(int v = 0; v < w->nv; v++)
         for(int c = 0; c < 3; c++)
-            w->xt[3 * v + c] = w->xprev[3 * v + c] + w->s.dt * w->v[3 * v + c]
-                               + (bt(w, v) == 0 ? w->s.gravity[c] * w->s.dt * w->s.dt : 0.0);
+            w->xt[3 * v + c] = w->xprev[3 * v + c]
+                               + (w->v[3 * v + c] * w->s.dt
+                                  + (bt(w, v) == 0 ? w->s.gravity[c] * (w->s.dt * w->s.dt) : 0.0));
     if(newton_iters_out)
         *newton_iters_out = k;
     if(pcg_iters_out)
         *pcg_iters_out = total_pcg;
     return 0;
+}
+
+/* exported ACCD entry points for the oracle's own tests (displacements are
+ * the motion over the step, as passed to point_triangle_ccd) */
+double orc_pt_ccd(const double* p, const double* t0, const double* t1, const double* t2,
+                  const double* dp, const double* dt0, const double* dt1, const double* dt2,
+                  double eta, double thickness)
+{
+    return pt_ccd(V(p), V(t0), V(t1), V(t2), V(dp), V(dt0), V(dt1), V(dt2), eta, thickness);
+}
+
+double orc_ee_ccd(const double* a0, const double* a1, const double* b0, const double* b1,
+                  const double* da0, const double* da1, const double* db0, const double* db1,
+                  double eta, double thickness)
+{
+    return ee_ccd(V(a0), V(a1), V(b0), V(b1), V(da0), V(da1), V(db0), V(db1), eta, thickness);
 }

@@ -105,6 +105,10 @@ def lib():
         L.orc_world_energy.argtypes = [C.c_void_p, dp]
         L.orc_world_newton_iteration.argtypes = [C.c_void_p, dp, ip, dp, ip]
         L.orc_world_time_step.argtypes = [C.c_void_p, ip, ip]
+        L.orc_pt_ccd.restype = C.c_double
+        L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
+        L.orc_ee_ccd.restype = C.c_double
+        L.orc_ee_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -316,3 +320,57 @@ class World:
         it = C.c_int32()
         lib().orc_world_time_step(self.h, C.byref(n), C.byref(it))
         return dict(newton_iterations=n.value, pcg_iterations=it.value)
+
+
+def pt_ccd(p, t0, t1, t2, dp, dt0, dt1, dt2, eta=0.2, thickness=0.0):
+    a = [np.ascontiguousarray(v, np.float64) for v in (p, t0, t1, t2, dp, dt0, dt1, dt2)]
+    return lib().orc_pt_ccd(*[P(v) for v in a], eta, thickness)
+
+
+def ee_ccd(a0, a1, b0, b1, da0, da1, db0, db1, eta=0.2, thickness=0.0):
+    a = [np.ascontiguousarray(v, np.float64) for v in (a0, a1, b0, b1, da0, da1, db0, db1)]
+    return lib().orc_ee_ccd(*[P(v) for v in a], eta, thickness)
+
+
+def barrier_grad_hess(x, code, dhat2, kappa):
+    x = np.ascontiguousarray(x, np.float64)
+    code = np.ascontiguousarray(code, np.int32)
+    ids = np.zeros(4, np.int32)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    nv = lib().orc_barrier_grad_hess(P(x), P(x), P(code, C.c_int32), dhat2, kappa,
+                                     P(ids, C.c_int32), P(g), P(H))
+    return ids[:nv], g[:3 * nv], H[:(3 * nv) ** 2].reshape(3 * nv, 3 * nv)
+
+
+def barrier_energy(x, code, dhat2, kappa):
+    x = np.ascontiguousarray(x, np.float64)
+    code = np.ascontiguousarray(code, np.int32)
+    return lib().orc_barrier_energy(P(x), P(x), P(code, C.c_int32), dhat2, kappa)
+
+
+def tet_grad_hess(x, tet, dminv, vol, mu, lam, dt):
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(tet, np.uint32)
+    Dm = np.ascontiguousarray(dminv, np.float64).reshape(9)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().orc_tet_grad_hess(P(x), P(t, C.c_uint32), P(Dm), vol, mu, lam, dt, P(g), P(H))
+    return g, H.reshape(12, 12)
+
+
+def tet_energy(x, tet, dminv, vol, mu, lam):
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(tet, np.uint32)
+    Dm = np.ascontiguousarray(dminv, np.float64).reshape(9)
+    return lib().orc_tet_energy(P(x), P(t, C.c_uint32), P(Dm), vol, mu, lam)
+
+
+def ground(x, n, off, dhat2, kappa):
+    x = np.ascontiguousarray(x, np.float64)
+    n = np.ascontiguousarray(n, np.float64)
+    g = np.zeros(3)
+    H = np.zeros(9)
+    lib().orc_ground_grad_hess(P(x), P(n), off, dhat2, kappa, P(g), P(H))
+    d = float(n @ x - off)
+    return lib().orc_ground_energy(d * d, dhat2) * kappa, g, H.reshape(3, 3)

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "../../include/sgipc.h"
+#include "context.h"
 #include "linsys.h"
 
 using namespace sgipc;
@@ -223,6 +224,136 @@ int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain)
         auto v = ls->mas.chain_host(ls->s);
         std::memcpy(chain, v.data(), v.size() * sizeof(int));
     });
+}
+
+
+// ------------------------------------------------------------------ context
+struct sgipc_context
+{
+    Context* c = nullptr;
+    ~sgipc_context() { delete c; }
+};
+
+static Context& ctx_of(sgipc_t h)
+{
+    SG_CHECK(h && h->c, SGIPC_EINVAL, "null context");
+    return *h->c;
+}
+
+int sgipc_create(const sgipc_scene_desc* desc, sgipc_t* out)
+{
+    return guard([&] {
+        SG_CHECK(desc && out, SGIPC_EINVAL, "null argument");
+        auto* h = new sgipc_context();
+        try
+        {
+            h->c = new Context(*desc);
+        }
+        catch(...)
+        {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int sgipc_destroy(sgipc_t h)
+{
+    return guard([&] { delete h; });
+}
+
+int sgipc_set_positions(sgipc_t h, const double* x)
+{
+    return guard([&] {
+        SG_CHECK(x, SGIPC_EINVAL, "null positions");
+        ctx_of(h).set_positions(x);
+    });
+}
+
+int sgipc_get_positions(sgipc_t h, double* x)
+{
+    return guard([&] {
+        SG_CHECK(x, SGIPC_EINVAL, "null output");
+        auto& c = ctx_of(h);
+        c.get(c.x_, x);
+    });
+}
+
+int sgipc_get_velocities(sgipc_t h, double* v)
+{
+    return guard([&] {
+        SG_CHECK(v, SGIPC_EINVAL, "null output");
+        auto& c = ctx_of(h);
+        c.get(c.v_, v);
+    });
+}
+
+int sgipc_set_xtilde(sgipc_t h, const double* xt)
+{
+    return guard([&] {
+        SG_CHECK(xt, SGIPC_EINVAL, "null x_tilde");
+        ctx_of(h).set_xtilde(xt);
+    });
+}
+
+int sgipc_get_xtilde(sgipc_t h, double* xt)
+{
+    return guard([&] {
+        SG_CHECK(xt, SGIPC_EINVAL, "null output");
+        ctx_of(h).get_xtilde(xt);
+    });
+}
+
+int sgipc_get_kappa(sgipc_t h, double* k)
+{
+    return guard([&] {
+        SG_CHECK(k, SGIPC_EINVAL, "null output");
+        *k = ctx_of(h).kappa();
+    });
+}
+
+int sgipc_set_kappa(sgipc_t h, double k)
+{
+    return guard([&] { ctx_of(h).set_kappa(k); });
+}
+
+int sgipc_lbvh_build(sgipc_t h, int32_t which, uint64_t* keys, uint32_t* nodes, double* aabbs)
+{
+    return guard([&] {
+        SG_CHECK(which == 0 || which == 1, SGIPC_EINVAL, "which must be 0 (faces) or 1 (edges)");
+        ctx_of(h).lbvh_build(which, keys, nodes, aabbs);
+    });
+}
+
+int sgipc_contact_query(sgipc_t h, int32_t* pairs, int32_t cap, int32_t* counts,
+                        uint32_t* ground, int32_t gcap, int32_t* n_ground)
+{
+    return guard([&] { ctx_of(h).contact_query(pairs, cap, counts, ground, gcap, n_ground); });
+}
+
+int sgipc_assemble(sgipc_t h, double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols,
+                   double* vals, int32_t cap)
+{
+    return guard([&] { ctx_of(h).assemble_out(grad, nnzb, rowptr, cols, vals, cap); });
+}
+
+int sgipc_energy(sgipc_t h, double* terms)
+{
+    return guard([&] {
+        SG_CHECK(terms, SGIPC_EINVAL, "null output");
+        ctx_of(h).energy_terms(terms);
+    });
+}
+
+int sgipc_newton_iteration(sgipc_t h, sgipc_iter_stats* st)
+{
+    return guard([&] { ctx_of(h).newton_iteration(st); });
+}
+
+int sgipc_time_step(sgipc_t h, sgipc_step_stats* st)
+{
+    return guard([&] { ctx_of(h).time_step(st); });
 }
 
 }  // extern "C"

@@ -1,0 +1,82 @@
+// context.h -- device-resident simulation state and the Newton driver
+// (counterpart of the reference's GIPC class, StiffGIPC/GIPC.cuh:25-249).
+#pragma once
+
+#include "../../include/sgipc.h"
+#include "linsys.h"
+#include "sim_kernels.h"
+
+namespace sgipc
+{
+
+class Context
+{
+  public:
+    explicit Context(const sgipc_scene_desc& d);
+    ~Context();
+
+    int  nv() const { return nv_; }
+    void set_positions(const double* x);
+    void get(const DBuf<double>& b, double* out);
+    void set_xtilde(const double* xt);
+    void get_xtilde(double* out) { get(xt_, out); }
+    double kappa() const { return kappa_; }
+    void   set_kappa(double k) { kappa_ = k; }
+
+    // hot-path pieces exposed through the C ABI
+    void lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs);
+    void contact_query(int32_t* pairs, int cap, int32_t* counts, uint32_t* ground, int gcap,
+                       int32_t* n_ground);
+    void assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols, double* vals,
+                      int cap);
+    void energy_terms(double* terms);
+    void newton_iteration(sgipc_iter_stats* st);
+    void time_step(sgipc_step_stats* st);
+
+    DBuf<double> x_, v_;
+
+  private:
+    // reference phases
+    void   build_bvh();                    // buildBVH
+    void   build_cp();                     // buildCP (self + ground proximity)
+    void   assemble(bool hessian);         // computeGradientAndHessian
+    double energy();                       // computeEnergy
+    bool   is_intersected();               // isIntersected
+    double ground_bound(double slack);     // ground_largestFeasibleStepSize
+    double self_bound(const int4* cand, const unsigned* count, int cap, double slack);
+    int    full_ccd(double alpha);         // buildBVH_FULLCCD + buildFullCP
+    double cfl_speed();
+    void   line_search(double& alpha, double cfl_alpha, int* nls, double* e0, double* e1);
+    void   suggest_kappa(double& k);
+    void   upper_bound_kappa(double& k);
+    void   init_kappa();
+    void   check_overflow();
+    double read_d(const double* p);
+    unsigned read_u(const unsigned* p);
+
+    hipStream_t s_ = nullptr;
+    sgipc_scene_desc desc_{};
+    int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0;
+    double dHat_ = 0, kappa_ = 0, bbox_diag2_ = 0, scene_diag_ = 0, mean_mass_ = 0;
+    int    newton_k_ = 0;
+    double xdelta0_  = 0;
+
+    DBuf<double>   xr_, xprev_, xt_, mass_, dx_, xsave_, gc_, gs_, rhs_, scratch_;
+    DBuf<double>   dminv_, vol_, mu_, lam_;
+    DBuf<int>      btype_, body_, tslot_;
+    DBuf<uint32_t> tets_, faces_, edges_, surf_, gp_;
+    DBuf<unsigned> gcount_, pcount_, ccount_;
+    DBuf<int4>     pairs_, pairs_raw_, cand_;
+    DBuf<int>      overflow_;
+    DBuf<double>   epart_, eterms_;
+    int            pair_cap_ = 0, cand_cap_ = 0;
+
+    Bsr   A_;
+    Coo   coo_;
+    Mas   mas_;
+    Pcg   pcg_;
+    Lbvh  bf_, be_;
+    SimDev dev_;
+};
+
+}  // namespace sgipc

@@ -1,0 +1,749 @@
+// context.hip -- the Newton driver (host C++ around the device kernels).
+//
+// Mirrors StiffGIPC/GIPC.cu:
+//   set-up           initFEM (main.cu:310-356), GIPC::init (:8534-8581)
+//   assembly         computeGradientAndHessian (:10012-10228)
+//   one iteration    solve_subIP loop body (:10746-10862)
+//   line search      lineSearch (:10573-10670)
+//   kappa            suggestKappa / upperBoundKappa / initKappa (:9746-9831);
+//                    postLineSearch's doubling never fires in the reference
+//                    (h_close_*Num are never read back, GIPC.cuh:69-70) and is
+//                    therefore not reproduced
+//   time step        IPC_Solver (:10940-11131) without friction
+// All state stays on the device; the host reads back only the scalars that
+// steer control flow (contact counts at capacity checks, step bounds, line
+// search energies, PCG convergence tests).
+#include <algorithm>
+#include <array>
+#include <cmath>
+
+#include "context.h"
+
+namespace sgipc
+{
+
+static double now_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    SG_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+Context::Context(const sgipc_scene_desc& d)
+{
+    desc_ = d;
+    nv_   = d.n_verts;
+    nt_   = d.n_tets;
+    nf_   = d.n_faces;
+    ne_   = d.n_edges;
+    ns_   = d.n_surf_verts;
+    SG_CHECK(nv_ > 0 && d.x, SGIPC_EINVAL, "scene has no vertices");
+    SG_CHECK(nt_ == 0 || (d.tets && d.tet_E && d.tet_nu && d.tet_rho), SGIPC_EINVAL,
+             "tet arrays missing");
+    SG_CHECK(nf_ == 0 || d.faces, SGIPC_EINVAL, "faces missing");
+    SG_CHECK(ne_ == 0 || d.edges, SGIPC_EINVAL, "edges missing");
+    SG_CHECK(ns_ == 0 || d.surf_verts, SGIPC_EINVAL, "surface vertices missing");
+    SG_CHECK(d.dt > 0 && d.dhat > 0, SGIPC_EINVAL, "dt and dhat must be positive");
+    SG_CHECK(d.friction_mu == 0.0, SGIPC_EINVAL, "friction is not available (friction_mu must be 0)");
+    for(int e = 0; e < 4 * nt_; e++)
+        SG_CHECK(d.tets[e] < (uint32_t)nv_, SGIPC_EINVAL, "tet vertex index out of range");
+    for(int e = 0; e < 3 * nf_; e++)
+        SG_CHECK(d.faces[e] < (uint32_t)nv_, SGIPC_EINVAL, "face vertex index out of range");
+    for(int e = 0; e < 2 * ne_; e++)
+        SG_CHECK(d.edges[e] < (uint32_t)nv_, SGIPC_EINVAL, "edge vertex index out of range");
+    for(int e = 0; e < ns_; e++)
+        SG_CHECK(d.surf_verts[e] < (uint32_t)nv_, SGIPC_EINVAL, "surface vertex out of range");
+    SG_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    dHat_  = d.dhat * d.dhat;
+    kappa_ = d.kappa > 0 ? d.kappa : 0.0;
+
+    const double*    xr = d.x_rest ? d.x_rest : d.x;
+    std::vector<int> bt(nv_, 0), body(nv_, -1);
+    if(d.btype)
+        bt.assign(d.btype, d.btype + nv_);
+    if(d.body_id)
+        body.assign(d.body_id, d.body_id + nv_);
+
+    // ---- FEM constants (initFEM): Dm^-1, |V|, Lame, lumped mass
+    std::vector<double> mass(nv_, 0.0), dminv((size_t)9 * nt_), vol(nt_), mu(nt_), lam(nt_);
+    for(int e = 0; e < nt_; e++)
+    {
+        const uint32_t* t = d.tets + 4 * (size_t)e;
+        double          D[9];
+        for(int i = 0; i < 3; i++)
+            for(int k = 0; k < 3; k++)
+                D[i * 3 + k] = xr[3 * t[k + 1] + i] - xr[3 * t[0] + i];
+        double c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8],
+               c02 = D[3] * D[7] - D[4] * D[6];
+        double det = D[0] * c00 + D[1] * c01 + D[2] * c02;
+        SG_CHECK(det != 0.0, SGIPC_EINVAL, "degenerate rest tetrahedron");
+        double inv[9] = {c00 / det,
+                         (D[2] * D[7] - D[1] * D[8]) / det,
+                         (D[1] * D[5] - D[2] * D[4]) / det,
+                         c01 / det,
+                         (D[0] * D[8] - D[2] * D[6]) / det,
+                         (D[2] * D[3] - D[0] * D[5]) / det,
+                         c02 / det,
+                         (D[1] * D[6] - D[0] * D[7]) / det,
+                         (D[0] * D[4] - D[1] * D[3]) / det};
+        for(int k = 0; k < 9; k++)
+            dminv[(size_t)k * nt_ + e] = inv[k];  // SoA
+        double V = std::fabs(det) / 6.0;
+        vol[e]   = V;
+        for(int k = 0; k < 4; k++)
+            mass[t[k]] += V * d.tet_rho[e] / 4;
+        double E = d.tet_E[e], nu = d.tet_nu[e];
+        mu[e]    = E / (2 * (1 + nu));
+        lam[e]   = E * nu / ((1 + nu) * (1 - 2 * nu));
+    }
+    double msum = 0;
+    for(double m : mass)
+        msum += m;
+    mean_mass_ = msum / nv_;
+
+    // ---- static BSR pattern: tet adjacency + every diagonal
+    std::vector<std::vector<int>> adj(nv_);
+    for(int v = 0; v < nv_; v++)
+        adj[v].push_back(v);
+    for(int e = 0; e < nt_; e++)
+    {
+        const uint32_t* t = d.tets + 4 * (size_t)e;
+        for(int a = 0; a < 4; a++)
+            for(int b = 0; b < 4; b++)
+                if(a != b)
+                    adj[t[a]].push_back((int)t[b]);
+    }
+    std::vector<int> rp(nv_ + 1, 0), cl;
+    for(int v = 0; v < nv_; v++)
+    {
+        auto& a = adj[v];
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+        rp[v + 1] = rp[v] + (int)a.size();
+    }
+    cl.reserve(rp[nv_]);
+    for(int v = 0; v < nv_; v++)
+        cl.insert(cl.end(), adj[v].begin(), adj[v].end());
+    std::vector<int> ts((size_t)16 * nt_);
+    for(int e = 0; e < nt_; e++)
+    {
+        const uint32_t* t = d.tets + 4 * (size_t)e;
+        for(int a = 0; a < 4; a++)
+            for(int b = 0; b < 4; b++)
+            {
+                auto beg = cl.begin() + rp[t[a]], end = cl.begin() + rp[t[a] + 1];
+                ts[16 * (size_t)e + 4 * a + b] =
+                    (int)(std::lower_bound(beg, end, (int)t[b]) - cl.begin());
+            }
+    }
+    A_.set_pattern(nv_, rp, cl, s_);
+
+    // ---- scene extents (GIPC::init, main.cu:853-863)
+    double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
+    for(int f = 0; f < nf_; f++)
+        for(int k = 0; k < 3; k++)
+            for(int c = 0; c < 3; c++)
+            {
+                double p = d.x[3 * d.faces[3 * f + k] + c];
+                lo[c]    = p < lo[c] ? p : lo[c];
+                up[c]    = up[c] < p ? p : up[c];
+            }
+    bbox_diag2_ = 0;
+    for(int c = 0; c < 3; c++)
+        bbox_diag2_ += (up[c] - lo[c]) * (up[c] - lo[c]);
+    double u2[3] = {-1e32, -1e32, -1e32}, l2[3] = {1e32, 1e32, 1e32};
+    bool   any   = false;
+    for(int v = 0; v < nv_; v++)
+        if(bt[v] == 0)
+        {
+            any = true;
+            for(int c = 0; c < 3; c++)
+            {
+                double p = d.x[3 * v + c];
+                l2[c]    = p < l2[c] ? p : l2[c];
+                u2[c]    = u2[c] < p ? p : u2[c];
+            }
+        }
+    scene_diag_ = 0;
+    if(any)
+        for(int c = 0; c < 3; c++)
+            scene_diag_ += (u2[c] - l2[c]) * (u2[c] - l2[c]);
+    scene_diag_ = std::sqrt(scene_diag_);
+
+    // ---- device state
+    size_t n3 = 3 * (size_t)nv_;
+    x_.upload(d.x, n3, s_);
+    xr_.upload(xr, n3, s_);
+    xprev_.upload(d.x, n3, s_);
+    std::vector<double> v0(n3, 0.0);
+    if(d.v)
+        v0.assign(d.v, d.v + n3);
+    v_.upload(v0, s_);
+    std::vector<double> xt(n3);
+    for(int i = 0; i < nv_; i++)
+        for(int c = 0; c < 3; c++)
+            xt[3 * i + c] = d.x[3 * i + c]
+                            + (v0[3 * i + c] * d.dt + (bt[i] == 0 ? d.gravity[c] * (d.dt * d.dt) : 0.0));
+    xt_.upload(xt, s_);
+    mass_.upload(mass, s_);
+    dx_.resize(n3);
+    dx_.zero(s_);
+    xsave_.resize(n3);
+    gc_.resize(n3);
+    gs_.resize(n3);
+    rhs_.resize(n3);
+    scratch_.resize(16);
+    dminv_.upload(dminv, s_);
+    vol_.upload(vol, s_);
+    mu_.upload(mu, s_);
+    lam_.upload(lam, s_);
+    btype_.upload(bt, s_);
+    body_.upload(body, s_);
+    tslot_.upload(ts, s_);
+    tets_.upload(d.tets, 4 * (size_t)nt_, s_);
+    faces_.upload(d.faces, 3 * (size_t)nf_, s_);
+    edges_.upload(d.edges, 2 * (size_t)ne_, s_);
+    surf_.upload(d.surf_verts, ns_, s_);
+    gp_.resize(ns_ + 1);
+    gcount_.resize(1);
+    pcount_.resize(5);
+    ccount_.resize(1);
+    pair_cap_ = std::max(2 * (ns_ + ne_), 100000);
+    cand_cap_ = std::max(4 * (ns_ + ne_), 200000);
+    pairs_.resize(pair_cap_);
+    pairs_raw_.resize(pair_cap_);
+    cand_.resize(cand_cap_);
+    overflow_.resize(4);
+    overflow_.zero(s_);
+    coo_.reserve(6 * pair_cap_);
+
+    dev_.nv = nv_, dev_.nt = nt_, dev_.ns = ns_, dev_.nf = nf_, dev_.ne = ne_;
+    dev_.dt = d.dt, dev_.dHat = dHat_, dev_.goff = d.ground_offset;
+    dev_.gn   = V3{d.ground_normal[0], d.ground_normal[1], d.ground_normal[2]};
+    dev_.grav = V3{d.gravity[0], d.gravity[1], d.gravity[2]};
+    dev_.use_ground = d.use_ground;
+    dev_.x = x_.data(), dev_.xt = xt_.data(), dev_.mass = mass_.data();
+    dev_.gshape = gs_.data(), dev_.gcontact = gc_.data(), dev_.vals = A_.vals.data();
+    dev_.dminv = dminv_.data(), dev_.vol = vol_.data(), dev_.mu = mu_.data(), dev_.lam = lam_.data();
+    dev_.btype = btype_.data(), dev_.body = body_.data(), dev_.diag = A_.diag.data();
+    dev_.tslot = tslot_.data(), dev_.rowptr = A_.rowptr.data(), dev_.cols = A_.cols.data();
+    dev_.tets = tets_.data(), dev_.surf = surf_.data(), dev_.faces = faces_.data();
+    dev_.edges = edges_.data();
+    dev_.gp = gp_.data(), dev_.gcount = gcount_.data();
+    dev_.pairs = pairs_.data(), dev_.pairs_raw = pairs_raw_.data(), dev_.pcount = pcount_.data();
+    dev_.pair_cap = pair_cap_;
+    dev_.coo_r = coo_.r.data(), dev_.coo_c = coo_.c.data(), dev_.coo_v = coo_.v.data();
+    dev_.coo_count = coo_.count.data(), dev_.coo_cap = coo_.cap;
+    dev_.overflow = overflow_.data();
+    coo_.count.zero(s_);
+
+    EnergyLayout L = energy_layout(dev_);
+    epart_.resize(L.off[3] + L.n[3] + 1);
+    eterms_.resize(8);
+
+    if(d.precond == SGIPC_PRECOND_MAS)
+    {
+        MasConfig mc;
+        mc.fp32 = d.mas_precision == SGIPC_MAS_FP32;
+        mas_.init(A_, d.part_id, mc, s_);
+    }
+    bf_.init(nf_, 3, faces_.data());
+    be_.init(ne_, 2, edges_.data());
+    build_bvh();
+    build_cp();
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+Context::~Context()
+{
+    if(s_)
+        (void)hipStreamDestroy(s_);
+}
+
+double Context::read_d(const double* p)
+{
+    double h;
+    SG_HIP(hipMemcpyAsync(&h, p, sizeof(double), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    return h;
+}
+
+unsigned Context::read_u(const unsigned* p)
+{
+    unsigned h;
+    SG_HIP(hipMemcpyAsync(&h, p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    return h;
+}
+
+void Context::check_overflow()
+{
+    int h[4];
+    SG_HIP(hipMemcpyAsync(h, overflow_.data(), sizeof(h), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    SG_CHECK(h[0] == 0 && h[1] == 0 && h[2] == 0, SGIPC_ECAPACITY,
+             "contact buffer capacity exceeded (pairs/COO/CCD candidates)");
+}
+
+void Context::set_positions(const double* x)
+{
+    x_.upload(x, 3 * (size_t)nv_, s_);
+    build_bvh();
+    build_cp();
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+void Context::get(const DBuf<double>& b, double* out)
+{
+    b.download(out, 3 * (size_t)nv_, s_);
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+void Context::set_xtilde(const double* xt)
+{
+    xt_.upload(xt, 3 * (size_t)nv_, s_);
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+// ------------------------------------------------------------------ phases
+void Context::build_bvh()
+{
+    bf_.build(x_.data(), nullptr, 0.0, s_);
+    be_.build(x_.data(), nullptr, 0.0, s_);
+}
+
+void Context::build_cp()
+{
+    launch_query_proximity(dev_, bf_, be_, xr_.data(), s_);
+    launch_ground_detect(dev_, s_);
+}
+
+void Context::assemble(bool hessian)
+{
+    if(hessian)
+    {
+        A_.vals.zero(s_);
+        coo_.count.zero(s_);
+    }
+    launch_init_diag(dev_, s_);
+    launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
+    launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
+    launch_fem(dev_, hessian, s_);
+    launch_rhs(dev_, rhs_.data(), s_);
+}
+
+double Context::energy()
+{
+    launch_energy(dev_, kappa_, epart_.data(), eterms_.data(), s_);
+    double t[5];
+    SG_HIP(hipMemcpyAsync(t, eterms_.data(), sizeof(t), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    return t[4];
+}
+
+bool Context::is_intersected()
+{
+    int* hit = overflow_.data() + 3;
+    SG_HIP(hipMemsetAsync(hit, 0, sizeof(int), s_));
+    if(desc_.use_ground)
+        launch_ground_intersect(dev_, hit, s_);
+    int h = 0;
+    SG_HIP(hipMemcpyAsync(&h, hit, sizeof(int), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    if(h)
+        return true;
+    launch_edge_tri(dev_, be_, hit, s_);
+    SG_HIP(hipMemcpyAsync(&h, hit, sizeof(int), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    return h != 0;
+}
+
+double Context::ground_bound(double slack)
+{
+    if(ns_ < 1)
+        return 1.0;
+    SG_HIP(hipMemsetAsync(scratch_.data(), 0, sizeof(double), s_));
+    launch_ground_ccd(dev_, dx_.data(), slack, scratch_.data(), s_);
+    return 1.0 / read_d(scratch_.data());
+}
+
+double Context::self_bound(const int4* cand, const unsigned* count, int cap, double slack)
+{
+    unsigned n = read_u(count);
+    if(n < 1)
+        return 1.0;
+    SG_HIP(hipMemsetAsync(scratch_.data() + 1, 0, sizeof(double), s_));
+    launch_self_ccd(x_.data(), dx_.data(), cand, count, std::min<int>(cap, (int)n), slack,
+                    scratch_.data() + 1, s_);
+    return 1.0 / read_d(scratch_.data() + 1);
+}
+
+int Context::full_ccd(double alpha)
+{
+    bf_.build(x_.data(), dx_.data(), alpha, s_);
+    be_.build(x_.data(), dx_.data(), alpha, s_);
+    launch_query_ccd(dev_, bf_, be_, dx_.data(), alpha, cand_.data(), ccount_.data(), cand_cap_,
+                     overflow_.data() + 2, s_);
+    check_overflow();
+    return (int)read_u(ccount_.data());
+}
+
+double Context::cfl_speed()
+{
+    SG_HIP(hipMemsetAsync(scratch_.data() + 2, 0, sizeof(double), s_));
+    launch_cfl(dev_, dx_.data(), scratch_.data() + 2, s_);
+    return read_d(scratch_.data() + 2);
+}
+
+void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0, double* e1)
+{
+    double E0 = energy();
+    if(e0)
+        *e0 = E0;
+    SG_HIP(hipMemsetAsync(scratch_.data() + 3, 0, sizeof(double), s_));
+    launch_dot(3 * nv_, gc_.data(), dx_.data(), scratch_.data() + 3, s_);
+    double c1m = desc_.armijo_c1 * read_d(scratch_.data() + 3);
+    SG_HIP(hipMemcpyAsync(xsave_.data(), x_.data(), 3 * (size_t)nv_ * sizeof(double),
+                          hipMemcpyDeviceToDevice, s_));
+    launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+    build_bvh();
+    while(is_intersected())
+    {
+        alpha /= 2.0;
+        alpha = std::min(cfl_alpha, alpha);
+        launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+        build_bvh();
+    }
+    build_cp();
+    double E  = energy();
+    double LF = alpha;
+    int    k  = 0;
+    for(int i = 0; i < 30; i++)
+    {
+        if(E <= E0 + c1m * alpha)
+            break;
+        alpha *= desc_.armijo_beta;
+        k++;
+        launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+        build_bvh();
+        build_cp();
+        E = energy();
+    }
+    if(alpha < LF)
+    {
+        bool recompute = false;
+        while(is_intersected())
+        {
+            alpha *= desc_.armijo_beta;
+            alpha = std::clamp(alpha, desc_.armijo_alpha_min, cfl_alpha);
+            launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+            build_bvh();
+            recompute = true;
+        }
+        if(recompute)
+            build_cp();
+    }
+    check_overflow();
+    if(nls)
+        *nls = k;
+    if(e1)
+        *e1 = E;
+}
+
+static double compute_H_b(double d, double dHat)
+{
+    double t = d - dHat;
+    return (std::log(d / dHat) * -2.0 - t * 4.0 / d) + 1.0 / (d * d) * (t * t);
+}
+
+void Context::suggest_kappa(double& k)
+{
+    double H_b = compute_H_b(1.0e-16 * bbox_diag2_, dHat_);
+    k = (mean_mass_ == 0.0) ? 1e11 / (4.0e-16 * bbox_diag2_ * H_b)
+                            : 1e11 * mean_mass_ / (4.0e-16 * bbox_diag2_ * H_b);
+}
+
+void Context::upper_bound_kappa(double& k)
+{
+    double H_b  = compute_H_b(1.0e-16 * bbox_diag2_, dHat_);
+    double kmax = 100 * 1e11 * mean_mass_ / (4.0e-16 * bbox_diag2_ * H_b);
+    if(mean_mass_ == 0.0)
+        kmax = 100 * 1e11 / (4.0e-16 * bbox_diag2_ * H_b);
+    if(k > kmax)
+        k = kmax;
+}
+
+void Context::init_kappa()
+{
+    if(read_u(pcount_.data()) == 0)
+        return;
+    // GE = kinetic + elastic gradient (gs), gc = unit-kappa ground + barrier
+    launch_init_diag(dev_, s_);
+    launch_fem(dev_, false, s_);
+    launch_ground(dev_, 1.0, false, gc_.data(), s_);
+    launch_barrier(dev_, 1.0, false, gc_.data(), s_);
+    SG_HIP(hipMemsetAsync(scratch_.data() + 4, 0, 2 * sizeof(double), s_));
+    launch_dot(3 * nv_, gc_.data(), gs_.data(), scratch_.data() + 4, s_);
+    launch_dot(3 * nv_, gc_.data(), gc_.data(), scratch_.data() + 5, s_);
+    double h[2];
+    SG_HIP(hipMemcpyAsync(h, scratch_.data() + 4, sizeof(h), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    double minKappa = -h[0] / h[1];
+    if(minKappa > 0.0)
+        kappa_ = minKappa;
+    suggest_kappa(minKappa);
+    if(kappa_ < minKappa)
+        kappa_ = minKappa;
+    upper_bound_kappa(kappa_);
+}
+
+// ------------------------------------------------------------- iteration
+void Context::newton_iteration(sgipc_iter_stats* st)
+{
+    sgipc_iter_stats loc{};
+    hipEvent_t       ev[5];
+    for(auto& e : ev)
+        SG_HIP(hipEventCreate(&e));
+    SG_HIP(hipEventRecord(ev[0], s_));
+    assemble(true);
+    check_overflow();
+    SG_HIP(hipMemsetAsync(scratch_.data() + 6, 0, sizeof(double), s_));
+    launch_max_abs(3 * nv_, dx_.data(), scratch_.data() + 6, s_);
+    double xdelta = read_d(scratch_.data() + 6);
+    SG_HIP(hipEventRecord(ev[1], s_));
+    loc.xdelta   = xdelta;
+    loc.n_pairs  = (int)read_u(pcount_.data());
+    loc.n_ground = (int)read_u(gcount_.data());
+    int k        = newton_k_;
+    if(k == 1)
+        xdelta0_ = xdelta;
+    if(k > 0 && (xdelta < scene_diag_ * desc_.abs_xdelta_tol || xdelta < desc_.rel_xdelta_tol * xdelta0_))
+    {
+        loc.converged = 1;
+        SG_HIP(hipStreamSynchronize(s_));
+        loc.ms_assembly = loc.ms_total = now_ms(ev[0], ev[1]);
+        for(auto& e : ev)
+            (void)hipEventDestroy(e);
+        if(st)
+            *st = loc;
+        return;
+    }
+    // ---- linear solve
+    SysView S{&A_, &coo_};
+    Precond pc = desc_.precond == SGIPC_PRECOND_MAS ? Precond::Mas : Precond::Diag;
+    if(pc == Precond::Mas)
+        mas_.build(S, s_);
+    PcgConfig cfg;
+    cfg.rel_tol      = desc_.pcg_rel_tol;
+    cfg.abs_tol      = desc_.pcg_abs_tol;
+    cfg.max_iter     = desc_.pcg_max_iter;
+    cfg.use_rz       = desc_.pcg_use_preconditioned_norm != 0;
+    PcgStats ps      = pcg_.solve(S, pc, &mas_, rhs_.data(), dx_.data(), cfg, s_);
+    loc.pcg_iterations   = ps.iterations;
+    loc.pcg_rel_residual = ps.rel_residual;
+    SG_HIP(hipEventRecord(ev[2], s_));
+    // ---- step bounds
+    double alpha = 1.0;
+    if(desc_.use_ground)
+        alpha = std::min(alpha, ground_bound(0.8));
+    alpha = std::min(alpha, self_bound(pairs_raw_.data(), pcount_.data(), pair_cap_, 0.8));
+    double temp_alpha = alpha, alpha_cfl = alpha;
+    int    nccd       = full_ccd(temp_alpha);
+    if(nccd > 0)
+    {
+        double maxSpeed = cfl_speed();
+        alpha_cfl       = std::sqrt(dHat_) / maxSpeed * 0.5;
+        alpha           = std::min(alpha, alpha_cfl);
+        if(temp_alpha > 2 * alpha_cfl)
+        {
+            alpha = std::min(temp_alpha, self_bound(cand_.data(), ccount_.data(), cand_cap_, 0.8));
+            alpha = std::max(alpha, alpha_cfl);
+        }
+    }
+    loc.alpha_ccd = alpha;
+    SG_HIP(hipEventRecord(ev[3], s_));
+    // ---- line search
+    line_search(alpha, alpha_cfl, &loc.line_search_steps, &loc.energy0, &loc.energy1);
+    loc.alpha = alpha;
+    if(kappa_ == 0.0)
+        init_kappa();
+    newton_k_++;
+    SG_HIP(hipEventRecord(ev[4], s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    loc.ms_assembly    = now_ms(ev[0], ev[1]);
+    loc.ms_solve       = now_ms(ev[1], ev[2]);
+    loc.ms_ccd         = now_ms(ev[2], ev[3]);
+    loc.ms_line_search = now_ms(ev[3], ev[4]);
+    loc.ms_total       = now_ms(ev[0], ev[4]);
+    for(auto& e : ev)
+        (void)hipEventDestroy(e);
+    if(st)
+        *st = loc;
+}
+
+void Context::time_step(sgipc_step_stats* st)
+{
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, s_));
+    upper_bound_kappa(kappa_);
+    if(kappa_ < 1e-16)
+        suggest_kappa(kappa_);
+    init_kappa();
+    dx_.zero(s_);
+    newton_k_ = 0;
+    xdelta0_  = 0;
+    int cap   = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
+    int k = 0, pcg_total = 0;
+    for(k = 0; k < cap; k++)
+    {
+        sgipc_iter_stats is{};
+        newton_iteration(&is);
+        if(is.converged)
+            break;
+        pcg_total += is.pcg_iterations;
+    }
+    launch_update_velocity(dev_, xprev_.data(), v_.data(), s_);
+    SG_HIP(hipEventRecord(e1, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    if(st)
+    {
+        st->newton_iterations = k;
+        st->pcg_iterations    = pcg_total;
+        st->ms_total          = now_ms(e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+// ------------------------------------------------------------ ABI helpers
+void Context::energy_terms(double* terms)
+{
+    launch_energy(dev_, kappa_, epart_.data(), eterms_.data(), s_);
+    double t[5];
+    SG_HIP(hipMemcpyAsync(t, eterms_.data(), sizeof(t), hipMemcpyDeviceToHost, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    terms[0] = t[4];
+    terms[1] = t[0];
+    terms[2] = t[1];
+    terms[3] = t[2];
+    terms[4] = t[3];
+    terms[5] = terms[6] = terms[7] = 0.0;
+}
+
+void Context::lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs)
+{
+    Lbvh& b = which == 0 ? bf_ : be_;
+    b.build(x_.data(), nullptr, 0.0, s_);
+    int n = b.n;
+    if(n > 0)
+    {
+        if(keys)
+            b.keys.download(keys, n, s_);
+        if(nodes)
+            b.nodes.download(nodes, 4 * (size_t)(2 * n - 1), s_);
+        if(aabbs)
+            b.bvs.download(aabbs, 6 * (size_t)(2 * n - 1), s_);
+    }
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+void Context::contact_query(int32_t* pairs, int cap, int32_t* counts, uint32_t* ground, int gcap,
+                            int32_t* n_ground)
+{
+    build_bvh();
+    build_cp();
+    check_overflow();
+    unsigned c[5];
+    SG_HIP(hipMemcpyAsync(c, pcount_.data(), sizeof(c), hipMemcpyDeviceToHost, s_));
+    unsigned g = read_u(gcount_.data());
+    if(counts)
+        for(int i = 0; i < 5; i++)
+            counts[i] = (int32_t)c[i];
+    SG_CHECK(!pairs || (int)c[0] <= cap, SGIPC_ECAPACITY, "pair output buffer too small");
+    SG_CHECK(!ground || (int)g <= gcap, SGIPC_ECAPACITY, "ground output buffer too small");
+    if(pairs && c[0])
+        SG_HIP(hipMemcpyAsync(pairs, pairs_.data(), c[0] * sizeof(int4), hipMemcpyDeviceToHost, s_));
+    if(ground && g)
+        SG_HIP(hipMemcpyAsync(ground, gp_.data(), g * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+    if(n_ground)
+        *n_ground = (int32_t)g;
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols,
+                           double* vals, int cap)
+{
+    assemble(true);
+    check_overflow();
+    if(grad)
+        rhs_.download(grad, 3 * (size_t)nv_, s_);
+    std::vector<double> v = A_.vals.to_host(s_);
+    unsigned            nc = read_u(coo_.count.data());
+    std::vector<int>    cr(nc), cc(nc);
+    std::vector<double> cv(9 * (size_t)nc);
+    if(nc)
+    {
+        coo_.r.download(cr.data(), nc, s_);
+        coo_.c.download(cc.data(), nc, s_);
+        coo_.v.download(cv.data(), 9 * (size_t)nc, s_);
+    }
+    SG_HIP(hipStreamSynchronize(s_));
+    // merge static pattern and COO tail row by row
+    std::vector<std::vector<std::pair<int, const double*>>> extra(nv_);
+    for(unsigned k = 0; k < nc; k++)
+        extra[cr[k]].push_back({cc[k], cv.data() + 9 * (size_t)k});
+    std::vector<int>    rp(nv_ + 1, 0), cl;
+    std::vector<double> vl;
+    for(int i = 0; i < nv_; i++)
+    {
+        std::vector<std::pair<int, std::array<double, 9>>> row;
+        for(int k = A_.h_rowptr[i]; k < A_.h_rowptr[i + 1]; k++)
+        {
+            std::array<double, 9> b;
+            for(int e = 0; e < 9; e++)
+                b[e] = v[9 * (size_t)k + e];
+            row.push_back({A_.h_cols[k], b});
+        }
+        for(auto& ex : extra[i])
+        {
+            bool found = false;
+            for(auto& r : row)
+                if(r.first == ex.first)
+                {
+                    for(int e = 0; e < 9; e++)
+                        r.second[e] += ex.second[e];
+                    found = true;
+                    break;
+                }
+            if(!found)
+            {
+                std::array<double, 9> b;
+                for(int e = 0; e < 9; e++)
+                    b[e] = ex.second[e];
+                row.push_back({ex.first, b});
+            }
+        }
+        std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for(auto& r : row)
+        {
+            cl.push_back(r.first);
+            vl.insert(vl.end(), r.second.begin(), r.second.end());
+        }
+        rp[i + 1] = (int)cl.size();
+    }
+    if(nnzb)
+        *nnzb = (int32_t)cl.size();
+    if(cap <= 0)
+        return;
+    SG_CHECK((int)cl.size() <= cap, SGIPC_ECAPACITY, "BSR output buffer too small");
+    if(rowptr)
+        std::copy(rp.begin(), rp.end(), rowptr);
+    if(cols)
+        std::copy(cl.begin(), cl.end(), cols);
+    if(vals)
+        std::copy(vl.begin(), vl.end(), vals);
+}
+
+}  // namespace sgipc

@@ -387,7 +387,7 @@ __device__ __forceinline__ int find_slot(const int* __restrict__ rowptr, const i
     return -1;
 }
 
-__global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ count,
+__global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ count, int pair_cap,
                                                  const int4* __restrict__ pairs,
                                                  const double* __restrict__ x, double dHat,
                                                  double kappa, const int* __restrict__ btype,
@@ -400,7 +400,8 @@ __global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ co
                                                  double* __restrict__ gcontact, int hessian)
 {
     unsigned pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if(pi >= *count)
+    unsigned np = *count;
+    if(pi >= (np < (unsigned)pair_cap ? np : (unsigned)pair_cap))
         return;
     int    ids[4];
     int    type = decode_pair(pairs[pi], ids);
@@ -537,13 +538,15 @@ __global__ void __launch_bounds__(256) k_energy_fem(int nt, const uint32_t* __re
 }
 
 __global__ void __launch_bounds__(256) k_energy_barrier(const unsigned* __restrict__ count,
+                                                        int pair_cap,
                                                         const int4* __restrict__ pairs,
                                                         const double* __restrict__ x,
                                                         double dHat, double kappa, double* out)
 {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     double   e = 0.0;
-    if(i < *count)
+    unsigned np = *count;
+    if(i < (np < (unsigned)pair_cap ? np : (unsigned)pair_cap))
     {
         int    ids[4];
         int    type = decode_pair(pairs[i], ids);
@@ -639,7 +642,7 @@ __global__ void k_update_velocity(int nv, const int* __restrict__ btype, const d
         double vv = fr ? (x[k] - xprev[k]) * (1 / dt) : 0.0;
         v[k]      = vv;
         xprev[k]  = x[k];
-        xt[k]     = xprev[k] + (vv * dt + (fr ? gg[c] * dt * dt : 0.0));
+        xt[k]     = xprev[k] + (vv * dt + (fr ? gg[c] * (dt * dt) : 0.0));
     }
 }
 
@@ -686,7 +689,7 @@ void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontac
     if(d.pair_cap == 0)
         return;
     hipLaunchKernelGGL(k_barrier, dim3(grid_for(d.pair_cap, 128)), dim3(128), 0, s, d.pcount,
-                       d.pairs, d.x, d.dHat, kappa, d.btype, d.rowptr, d.cols, d.vals, d.coo_r,
+                       d.pair_cap, d.pairs, d.x, d.dHat, kappa, d.btype, d.rowptr, d.cols, d.vals, d.coo_r,
                        d.coo_c, d.coo_v, d.coo_count, d.coo_cap, d.overflow, gcontact,
                        hessian ? 1 : 0);
     SG_LAUNCH_CHECK();
@@ -721,7 +724,8 @@ void launch_energy(const SimDev& d, double kappa, double* partial, double* terms
         hipLaunchKernelGGL(k_energy_fem, dim3(L.n[1]), dim3(256), 0, s, d.nt, d.tets, d.dminv,
                            d.vol, d.mu, d.lam, d.x, d.dt * d.dt, partial + L.off[1]);
     if(L.n[2])
-        hipLaunchKernelGGL(k_energy_barrier, dim3(L.n[2]), dim3(256), 0, s, d.pcount, d.pairs,
+        hipLaunchKernelGGL(k_energy_barrier, dim3(L.n[2]), dim3(256), 0, s, d.pcount,
+                           d.pair_cap, d.pairs,
                            d.x, d.dHat, kappa, partial + L.off[2]);
     if(L.n[3])
         hipLaunchKernelGGL(k_energy_ground, dim3(L.n[3]), dim3(256), 0, s, d.gcount, d.gp, d.x,

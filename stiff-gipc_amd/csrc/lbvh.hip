@@ -1,0 +1,466 @@
+// lbvh.hip -- linear BVH over collision faces / edges (lbvh_f / lbvh_e).
+//
+// Construction follows StiffGIPC/mlbvh.cu:2074-2157 (Karras 2012):
+//   leaf AABBs (:1149-1204), scene box (calcMaxBV :1745), keys
+//   morton30(centre) << 32 | index (:1208-1227), sort, leaf/internal nodes
+//   (:1229-1269, determine_range :135, find_split :198), bottom-up refit (:1271).
+// Built with -ffp-contract=off: keys, node arrays and boxes are bit-identical
+// to the CPU restatement.
+//
+// The key sort is this file's own LSD radix sort on 8-bit digits (keys only:
+// the primitive index is the key's low word).  Each 256-thread block ranks
+// its 2048-key tile stably with wave64 multi-split: eight ballots give every
+// lane the mask of same-digit peers, per-wave digit counts are combined in
+// LDS.  Digits above the index width and above the 30-bit Morton code are
+// skipped, so a 50k-primitive tree needs 6 passes.
+#include "sim_kernels.h"
+
+namespace sgipc
+{
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems   = 8;
+constexpr int kSortTile    = kSortThreads * kSortItems;
+
+__global__ void k_leaf_aabb(int n, int arity, const uint32_t* __restrict__ prims,
+                            const double* __restrict__ x, const double* __restrict__ dx,
+                            double alpha, double* __restrict__ leaf)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= n)
+        return;
+    double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
+    for(int k = 0; k < arity; k++)
+    {
+        uint32_t v = prims[(size_t)arity * e + k];
+        for(int c = 0; c < 3; c++)
+        {
+            double p = x[3 * (size_t)v + c];
+            lo[c]    = (p < lo[c]) ? p : lo[c];
+            up[c]    = (up[c] < p) ? p : up[c];
+            if(dx)
+            {
+                double q = p - dx[3 * (size_t)v + c] * alpha;
+                lo[c]    = (q < lo[c]) ? q : lo[c];
+                up[c]    = (up[c] < q) ? q : up[c];
+            }
+        }
+    }
+    for(int c = 0; c < 3; c++)
+    {
+        leaf[6 * (size_t)e + c]     = up[c];
+        leaf[6 * (size_t)e + 3 + c] = lo[c];
+    }
+}
+
+// min/max reduction of the leaf boxes: per-block partials then one block
+__global__ void __launch_bounds__(256) k_box_partial(int n, const double* __restrict__ leaf,
+                                                     double* __restrict__ part)
+{
+    __shared__ double s[6][4];
+    int    e = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6];
+    for(int c = 0; c < 3; c++)
+    {
+        v[c]     = (e < n) ? leaf[6 * (size_t)e + c] : -1e32;
+        v[3 + c] = (e < n) ? leaf[6 * (size_t)e + 3 + c] : 1e32;
+    }
+    for(int c = 0; c < 3; c++)
+    {
+        v[c]     = wave_max(v[c]);
+        v[3 + c] = wave_min(v[3 + c]);
+    }
+    if(__lane_id() == 0)
+        for(int c = 0; c < 6; c++)
+            s[c][threadIdx.x >> 6] = v[c];
+    __syncthreads();
+    if(threadIdx.x < 6)
+    {
+        int    c = threadIdx.x;
+        double r = s[c][0];
+        for(int w = 1; w < 4; w++)
+            r = (c < 3) ? ((r < s[c][w]) ? s[c][w] : r) : ((s[c][w] < r) ? s[c][w] : r);
+        part[6 * (size_t)blockIdx.x + c] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_box_final(int nb, const double* __restrict__ part,
+                                                   double* __restrict__ scene)
+{
+    __shared__ double s[6][4];
+    double v[6] = {-1e32, -1e32, -1e32, 1e32, 1e32, 1e32};
+    for(int i = threadIdx.x; i < nb; i += blockDim.x)
+        for(int c = 0; c < 6; c++)
+        {
+            double p = part[6 * (size_t)i + c];
+            v[c]     = (c < 3) ? ((v[c] < p) ? p : v[c]) : ((p < v[c]) ? p : v[c]);
+        }
+    for(int c = 0; c < 3; c++)
+    {
+        v[c]     = wave_max(v[c]);
+        v[3 + c] = wave_min(v[3 + c]);
+    }
+    if(__lane_id() == 0)
+        for(int c = 0; c < 6; c++)
+            s[c][threadIdx.x >> 6] = v[c];
+    __syncthreads();
+    if(threadIdx.x < 6)
+    {
+        int    c = threadIdx.x;
+        double r = s[c][0];
+        for(int w = 1; w < 4; w++)
+            r = (c < 3) ? ((r < s[c][w]) ? s[c][w] : r) : ((s[c][w] < r) ? s[c][w] : r);
+        scene[c] = r;
+    }
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ double clamp_ref(double v, double lo, double hi)
+{
+    double a = (v < lo) ? lo : v;  // std::max(v, lo)
+    return (hi < a) ? hi : a;      // std::min(a, hi)
+}
+
+__device__ __forceinline__ uint32_t morton_code(double x, double y, double z)
+{
+    const double res = 1024.0;
+    x                = clamp_ref(x * res, 0.0, res - 1.0);
+    y                = clamp_ref(y * res, 0.0, res - 1.0);
+    z                = clamp_ref(z * res, 0.0, res - 1.0);
+    return (expand_bits((uint32_t)x) << 2) + (expand_bits((uint32_t)y) << 1)
+           + expand_bits((uint32_t)z);
+}
+
+__global__ void k_morton(int n, const double* __restrict__ leaf, const double* __restrict__ scene,
+                         uint64_t* __restrict__ keys)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= n)
+        return;
+    double lo[3] = {scene[3], scene[4], scene[5]};
+    double sz[3] = {scene[0] - lo[0], scene[1] - lo[1], scene[2] - lo[2]};
+    double c[3];
+    for(int k = 0; k < 3; k++)
+        c[k] = (leaf[6 * (size_t)e + k] + leaf[6 * (size_t)e + 3 + k]) * 0.5;
+    uint64_t mc = morton_code((c[0] - lo[0]) / sz[0], (c[1] - lo[1]) / sz[1], (c[2] - lo[2]) / sz[2]);
+    keys[e]     = (mc << 32) | (uint64_t)(uint32_t)e;
+}
+
+// ------------------------------------------------------------ radix sort
+__global__ void __launch_bounds__(kSortThreads) k_sort_hist(int n, int shift,
+                                                            const uint64_t* __restrict__ keys,
+                                                            unsigned* __restrict__ hist, int nblk)
+{
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kSortTile;
+    for(int j = 0; j < kSortItems; j++)
+    {
+        int i = base + j * kSortThreads + threadIdx.x;
+        if(i < n)
+            atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of hist (digit-major, 256 * nblk entries), one workgroup
+__global__ void __launch_bounds__(1024) k_sort_scan(int total, unsigned* __restrict__ hist)
+{
+    __shared__ unsigned wsum[16];
+    __shared__ unsigned carry_s;
+    if(threadIdx.x == 0)
+        carry_s = 0;
+    __syncthreads();
+    for(int base = 0; base < total; base += 1024)
+    {
+        int      i = base + threadIdx.x;
+        unsigned v = (i < total) ? hist[i] : 0;
+        unsigned s = v;
+        for(int o = 1; o < 64; o <<= 1)
+        {
+            unsigned u = __shfl_up(s, o, 64);
+            if((int)__lane_id() >= o)
+                s += u;
+        }
+        int w = threadIdx.x >> 6;
+        if(__lane_id() == 63)
+            wsum[w] = s;
+        __syncthreads();
+        if(threadIdx.x < 16)
+        {
+            unsigned ws = wsum[threadIdx.x];
+            for(int o = 1; o < 16; o <<= 1)
+            {
+                unsigned u = __shfl_up(ws, o, 16);
+                if((int)threadIdx.x >= o)
+                    ws += u;
+            }
+            wsum[threadIdx.x] = ws;
+        }
+        __syncthreads();
+        unsigned excl = s - v + (w > 0 ? wsum[w - 1] : 0) + carry_s;
+        if(i < total)
+            hist[i] = excl;
+        __syncthreads();
+        if(threadIdx.x == 1023)
+            carry_s = excl + v;
+        __syncthreads();
+    }
+}
+
+// stable scatter: keys are visited in tile order (round j, then thread)
+__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(int n, int shift,
+                                                               const uint64_t* __restrict__ in,
+                                                               uint64_t* __restrict__ out,
+                                                               const unsigned* __restrict__ hist,
+                                                               int nblk)
+{
+    __shared__ unsigned base_d[256];
+    __shared__ unsigned wcnt[4][256];
+    const int lane = __lane_id();
+    const int w    = threadIdx.x >> 6;
+    base_d[threadIdx.x] = hist[(size_t)threadIdx.x * nblk + blockIdx.x];
+    const int tile      = blockIdx.x * kSortTile;
+    for(int j = 0; j < kSortItems; j++)
+    {
+        for(int q = 0; q < 4; q++)
+            wcnt[q][threadIdx.x] = 0;
+        __syncthreads();
+        int      i    = tile + j * kSortThreads + threadIdx.x;
+        bool     live = i < n;
+        uint64_t k    = live ? in[i] : 0;
+        unsigned d    = (unsigned)((k >> shift) & 0xFF);
+        unsigned long long peers = __ballot(live);
+#pragma unroll
+        for(int b = 0; b < 8; b++)
+        {
+            unsigned long long m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        unsigned long long lt   = (lane == 0) ? 0ull : (peers & ((~0ull) >> (64 - lane)));
+        unsigned           rank = (unsigned)__popcll(lt);
+        if(live && rank == 0)
+            wcnt[w][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if(live)
+        {
+            unsigned off = base_d[d] + rank;
+            for(int q = 0; q < w; q++)
+                off += wcnt[q][d];
+            out[off] = k;
+        }
+        __syncthreads();
+        base_d[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x]
+                               + wcnt[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------- Karras build
+__device__ __forceinline__ int cub64(uint64_t a, uint64_t b) { return __clzll(a ^ b); }
+
+__global__ void k_leaf_nodes(int n, const uint64_t* __restrict__ keys, uint32_t* __restrict__ nodes,
+                             const double* __restrict__ leaf, double* __restrict__ bvs,
+                             unsigned* __restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    if(i < n - 1)
+    {
+        uint32_t* nd = nodes + 4 * (size_t)i;
+        nd[0] = nd[1] = nd[2] = nd[3] = 0xFFFFFFFFu;
+        flags[i]                      = 0u;
+    }
+    uint32_t  el = (uint32_t)(keys[i] & 0xFFFFFFFFull);
+    uint32_t* nd = nodes + 4 * (size_t)(i + n - 1);
+    nd[0] = nd[1] = nd[2] = 0xFFFFFFFFu;
+    nd[3]                 = el;
+    for(int c = 0; c < 6; c++)
+        bvs[6 * (size_t)(i + n - 1) + c] = leaf[6 * (size_t)el + c];
+}
+
+__global__ void k_internal_nodes(int n, const uint64_t* __restrict__ code, uint32_t* __restrict__ nodes)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if(idx >= n - 1)
+        return;
+    int first, last;
+    if(idx == 0)
+    {
+        first = 0;
+        last  = n - 1;
+    }
+    else
+    {
+        uint64_t self = code[idx];
+        int      Ld   = cub64(self, code[idx - 1]);
+        int      Rd   = cub64(self, code[idx + 1]);
+        int      d    = (Rd > Ld) ? 1 : -1;
+        int      dmin = Ld < Rd ? Ld : Rd;
+        int      lmax = 2;
+        int      delta = -1;
+        int      it    = idx + d * lmax;
+        if(0 <= it && it < n)
+            delta = cub64(self, code[it]);
+        while(delta > dmin)
+        {
+            lmax <<= 1;
+            it    = idx + d * lmax;
+            delta = -1;
+            if(0 <= it && it < n)
+                delta = cub64(self, code[it]);
+        }
+        int l = 0;
+        for(int t = lmax >> 1; t > 0; t >>= 1)
+        {
+            it    = idx + (l + t) * d;
+            delta = -1;
+            if(0 <= it && it < n)
+                delta = cub64(self, code[it]);
+            if(delta > dmin)
+                l += t;
+        }
+        int j = idx + l * d;
+        first = d < 0 ? j : idx;
+        last  = d < 0 ? idx : j;
+    }
+    int gamma;
+    {
+        uint64_t fc = code[first], lc = code[last];
+        if(fc == lc)
+            gamma = (first + last) >> 1;
+        else
+        {
+            int dn     = cub64(fc, lc);
+            int split  = first;
+            int stride = last - first;
+            do
+            {
+                stride     = (stride + 1) >> 1;
+                int middle = split + stride;
+                if(middle < last && cub64(fc, code[middle]) > dn)
+                    split = middle;
+            } while(stride > 1);
+            gamma = split;
+        }
+    }
+    uint32_t L = (uint32_t)gamma, R = (uint32_t)gamma + 1;
+    if(min(first, last) == gamma)
+        L += n - 1;
+    if(max(first, last) == gamma + 1)
+        R += n - 1;
+    nodes[4 * (size_t)idx + 1] = L;
+    nodes[4 * (size_t)idx + 2] = R;
+    nodes[4 * (size_t)L]       = (uint32_t)idx;
+    nodes[4 * (size_t)R]       = (uint32_t)idx;
+}
+
+// bottom-up refit: the second child to arrive merges (coherent loads)
+__global__ void k_refit(int n, const uint32_t* __restrict__ nodes, double* bvs, unsigned* flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    uint32_t p = nodes[4 * (size_t)(i + n - 1)];
+    while(p != 0xFFFFFFFFu)
+    {
+        __threadfence();
+        if(atomicAdd(flags + p, 1u) == 0u)
+            return;
+        __threadfence();
+        uint32_t        L  = nodes[4 * (size_t)p + 1], R = nodes[4 * (size_t)p + 2];
+        volatile double* vb = bvs;
+        double           o[6];
+        for(int c = 0; c < 3; c++)
+        {
+            double a = vb[6 * (size_t)L + c], b = vb[6 * (size_t)R + c];
+            o[c]     = (a < b) ? b : a;
+            a        = vb[6 * (size_t)L + 3 + c];
+            b        = vb[6 * (size_t)R + 3 + c];
+            o[3 + c] = (b < a) ? b : a;
+        }
+        for(int c = 0; c < 6; c++)
+            vb[6 * (size_t)p + c] = o[c];
+        p = nodes[4 * (size_t)p];
+    }
+}
+
+void Lbvh::init(int n_, int arity_, const uint32_t* prims_)
+{
+    n     = n_;
+    arity = arity_;
+    prims = prims_;
+    if(n <= 0)
+        return;
+    leaf.resize(6 * (size_t)n);
+    bvs.resize(6 * (size_t)(2 * n - 1));
+    nodes.resize(4 * (size_t)(2 * n - 1));
+    keys.resize(n);
+    keys_alt.resize(n);
+    flags.resize(n);
+    int nblk = (n + kSortTile - 1) / kSortTile;
+    hist.resize((size_t)256 * nblk);
+    scene.resize(6);
+    partial.resize(6 * (size_t)grid_for(n, 256));
+}
+
+void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
+{
+    if(n <= 0)
+        return;
+    hipLaunchKernelGGL(k_leaf_aabb, dim3(grid_for(n, 256)), dim3(256), 0, s, n, arity, prims, x,
+                       dx, alpha, leaf.data());
+    unsigned nb = grid_for(n, 256);
+    hipLaunchKernelGGL(k_box_partial, dim3(nb), dim3(256), 0, s, n, leaf.data(), partial.data());
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, s, (int)nb, partial.data(), scene.data());
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, s, n, leaf.data(),
+                       scene.data(), keys.data());
+    SG_LAUNCH_CHECK();
+    // radix passes: index bits [0, ib), morton bits [32, 62)
+    int ib = 0;
+    while((1ll << ib) < n)
+        ib++;
+    int nblk = (n + kSortTile - 1) / kSortTile;
+    uint64_t* a = keys.data();
+    uint64_t* b = keys_alt.data();
+    std::vector<int> shifts;
+    for(int sh = 0; sh < ib; sh += 8)
+        shifts.push_back(sh);
+    for(int sh = 32; sh < 62; sh += 8)
+        shifts.push_back(sh);
+    for(int sh : shifts)
+    {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a,
+                           hist.data(), nblk);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist.data());
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
+                           hist.data(), nblk);
+        SG_LAUNCH_CHECK();
+        std::swap(a, b);
+    }
+    if(a != keys.data())
+        SG_HIP(hipMemcpyAsync(keys.data(), a, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_leaf_nodes, dim3(grid_for(n, 256)), dim3(256), 0, s, n, keys.data(),
+                       nodes.data(), leaf.data(), bvs.data(), flags.data());
+    if(n > 1)
+    {
+        hipLaunchKernelGGL(k_internal_nodes, dim3(grid_for(n - 1, 256)), dim3(256), 0, s, n,
+                           keys.data(), nodes.data());
+        hipLaunchKernelGGL(k_refit, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nodes.data(),
+                           bvs.data(), flags.data());
+    }
+    SG_LAUNCH_CHECK();
+}
+
+}  // namespace sgipc

@@ -111,11 +111,13 @@ __global__ void __launch_bounds__(256) k_spmv_csr(int nb, const int* __restrict_
 }
 
 // COO tail: y[r] += B x[c]  (sign = -1 for residuals)
-__global__ void k_spmv_coo(const unsigned* __restrict__ count, const int* __restrict__ r,
-                           const int* __restrict__ c, const double* __restrict__ v,
-                           const double* __restrict__ x, double* y, double* dot, double sign)
+__global__ void k_spmv_coo(const unsigned* __restrict__ count, unsigned cap,
+                           const int* __restrict__ r, const int* __restrict__ c,
+                           const double* __restrict__ v, const double* __restrict__ x, double* y,
+                           double* dot, double sign)
 {
     unsigned n = *count;
+    n          = n < cap ? n : cap;
     unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double   d = 0.0;
     if(t < n)
@@ -147,8 +149,8 @@ static void launch_coo(const Coo* coo, const double* x, double* y, double* dot, 
         return;
     // the live count is device-resident; launch over the capacity
     hipLaunchKernelGGL(k_spmv_coo, dim3(grid_for(coo->cap, 256)), dim3(256), 0, s,
-                       coo->count.data(), coo->r.data(), coo->c.data(), coo->v.data(), x, y,
-                       dot, sign);
+                       coo->count.data(), (unsigned)coo->cap, coo->r.data(), coo->c.data(),
+                       coo->v.data(), x, y, dot, sign);
     SG_LAUNCH_CHECK();
 }
 

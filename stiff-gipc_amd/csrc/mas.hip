@@ -46,12 +46,13 @@ static int compute_num_levels(int nv)
 }
 
 // ------------------------------------------------------------ hierarchy
-__global__ void k_l0_mask_coo(const unsigned* __restrict__ count, const int* __restrict__ cr,
-                              const int* __restrict__ cc, const int* __restrict__ slot_of,
-                              unsigned* mask)
+__global__ void k_l0_mask_coo(const unsigned* __restrict__ count, unsigned cap,
+                              const int* __restrict__ cr, const int* __restrict__ cc,
+                              const int* __restrict__ slot_of, unsigned* mask)
 {
     unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= *count)
+    unsigned n = *count;
+    if(t >= (n < cap ? n : cap))
         return;
     int a = slot_of[cr[t]], b = slot_of[cc[t]];
     if(a != b && (a >> 4) == (b >> 4))
@@ -60,12 +61,13 @@ __global__ void k_l0_mask_coo(const unsigned* __restrict__ count, const int* __r
 
 // level l >= 1: mask bits from static cross-bank edges and COO entries
 __global__ void k_lx_mask(int nx, const int* __restrict__ xa, const int* __restrict__ xb,
-                          const unsigned* __restrict__ coo_count, const int* __restrict__ cr,
-                          const int* __restrict__ cc, const int* __restrict__ node_l,
-                          unsigned* mask)
+                          const unsigned* __restrict__ coo_count, int coo_cap,
+                          const int* __restrict__ cr, const int* __restrict__ cc,
+                          const int* __restrict__ node_l, unsigned* mask)
 {
     int t     = blockIdx.x * blockDim.x + threadIdx.x;
     int ncoo  = coo_count ? (int)*coo_count : 0;
+    ncoo      = ncoo < coo_cap ? ncoo : coo_cap;
     int u, v;
     if(t < nx)
     {
@@ -251,7 +253,7 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
                                                     const int* __restrict__ cols,
                                                     const double* __restrict__ vals,
                                                     const unsigned* __restrict__ coo_count,
-                                                    const int* __restrict__ cr,
+                                                    int coo_cap, const int* __restrict__ cr,
                                                     const int* __restrict__ cc,
                                                     const double* __restrict__ cv, int L, int nv,
                                                     const int* __restrict__ slot_of,
@@ -260,6 +262,7 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
 {
     int  t    = blockIdx.x * blockDim.x + threadIdx.x;
     int  ncoo = coo_count ? (int)*coo_count : 0;
+    ncoo      = ncoo < coo_cap ? ncoo : coo_cap;
     int  i = 0, j = 0;
     const double* blk = nullptr;
     bool live = true;
@@ -614,8 +617,8 @@ void Mas::build(const SysView& S, hipStream_t s)
     if(coo)
     {
         hipLaunchKernelGGL(k_l0_mask_coo, dim3(grid_for(coo_cap, 256)), dim3(256), 0, s,
-                           coo->count.data(), coo->r.data(), coo->c.data(), slot_of_.data(),
-                           mask_.data());
+                           coo->count.data(), (unsigned)coo_cap, coo->r.data(), coo->c.data(),
+                           slot_of_.data(), mask_.data());
         SG_LAUNCH_CHECK();
     }
     cluster_level(0, s);
@@ -626,7 +629,7 @@ void Mas::build(const SysView& S, hipStream_t s)
         if(ne > 0)
         {
             hipLaunchKernelGGL(k_lx_mask, dim3(grid_for(ne, 256)), dim3(256), 0, s, nx_,
-                               xa_.data(), xb_.data(), coo ? coo->count.data() : nullptr,
+                               xa_.data(), xb_.data(), coo ? coo->count.data() : nullptr, coo_cap,
                                coo ? coo->r.data() : nullptr, coo ? coo->c.data() : nullptr,
                                node_.data() + (size_t)l * nv_, mask_.data());
             SG_LAUNCH_CHECK();
@@ -670,8 +673,9 @@ void Mas::assemble(const SysView& S, hipStream_t s)
     long long n = (long long)A.nnzb + (coo ? coo->cap : 0);
     hipLaunchKernelGGL(k_mas_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, A.nnzb,
                        A.rows.data(), A.cols.data(), A.vals.data(),
-                       coo ? coo->count.data() : nullptr, coo ? coo->r.data() : nullptr,
-                       coo ? coo->c.data() : nullptr, coo ? coo->v.data() : nullptr, L_, nv_,
+                       coo ? coo->count.data() : nullptr, coo ? coo->cap : 0,
+                       coo ? coo->r.data() : nullptr, coo ? coo->c.data() : nullptr,
+                       coo ? coo->v.data() : nullptr, L_, nv_,
                        slot_of_.data(), chain_.data(), B_.data());
     SG_LAUNCH_CHECK();
 }

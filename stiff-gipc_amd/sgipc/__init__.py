@@ -1,6 +1,6 @@
 """sgipc -- host-side Python mirror of the libsgipc.so C ABI (MI355X / gfx950
 implementation of the Stiff-GIPC Newton-step hot path)."""
 from . import scene  # noqa: F401
-from ._lib import (LinearSystem, SgipcError, check, device_info, lib,  # noqa: F401
+from ._lib import (LinearSystem, Simulation, SgipcError, check, device_info, lib,  # noqa: F401
                    SGIPC_PRECOND_DIAG, SGIPC_PRECOND_MAS, SGIPC_MAS_FP32, SGIPC_MAS_FP64,
                    SGIPC_SOLVE_FP64, SGIPC_SOLVE_MIXED)

@@ -181,3 +181,99 @@ class LinearSystem:
         c = np.zeros(self.nb * (L - 1), np.int32)
         check(lib().sgipc_ls_mas_chain(self.h, P(c, C.c_int32)))
         return c.reshape(self.nb, L - 1)
+
+
+class Simulation:
+    """Device-resident Stiff-GIPC state for one scene (GIPC counterpart)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self._desc = scene.desc()
+        h = C.c_void_p()
+        check(lib().sgipc_create(C.byref(self._desc), C.byref(h)))
+        self.h = h
+        self.nv = scene.n_verts
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgipc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _get(self, f):
+        a = np.zeros(3 * self.nv)
+        check(f(self.h, P(a)))
+        return a.reshape(-1, 3)
+
+    def positions(self):
+        return self._get(lib().sgipc_get_positions)
+
+    def velocities(self):
+        return self._get(lib().sgipc_get_velocities)
+
+    def xtilde(self):
+        return self._get(lib().sgipc_get_xtilde)
+
+    def set_positions(self, x):
+        check(lib().sgipc_set_positions(self.h, P(np.ascontiguousarray(x, np.float64).ravel())))
+
+    def set_xtilde(self, xt):
+        check(lib().sgipc_set_xtilde(self.h, P(np.ascontiguousarray(xt, np.float64).ravel())))
+
+    @property
+    def kappa(self):
+        k = C.c_double()
+        check(lib().sgipc_get_kappa(self.h, C.byref(k)))
+        return k.value
+
+    @kappa.setter
+    def kappa(self, k):
+        check(lib().sgipc_set_kappa(self.h, float(k)))
+
+    def lbvh(self, which):
+        n = self.scene.faces.shape[0] if which == 0 else self.scene.edges.shape[0]
+        keys = np.zeros(n, np.uint64)
+        nodes = np.zeros((2 * n - 1, 4), np.uint32)
+        aabbs = np.zeros((2 * n - 1, 6), np.float64)
+        check(lib().sgipc_lbvh_build(self.h, which, P(keys, C.c_uint64), P(nodes, C.c_uint32),
+                                     P(aabbs)))
+        return keys, nodes, aabbs
+
+    def contacts(self, cap=None):
+        cap = cap or max(1000, 4 * (self.scene.surf_verts.size + self.scene.edges.shape[0]))
+        pairs = np.zeros((cap, 4), np.int32)
+        counts = np.zeros(5, np.int32)
+        gcap = max(1, self.scene.surf_verts.size)
+        ground = np.zeros(gcap, np.uint32)
+        ng = C.c_int32()
+        check(lib().sgipc_contact_query(self.h, P(pairs, C.c_int32), cap, P(counts, C.c_int32),
+                                        P(ground, C.c_uint32), gcap, C.byref(ng)))
+        return pairs[: counts[0]].copy(), counts, ground[: ng.value].copy()
+
+    def assemble(self):
+        g = np.zeros(3 * self.nv)
+        nnz = C.c_int32()
+        check(lib().sgipc_assemble(self.h, P(g), C.byref(nnz), None, None, None, 0))
+        rowptr = np.zeros(self.nv + 1, np.int32)
+        cols = np.zeros(nnz.value, np.int32)
+        vals = np.zeros((nnz.value, 3, 3))
+        check(lib().sgipc_assemble(self.h, P(g), C.byref(nnz), P(rowptr, C.c_int32),
+                                   P(cols, C.c_int32), P(vals), nnz.value))
+        return g, rowptr, cols, vals
+
+    def energy(self):
+        t = np.zeros(8)
+        check(lib().sgipc_energy(self.h, P(t)))
+        return t
+
+    def newton_iteration(self):
+        st = IterStats()
+        check(lib().sgipc_newton_iteration(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def time_step(self):
+        st = StepStats()
+        check(lib().sgipc_time_step(self.h, C.byref(st)))
+        return dict(newton_iterations=st.newton_iterations, pcg_iterations=st.pcg_iterations,
+                    ms_total=st.ms_total)

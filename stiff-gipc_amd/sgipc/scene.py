@@ -316,3 +316,24 @@ def cubes_scene(n_side=4, cells=8, size=0.2, spacing=0.6, fall=0.5, seed=0, nu=0
 
 def _diag(x):
     return float(np.linalg.norm(x.max(0) - x.min(0)))
+
+
+def stack_scene(n=2, cells=3, size=0.1, gap=0.5e-3, rot_deg=7.0, seed=0, E=1e6, nu=0.4,
+                rho=1000.0, **kw):
+    """n small tet cubes stacked on the ground, each rotated by rot_deg about
+    the vertical axis (alternating sign) with a gap of 'gap' metres between
+    neighbours and to the ground: contacts (ground, PT, EE) from step 0."""
+    x0, t0, p0 = hex_grid((cells,) * 3, size / cells)
+    x0 = x0 - x0.mean(0)
+    bodies = []
+    y = gap + size / 2
+    for k in range(n):
+        a = np.deg2rad(rot_deg * (1 if k % 2 == 0 else -1) * (k > 0))
+        R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+        bodies.append((x0 @ R.T + np.array([0.0, y, 0.0]), t0, p0, E, nu, rho, 0))
+        y += size + gap
+    defaults = dict(dt=0.01, dhat=1e-3, name=f"stack{n}x{cells}^3")
+    defaults.update(kw)
+    sc = assemble_bodies(bodies, **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
