@@ -1,0 +1,181 @@
+"""Benchmark: ms per Newton iteration of the Stiff-GIPC hot path on one MI355X.
+
+Workload (the metric's mid configuration): 64 tet cubes of 8^3 hex cells
+(3072 tets each, 196,608 tets, 46,656 vertices) on a 4x4x4 grid with seeded
++/-15% jitter and random rotations, Young's modulus log-uniform in
+[1e5, 1e9] Pa, nu = 0.4, dropped 0.5 m onto the ground; dt = 0.01 s,
+dhat = 1e-3 m, MAS-PCG to 1e-4 relative (preconditioned) residual.
+
+A bench "step" is one Newton iteration that solves (assembly, MAS build,
+PCG, CCD bounds, line search) of the running time integration.  The scene is
+first integrated for --settle time steps (untimed) so the timed iterations
+include ground and cube-cube contacts.  Mode: FP64 assembly/SpMV/PCG with the
+reference's FP32 MAS apply (the reference's "plain FP64" path).
+
+Prints ONE JSON line on rank 0.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "stiff-gipc_amd"))
+
+METRIC = "ms/Newton step + PCG iters at 1e-4 res: plain FP64 MAS vs FP32+learned"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(name, sz, nparts):
+    """Bytes one launch must move at minimum (each operand read or written once)."""
+    nv, nt, nnzb, ncoo = sz["n_verts"], sz["n_tets"], sz["nnzb"], sz["coo_blocks"]
+    if name == "spmv":
+        # static BSR: 72 B block + 4 B column; rowptr; x read, y write (24 B each);
+        # COO tail: block + 2 ids + x gather + y read-modify-write
+        return nnzb * 76 + (nv + 1) * 4 + nv * 48 + ncoo * (72 + 8 + 24 + 48)
+    if name == "mas_apply_l0":
+        # FP32 48x48 inverse per level-0 bank; per dof: read x, p, Ap, r, write
+        # x, r (FP64) and Z (FP32); slot map
+        return nparts * (48 * 48 * 4 + 16 * 4) + 3 * nv * (6 * 8 + 4)
+    if name == "fem":
+        # tet ids, Dm^-1, vol/mu/lambda, 12 coordinates, 16 slot ids,
+        # 16 blocks accumulated, 12 gradient entries accumulated
+        return nt * (16 + 72 + 24 + 96 + 64 + 16 * 72 + 12 * 8)
+    return 0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed Newton iterations")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed Newton iterations")
+    ap.add_argument("--settle", type=int, default=40, help="untimed time steps before warmup")
+    ap.add_argument("--n-side", type=int, default=4)
+    ap.add_argument("--cells", type=int, default=8)
+    ap.add_argument("--mas-fp64", action="store_true", help="all-FP64 MAS instead of FP32 apply")
+    ap.add_argument("--cpu-iters", type=int, default=1, help="oracle Newton iterations timed")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU decomposition is design-only", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import sgipc
+    from sgipc import scene as S
+
+    log("device:", sgipc.device_info())
+    sc = S.cubes_scene(n_side=args.n_side, cells=args.cells, size=0.2, spacing=0.6, fall=0.5,
+                       seed=0)
+    sc.mas_precision = sgipc.SGIPC_MAS_FP64 if args.mas_fp64 else sgipc.SGIPC_MAS_FP32
+    sc.pcg_rel_tol = 1e-4
+    nparts = int(sc.part_id.max()) + 1
+    log(f"scene: {sc.n_verts} verts, {sc.n_tets} tets, {sc.faces.shape[0]} faces, "
+        f"{sc.edges.shape[0]} edges, {nparts} banks")
+    t0 = time.time()
+    sim = sgipc.Simulation(sc)
+    log(f"create: {time.time() - t0:.2f} s")
+
+    # ---- settle (untimed): integrate until the lowest layer is in contact
+    t0 = time.time()
+    for k in range(args.settle):
+        st = sim.time_step()
+        if k % 10 == 9 or k == args.settle - 1:
+            x = sim.positions()
+            log(f"settle step {k + 1}/{args.settle}: newton={st['newton_iterations']} "
+                f"pcg={st['pcg_iterations']} ymin={x[:, 1].min():.4f} "
+                f"({time.time() - t0:.1f} s)")
+    state = (sim.positions(), sim.velocities(), sim.xtilde(), sim.kappa)
+
+    # ---- warmup + timed Newton iterations
+    if args.warmup:
+        sim.advance(args.warmup)
+    sim.profile(True)
+    t0 = time.perf_counter()
+    adv = sim.advance(args.steps)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    sz = sim.sizes()
+    log(f"timed: {args.steps} Newton iterations in {(t1 - t0):.3f} s -> {ms:.3f} ms/iter; "
+        f"pcg/iter={adv['pcg_iterations'] / args.steps:.1f} "
+        f"max_rel_res={adv['max_pcg_rel_residual']:.2e} steps_begun={adv['time_steps_begun']} "
+        f"sizes={sz}")
+
+    # ---- roofline of the dominant profiled kernel (HIP events on the library stream)
+    kern = {}
+    for name in ("spmv", "mas_apply_l0", "fem", "mas_invert"):
+        n, tot = sim.profile_query(name)
+        if n:
+            kern[name] = (n, tot)
+            log(f"kernel {name}: {n} launches, {tot:.3f} ms total, {tot / n * 1e3:.2f} us avg")
+    hbm_kernels = {k: v for k, v in kern.items() if k in ("spmv", "mas_apply_l0", "fem")}
+    dom = max(hbm_kernels, key=lambda k: hbm_kernels[k][1]) if hbm_kernels else None
+    roof = None
+    if dom:
+        n, tot = kern[dom]
+        avg_s = tot / n * 1e-3
+        byt = algorithmic_bytes(dom, sz, nparts)
+        ach = byt / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": int(byt), "avg_launch_us": round(avg_s * 1e6, 3),
+                "launches": n}
+
+    # ---- CPU baseline: the oracle (single thread) from the same settled state
+    cpu = None
+    if not args.no_cpu and args.cpu_iters > 0:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as O
+            w = O.World(sc)
+            w.set_state(*state)
+            c0 = time.perf_counter()
+            w.advance(args.cpu_iters)
+            c1 = time.perf_counter()
+            cms = (c1 - c0) * 1e3 / args.cpu_iters
+            cpu = {"value": round(cms, 3), "unit": "ms/Newton step", "cores": 1, "kind": "port",
+                   "sample": f"{args.cpu_iters} Newton iterations of the same 64-cube scene "
+                             f"from the settled state (oracle, FP64, single thread)"}
+            log(f"cpu baseline: {cms:.1f} ms/iter")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e!r}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms/Newton step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
+        "max_pcg_rel_residual": adv["max_pcg_rel_residual"],
+        "config": {
+            "workload": f"{args.n_side ** 3} tet cubes of {args.cells}^3 hex cells "
+                        f"({sc.n_tets} tets, {sc.n_verts} verts), E log-uniform [1e5,1e9] Pa, "
+                        f"nu 0.4, 0.5 m drop onto ground, dt 0.01, dhat 1e-3, "
+                        f"{args.settle} settle steps",
+            "mode": "FP64 PCG/SpMV, " + ("FP64" if args.mas_fp64 else "FP32") + " MAS apply",
+            "pcg_rel_tol": 1e-4,
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

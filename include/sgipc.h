@@ -167,6 +167,32 @@ typedef struct sgipc_step_stats {
  * velocity update, next x_tilde. */
 int sgipc_time_step(sgipc_t ctx, sgipc_step_stats* stats);
 
+typedef struct sgipc_advance_stats {
+    int32_t newton_iterations; /* Newton solves performed (== n requested)  */
+    int32_t time_steps_begun;
+    int32_t pcg_iterations;
+    int32_t pad;
+    double  max_pcg_rel_residual;
+    double  ms_total;
+} sgipc_advance_stats;
+
+/* Advance the integration by exactly n Newton iterations that solve (the
+ * stop test's final assemble-and-check pass of a time step is not counted),
+ * ending and beginning time steps as IPC_Solver does (GIPC.cu:10940-11131).
+ * Used to time the hot path per Newton iteration. */
+int sgipc_advance(sgipc_t ctx, int32_t n, sgipc_advance_stats* stats);
+
+/* Per-kernel device timing with HIP events on the library stream.
+ * Names: "spmv" (PCG A p), "mas_apply_l0", "fem", "mas_invert".
+ * enable = 1 clears and starts collection; query returns the number of
+ * timed launches and their summed duration. */
+int sgipc_profile_enable(sgipc_t ctx, int32_t enable);
+int sgipc_profile_query(sgipc_t ctx, const char* kernel, int32_t* launches,
+                        double* total_ms);
+/* problem sizes for algorithmic byte counts: out[8] = {n_verts, n_tets,
+ * nnzb static, coo blocks, mas banks, mas levels, self pairs, ground pairs} */
+int sgipc_problem_sizes(sgipc_t ctx, int64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

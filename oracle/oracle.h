@@ -158,6 +158,11 @@ int  orc_world_newton_iteration(orc_world* w, double* alpha_out,
                                 int* converged_out);
 int  orc_world_time_step(orc_world* w, int* newton_iters_out,
                          int* pcg_iters_out);
+/* state at a step boundary (x = x_prev) and kappa; rebuilds contacts */
+void orc_world_set_state(orc_world* w, const double* x, const double* v,
+                         const double* xt, double kappa);
+/* exactly n solving Newton iterations across step boundaries */
+int  orc_world_advance(orc_world* w, int n, int* steps_out, int* pcg_out);
 /* additive CCD time of impact (ACCD.cu:373-524), eta = 1 - slackness */
 double orc_pt_ccd(const double* p, const double* t0, const double* t1,
                   const double* t2, const double* dp, const double* dt0,

@@ -39,7 +39,11 @@ struct orc_world
     double *  gcontact, *gshape;
     int       newton_k;
     double    xdelta0;
+    int       in_step;
 };
+
+static void begin_step(orc_world* w);
+static void end_step(orc_world* w);
 
 static void* dup(const void* p, size_t bytes)
 {
@@ -1151,7 +1155,7 @@ int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_o
     return 0;
 }
 
-int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
+static void begin_step(orc_world* w)
 {
     upper_bound_kappa(w, &w->kappa);
     if(w->kappa < 1e-16)
@@ -1160,16 +1164,11 @@ int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
     memset(w->dx, 0, sizeof(double) * 3 * w->nv);
     w->newton_k = 0;
     w->xdelta0  = 0.0;
-    int total_pcg = 0, k = 0;
-    int cap = w->s.newton_max_iter > 0 ? w->s.newton_max_iter : 10000;
-    for(k = 0; k < cap; k++)
-    {
-        int it = 0, conv = 0;
-        orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
-        if(conv)
-            break;
-        total_pcg += it;
-    }
+    w->in_step  = 1;
+}
+
+static void end_step(orc_world* w)
+{
     for(int v = 0; v < w->nv; v++)
         for(int c = 0; c < 3; c++)
         {
@@ -1184,6 +1183,23 @@ int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
             w->xt[3 * v + c] = w->xprev[3 * v + c]
                                + (w->v[3 * v + c] * w->s.dt
                                   + (bt(w, v) == 0 ? w->s.gravity[c] * (w->s.dt * w->s.dt) : 0.0));
+    w->in_step = 0;
+}
+
+int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
+{
+    begin_step(w);
+    int total_pcg = 0, k = 0;
+    int cap = w->s.newton_max_iter > 0 ? w->s.newton_max_iter : 10000;
+    for(k = 0; k < cap; k++)
+    {
+        int it = 0, conv = 0;
+        orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
+        if(conv)
+            break;
+        total_pcg += it;
+    }
+    end_step(w);
     if(newton_iters_out)
         *newton_iters_out = k;
     if(pcg_iters_out)
@@ -1205,4 +1221,48 @@ double orc_ee_ccd(const double* a0, const double* a1, const double* b0, const do
                   double eta, double thickness)
 {
     return ee_ccd(V(a0), V(a1), V(b0), V(b1), V(da0), V(da1), V(db0), V(db1), eta, thickness);
+}
+
+/* state at a time-step boundary (x = x_prev), as exported by the GPU run */
+void orc_world_set_state(orc_world* w, const double* x, const double* v, const double* xt,
+                         double kappa)
+{
+    memcpy(w->x, x, sizeof(double) * 3 * w->nv);
+    memcpy(w->xprev, x, sizeof(double) * 3 * w->nv);
+    memcpy(w->v, v, sizeof(double) * 3 * w->nv);
+    memcpy(w->xt, xt, sizeof(double) * 3 * w->nv);
+    w->kappa    = kappa;
+    w->in_step  = 0;
+    build_cp(w);
+}
+
+/* exactly n Newton iterations that solve, across time-step boundaries */
+int orc_world_advance(orc_world* w, int n, int* steps_out, int* pcg_out)
+{
+    int done = 0, steps = 0, pcg = 0;
+    int cap  = w->s.newton_max_iter > 0 ? w->s.newton_max_iter : 10000;
+    while(done < n)
+    {
+        if(!w->in_step)
+        {
+            begin_step(w);
+            steps++;
+        }
+        int it = 0, conv = 0;
+        orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
+        if(conv)
+        {
+            end_step(w);
+            continue;
+        }
+        done++;
+        pcg += it;
+        if(w->newton_k >= cap)
+            end_step(w);
+    }
+    if(steps_out)
+        *steps_out = steps;
+    if(pcg_out)
+        *pcg_out = pcg;
+    return done;
 }

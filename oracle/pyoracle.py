@@ -105,6 +105,8 @@ def lib():
         L.orc_world_energy.argtypes = [C.c_void_p, dp]
         L.orc_world_newton_iteration.argtypes = [C.c_void_p, dp, ip, dp, ip]
         L.orc_world_time_step.argtypes = [C.c_void_p, ip, ip]
+        L.orc_world_set_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_double]
+        L.orc_world_advance.argtypes = [C.c_void_p, C.c_int, ip, ip]
         L.orc_pt_ccd.restype = C.c_double
         L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         L.orc_ee_ccd.restype = C.c_double
@@ -314,6 +316,16 @@ class World:
         cv = C.c_int32()
         lib().orc_world_newton_iteration(self.h, C.byref(a), C.byref(it), C.byref(xd), C.byref(cv))
         return dict(alpha=a.value, pcg_iterations=it.value, xdelta=xd.value, converged=cv.value)
+
+    def set_state(self, x, v, xt, kappa):
+        a = [np.ascontiguousarray(q, np.float64).ravel() for q in (x, v, xt)]
+        lib().orc_world_set_state(self.h, P(a[0]), P(a[1]), P(a[2]), float(kappa))
+
+    def advance(self, n):
+        st = C.c_int32()
+        it = C.c_int32()
+        done = lib().orc_world_advance(self.h, int(n), C.byref(st), C.byref(it))
+        return dict(newton_iterations=done, time_steps_begun=st.value, pcg_iterations=it.value)
 
     def time_step(self):
         n = C.c_int32()

@@ -356,4 +356,33 @@ int sgipc_time_step(sgipc_t h, sgipc_step_stats* st)
     return guard([&] { ctx_of(h).time_step(st); });
 }
 
+int sgipc_advance(sgipc_t h, int32_t n, sgipc_advance_stats* st)
+{
+    return guard([&] {
+        SG_CHECK(n >= 0, SGIPC_EINVAL, "n must be non-negative");
+        ctx_of(h).advance(n, st);
+    });
+}
+
+int sgipc_profile_enable(sgipc_t h, int32_t on)
+{
+    return guard([&] { ctx_of(h).profile_enable(on != 0); });
+}
+
+int sgipc_profile_query(sgipc_t h, const char* kernel, int32_t* launches, double* total_ms)
+{
+    return guard([&] {
+        SG_CHECK(kernel, SGIPC_EINVAL, "null kernel name");
+        ctx_of(h).profile_query(kernel, launches, total_ms);
+    });
+}
+
+int sgipc_problem_sizes(sgipc_t h, int64_t* out)
+{
+    return guard([&] {
+        SG_CHECK(out, SGIPC_EINVAL, "null output");
+        ctx_of(h).problem_sizes(out);
+    });
+}
+
 }  // extern "C"

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -132,6 +133,97 @@ class DBuf
     T*     p_   = nullptr;
     size_t n_   = 0;
     size_t cap_ = 0;
+};
+
+// ---------------------------------------------------------------- profiler
+// Event pairs recorded around selected launches on the launching stream;
+// resolved lazily (after the stream has been synchronised).
+class Profiler
+{
+  public:
+    bool enabled = false;
+    struct Entry
+    {
+        int    launches = 0;
+        double total_ms = 0.0;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    };
+    ~Profiler()
+    {
+        for(auto& kv : e_)
+            for(auto& p : kv.second.pending)
+                destroy(p);
+        for(auto& p : pool_)
+            destroy(p);
+    }
+    void begin(const char* name, hipStream_t s)
+    {
+        if(!enabled)
+            return;
+        auto  p   = get();
+        auto& ent = e_[name];
+        ent.pending.push_back(p);
+        SG_HIP(hipEventRecord(p.first, s));
+    }
+    void end(const char* name, hipStream_t s)
+    {
+        if(!enabled)
+            return;
+        auto& ent = e_[name];
+        SG_HIP(hipEventRecord(ent.pending.back().second, s));
+    }
+    // call after the stream has been synchronised
+    void resolve()
+    {
+        for(auto& kv : e_)
+        {
+            for(auto& p : kv.second.pending)
+            {
+                float ms = 0;
+                SG_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+                kv.second.total_ms += ms;
+                kv.second.launches++;
+                pool_.push_back(p);
+            }
+            kv.second.pending.clear();
+        }
+    }
+    void clear()
+    {
+        for(auto& kv : e_)
+        {
+            for(auto& p : kv.second.pending)
+                pool_.push_back(p);
+        }
+        e_.clear();
+    }
+    Entry query(const std::string& n)
+    {
+        auto it = e_.find(n);
+        return it == e_.end() ? Entry{} : Entry{it->second.launches, it->second.total_ms, {}};
+    }
+
+  private:
+    std::pair<hipEvent_t, hipEvent_t> get()
+    {
+        if(!pool_.empty())
+        {
+            auto p = pool_.back();
+            pool_.pop_back();
+            return p;
+        }
+        std::pair<hipEvent_t, hipEvent_t> p;
+        SG_HIP(hipEventCreate(&p.first));
+        SG_HIP(hipEventCreate(&p.second));
+        return p;
+    }
+    static void destroy(std::pair<hipEvent_t, hipEvent_t>& p)
+    {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    std::map<std::string, Entry>                   e_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool_;
 };
 
 // ---------------------------------------------------------------- launch

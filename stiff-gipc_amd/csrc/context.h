@@ -32,6 +32,10 @@ class Context
     void energy_terms(double* terms);
     void newton_iteration(sgipc_iter_stats* st);
     void time_step(sgipc_step_stats* st);
+    void advance(int n, sgipc_advance_stats* st);
+    void profile_enable(bool on);
+    void profile_query(const char* name, int32_t* launches, double* total_ms);
+    void problem_sizes(int64_t* out);
 
     DBuf<double> x_, v_;
 
@@ -51,6 +55,8 @@ class Context
     void   upper_bound_kappa(double& k);
     void   init_kappa();
     void   check_overflow();
+    void   begin_step();                   // IPC_Solver prologue (kappa)
+    void   end_step();                     // updateVelocities + computeXTilta
     double read_d(const double* p);
     unsigned read_u(const unsigned* p);
 
@@ -59,6 +65,8 @@ class Context
     int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0;
     double dHat_ = 0, kappa_ = 0, bbox_diag2_ = 0, scene_diag_ = 0, mean_mass_ = 0;
     int    newton_k_ = 0;
+    bool   in_step_  = false;
+    Profiler prof_;
     double xdelta0_  = 0;
 
     DBuf<double>   xr_, xprev_, xt_, mass_, dx_, xsave_, gc_, gs_, rhs_, scratch_;

@@ -247,6 +247,8 @@ Context::Context(const sgipc_scene_desc& d)
         mc.fp32 = d.mas_precision == SGIPC_MAS_FP32;
         mas_.init(A_, d.part_id, mc, s_);
     }
+    mas_.prof = &prof_;
+    pcg_.prof = &prof_;
     bf_.init(nf_, 3, faces_.data());
     be_.init(ne_, 2, edges_.data());
     build_bvh();
@@ -328,7 +330,9 @@ void Context::assemble(bool hessian)
     launch_init_diag(dev_, s_);
     launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
     launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
+    prof_.begin("fem", s_);
     launch_fem(dev_, hessian, s_);
+    prof_.end("fem", s_);
     launch_rhs(dev_, rhs_.data(), s_);
 }
 
@@ -577,8 +581,93 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     loc.ms_total       = now_ms(ev[0], ev[4]);
     for(auto& e : ev)
         (void)hipEventDestroy(e);
+    prof_.resolve();
     if(st)
         *st = loc;
+}
+
+void Context::begin_step()
+{
+    upper_bound_kappa(kappa_);
+    if(kappa_ < 1e-16)
+        suggest_kappa(kappa_);
+    init_kappa();
+    dx_.zero(s_);
+    newton_k_ = 0;
+    xdelta0_  = 0;
+    in_step_  = true;
+}
+
+void Context::end_step()
+{
+    launch_update_velocity(dev_, xprev_.data(), v_.data(), s_);
+    in_step_ = false;
+}
+
+void Context::advance(int n, sgipc_advance_stats* st)
+{
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, s_));
+    sgipc_advance_stats loc{};
+    int cap = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
+    while(loc.newton_iterations < n)
+    {
+        if(!in_step_)
+        {
+            begin_step();
+            loc.time_steps_begun++;
+        }
+        sgipc_iter_stats is{};
+        newton_iteration(&is);
+        if(is.converged)
+        {
+            end_step();
+            continue;
+        }
+        loc.newton_iterations++;
+        loc.pcg_iterations += is.pcg_iterations;
+        loc.max_pcg_rel_residual = std::max(loc.max_pcg_rel_residual, is.pcg_rel_residual);
+        if(newton_k_ >= cap)
+            end_step();
+    }
+    SG_HIP(hipEventRecord(e1, s_));
+    SG_HIP(hipStreamSynchronize(s_));
+    loc.ms_total = now_ms(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if(st)
+        *st = loc;
+}
+
+void Context::profile_enable(bool on)
+{
+    prof_.clear();
+    prof_.enabled = on;
+}
+
+void Context::profile_query(const char* name, int32_t* launches, double* total_ms)
+{
+    SG_HIP(hipStreamSynchronize(s_));
+    prof_.resolve();
+    auto e = prof_.query(name);
+    if(launches)
+        *launches = e.launches;
+    if(total_ms)
+        *total_ms = e.total_ms;
+}
+
+void Context::problem_sizes(int64_t* out)
+{
+    out[0] = nv_;
+    out[1] = nt_;
+    out[2] = A_.nnzb;
+    out[3] = (int64_t)read_u(coo_.count.data());
+    out[4] = mas_.ready() ? mas_.banks() : 0;
+    out[5] = mas_.ready() ? mas_.levels() : 0;
+    out[6] = (int64_t)read_u(pcount_.data());
+    out[7] = (int64_t)read_u(gcount_.data());
 }
 
 void Context::time_step(sgipc_step_stats* st)
@@ -587,14 +676,8 @@ void Context::time_step(sgipc_step_stats* st)
     SG_HIP(hipEventCreate(&e0));
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, s_));
-    upper_bound_kappa(kappa_);
-    if(kappa_ < 1e-16)
-        suggest_kappa(kappa_);
-    init_kappa();
-    dx_.zero(s_);
-    newton_k_ = 0;
-    xdelta0_  = 0;
-    int cap   = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
+    begin_step();
+    int cap = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
     int k = 0, pcg_total = 0;
     for(k = 0; k < cap; k++)
     {
@@ -604,7 +687,7 @@ void Context::time_step(sgipc_step_stats* st)
             break;
         pcg_total += is.pcg_iterations;
     }
-    launch_update_velocity(dev_, xprev_.data(), v_.data(), s_);
+    end_step();
     SG_HIP(hipEventRecord(e1, s_));
     SG_HIP(hipStreamSynchronize(s_));
     if(st)

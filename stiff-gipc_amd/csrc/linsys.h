@@ -92,6 +92,8 @@ class Mas
     const std::vector<int>& level_sizes() const { return h_lvl_; }  // (count, off) pairs
     std::vector<int> chain_host(hipStream_t s) const;
     bool ready() const { return built_; }
+    int  banks() const { return total_clusters_ / 16; }
+    Profiler* prof = nullptr;
 
   private:
     void cluster_level(int level, hipStream_t s);
@@ -156,6 +158,7 @@ class Pcg
     // block-Jacobi inverse of the diagonal blocks (DiagPreconditioner)
     void build_diag(const Bsr& A, hipStream_t s);
     void apply_diag(int nb, const double* r, double* z, hipStream_t s);
+    Profiler* prof = nullptr;
 
   private:
     void ensure(int n, int maxit);

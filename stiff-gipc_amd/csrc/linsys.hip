@@ -371,7 +371,11 @@ PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, dou
     double rel2 = cfg.rel_tol * cfg.rel_tol, abs2 = cfg.abs_tol * cfg.abs_tol;
     for(k = 1; k < maxit; k++)
     {
+        if(prof)
+            prof->begin("spmv", s);
         spmv(S, p_.data(), Ap_.data(), pAp + k, s);
+        if(prof)
+            prof->end("spmv", s);
         if(k % every == 0)
         {
             hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, r_.data(),

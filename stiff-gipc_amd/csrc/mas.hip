@@ -683,25 +683,33 @@ void Mas::assemble(const SysView& S, hipStream_t s)
 void Mas::invert(hipStream_t s)
 {
     unsigned nbanks = total_clusters_ / BANK;
+    if(prof)
+        prof->begin("mas_invert", s);
     if(cfg_.fp32)
         hipLaunchKernelGGL(k_mas_invert<float>, dim3(nbanks), dim3(64), 0, s, B_.data(),
                            inv32_.data());
     else
         hipLaunchKernelGGL(k_mas_invert<double>, dim3(nbanks), dim3(64), 0, s, B_.data(),
                            inv64_.data());
+    if(prof)
+        prof->end("mas_invert", s);
     SG_LAUNCH_CHECK();
 }
 
 template <typename T>
-static void mas_apply_t(int nparts, int L, int nv, const std::vector<int>& lvl,
+static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::vector<int>& lvl,
                         const int* vert_of_slot, const int* slot_of, const int* chain,
                         const int* going_next, const int* bank_first, const int* bank_ncl,
                         const T* inv, T* R, T* Z, double* x, double* r, const double* p,
                         const double* Ap, const double* rz_prev, const double* pAp, double* z,
                         double* rz_out, hipStream_t s)
 {
+    if(prof)
+        prof->begin("mas_apply_l0", s);
     hipLaunchKernelGGL(k_mas_l0<T>, dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
                        rz_prev, pAp, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0);
+    if(prof)
+        prof->end("mas_apply_l0", s);
     SG_LAUNCH_CHECK();
     for(int l = 1; l < L; l++)
     {
@@ -724,12 +732,12 @@ void Mas::apply_fused(double* x, double* r, const double* p, const double* Ap,
 {
     SG_CHECK(built_, -5, "MAS apply before build");
     if(cfg_.fp32)
-        mas_apply_t<float>(nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
+        mas_apply_t<float>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
                            chain_.data(), going_next_.data(), bank_first_.data(),
                            bank_ncl_.data(), inv32_.data(), R32_.data(), Z32_.data(), x, r, p,
                            Ap, rz_prev, pAp, z, rz_out, s);
     else
-        mas_apply_t<double>(nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
+        mas_apply_t<double>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
                             chain_.data(), going_next_.data(), bank_first_.data(),
                             bank_ncl_.data(), inv64_.data(), R64_.data(), Z64_.data(), x, r, p,
                             Ap, rz_prev, pAp, z, rz_out, s);

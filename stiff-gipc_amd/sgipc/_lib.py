@@ -58,6 +58,12 @@ class StepStats(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class AdvanceStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("time_steps_begun", C.c_int32),
+                ("pcg_iterations", C.c_int32), ("pad", C.c_int32),
+                ("max_pcg_rel_residual", C.c_double), ("ms_total", C.c_double)]
+
+
 class SgipcError(RuntimeError):
     pass
 
@@ -72,7 +78,8 @@ EXPORTS = [
     "sgipc_create", "sgipc_destroy", "sgipc_set_positions", "sgipc_get_positions",
     "sgipc_get_velocities", "sgipc_set_xtilde", "sgipc_get_xtilde", "sgipc_get_kappa",
     "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
-    "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step",
+    "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step", "sgipc_advance",
+    "sgipc_profile_enable", "sgipc_profile_query", "sgipc_problem_sizes",
 ]
 
 
@@ -106,6 +113,10 @@ def lib():
             L.sgipc_energy.argtypes = [C.c_void_p, dp]
             L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
             L.sgipc_time_step.argtypes = [C.c_void_p, C.POINTER(StepStats)]
+            L.sgipc_advance.argtypes = [C.c_void_p, C.c_int32, C.POINTER(AdvanceStats)]
+            L.sgipc_profile_enable.argtypes = [C.c_void_p, C.c_int32]
+            L.sgipc_profile_query.argtypes = [C.c_void_p, C.c_char_p, ip, dp]
+            L.sgipc_problem_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -277,3 +288,24 @@ class Simulation:
         check(lib().sgipc_time_step(self.h, C.byref(st)))
         return dict(newton_iterations=st.newton_iterations, pcg_iterations=st.pcg_iterations,
                     ms_total=st.ms_total)
+
+    def advance(self, n):
+        st = AdvanceStats()
+        check(lib().sgipc_advance(self.h, int(n), C.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_ if f != "pad"}
+
+    def profile(self, on=True):
+        check(lib().sgipc_profile_enable(self.h, int(on)))
+
+    def profile_query(self, name):
+        n = C.c_int32()
+        ms = C.c_double()
+        check(lib().sgipc_profile_query(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def sizes(self):
+        out = (C.c_int64 * 8)()
+        check(lib().sgipc_problem_sizes(self.h, out))
+        keys = ["n_verts", "n_tets", "nnzb", "coo_blocks", "mas_banks", "mas_levels", "pairs",
+                "ground"]
+        return dict(zip(keys, list(out)))
