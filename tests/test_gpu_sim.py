@@ -154,3 +154,24 @@ def test_newton_fp32_mas_within_solver_tolerance(oracle):
         if st["converged"] or so["converged"]:
             break
         assert err < 1e-6
+
+
+def test_advance_matches_oracle(oracle):
+    """advance(n) performs exactly n solving Newton iterations across time-step
+    boundaries; iteration counts, steps begun and positions follow the oracle
+    (FP64 MAS, 1e-8 relative)."""
+    import sgipc
+    sc = scenes()["stack"]()
+    sc.mas_precision = sgipc.SGIPC_MAS_FP64
+    sim = sgipc.Simulation(sc)
+    w = oracle.World(sc)
+    for chunk in (3, 5, 4):
+        ag = sim.advance(chunk)
+        ao = w.advance(chunk)
+        xg, xo = sim.positions(), w.positions()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[advance] chunk={chunk} gpu={ag} oracle={ao} err={err:.2e}", flush=True)
+        assert ag["newton_iterations"] == chunk
+        assert ag["time_steps_begun"] == ao["time_steps_begun"]
+        assert ag["pcg_iterations"] == ao["pcg_iterations"]
+        assert err < 1e-8
