@@ -184,9 +184,10 @@ int sgipc_advance(sgipc_t ctx, int32_t n, sgipc_advance_stats* stats);
 
 /* Per-kernel device timing with HIP events on the library stream.
  * Names: "spmv" (PCG A p), "mas_apply_l0", "fem", "mas_invert".
- * enable = 1 clears and starts collection; query returns the number of
- * timed launches and their summed duration. */
-int sgipc_profile_enable(sgipc_t ctx, int32_t enable);
+ * every >= 1 clears and starts collection, timing every every-th launch of
+ * each name (an event pair costs microseconds of stream time); 0 stops it.
+ * query returns the number of timed launches and their summed duration. */
+int sgipc_profile_enable(sgipc_t ctx, int32_t every);
 int sgipc_profile_query(sgipc_t ctx, const char* kernel, int32_t* launches,
                         double* total_ms);
 /* problem sizes for algorithmic byte counts: out[8] = {n_verts, n_tets,

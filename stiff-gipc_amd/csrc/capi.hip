@@ -364,9 +364,9 @@ int sgipc_advance(sgipc_t h, int32_t n, sgipc_advance_stats* st)
     });
 }
 
-int sgipc_profile_enable(sgipc_t h, int32_t on)
+int sgipc_profile_enable(sgipc_t h, int32_t every)
 {
-    return guard([&] { ctx_of(h).profile_enable(on != 0); });
+    return guard([&] { ctx_of(h).profile_enable(every); });
 }
 
 int sgipc_profile_query(sgipc_t h, const char* kernel, int32_t* launches, double* total_ms)

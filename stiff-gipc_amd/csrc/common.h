@@ -146,6 +146,8 @@ class Profiler
     {
         int    launches = 0;
         double total_ms = 0.0;
+        long   calls    = 0;
+        bool   open     = false;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     };
     ~Profiler()
@@ -156,12 +158,18 @@ class Profiler
         for(auto& p : pool_)
             destroy(p);
     }
+    // every sample-th launch of each name is timed (an event pair costs a few
+    // microseconds of stream time, so sparse sampling keeps timed runs honest)
+    int sample = 1;
     void begin(const char* name, hipStream_t s)
     {
         if(!enabled)
             return;
-        auto  p   = get();
         auto& ent = e_[name];
+        ent.open  = (ent.calls++ % (sample > 0 ? sample : 1)) == 0;
+        if(!ent.open)
+            return;
+        auto p = get();
         ent.pending.push_back(p);
         SG_HIP(hipEventRecord(p.first, s));
     }
@@ -170,6 +178,9 @@ class Profiler
         if(!enabled)
             return;
         auto& ent = e_[name];
+        if(!ent.open)
+            return;
+        ent.open = false;
         SG_HIP(hipEventRecord(ent.pending.back().second, s));
     }
     // call after the stream has been synchronised
@@ -200,7 +211,13 @@ class Profiler
     Entry query(const std::string& n)
     {
         auto it = e_.find(n);
-        return it == e_.end() ? Entry{} : Entry{it->second.launches, it->second.total_ms, {}};
+        Entry r;
+        if(it != e_.end())
+        {
+            r.launches = it->second.launches;
+            r.total_ms = it->second.total_ms;
+        }
+        return r;
     }
 
   private:
@@ -278,6 +295,51 @@ __device__ __forceinline__ double wave_min(double v)
 __device__ __forceinline__ void atomic_max_pos(double* addr, double v)
 {
     atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// ------------------------------------------------------------ slot sets
+// A global FP64 sum is accumulated by one atomic per workgroup into one of
+// kSlots addresses kSlotStride doubles (128 B) apart, chosen by
+// blockIdx % kSlots, so no address sees more than a few dozen atomics (a
+// single address serialises every atomic of the grid).  A consumer sums the
+// kSlots values with one wave.  A "slot set" is kSlots * kSlotStride doubles
+// and must be zero before accumulation starts.
+constexpr int kSlots      = 64;
+constexpr int kSlotStride = 16;
+constexpr int kSlotSet    = kSlots * kSlotStride;
+
+// every lane of the calling wave receives the total of the set
+__device__ __forceinline__ double slot_set_sum(const double* set)
+{
+    return wave_sum(set[__lane_id() * kSlotStride]);
+}
+
+// workgroup-wide sum of v added into the set; call from every thread of the
+// workgroup (contains __syncthreads); blockDim.x a multiple of 64, <= 1024
+__device__ __forceinline__ void slot_set_add(double* set, double v)
+{
+    __shared__ double part[16];
+    v           = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if(__lane_id() == 0)
+        part[w] = v;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        double t  = 0.0;
+        int    nw = blockDim.x >> 6;
+        for(int i = 0; i < nw; i++)
+            t += part[i];
+        if(t != 0.0)
+            atomicAdd(set + (blockIdx.x % kSlots) * kSlotStride, t);
+    }
+}
+
+// clear one slot set (kSlots threads of any workgroup)
+__device__ __forceinline__ void slot_set_clear(double* set, int lane)
+{
+    if(lane < kSlots)
+        set[lane * kSlotStride] = 0.0;
 }
 
 // wave-aggregated append: one atomic per wave, returns this lane's slot

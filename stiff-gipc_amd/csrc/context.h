@@ -33,7 +33,7 @@ class Context
     void newton_iteration(sgipc_iter_stats* st);
     void time_step(sgipc_step_stats* st);
     void advance(int n, sgipc_advance_stats* st);
-    void profile_enable(bool on);
+    void profile_enable(int every);
     void profile_query(const char* name, int32_t* launches, double* total_ms);
     void problem_sizes(int64_t* out);
 

@@ -215,7 +215,7 @@ Context::Context(const sgipc_scene_desc& d)
     cand_.resize(cand_cap_);
     overflow_.resize(4);
     overflow_.zero(s_);
-    coo_.reserve(6 * pair_cap_);
+    coo_.reserve(6 * pair_cap_, nv_);
 
     dev_.nv = nv_, dev_.nt = nt_, dev_.ns = ns_, dev_.nf = nf_, dev_.ne = ne_;
     dev_.dt = d.dt, dev_.dHat = dHat_, dev_.goff = d.ground_offset;
@@ -334,6 +334,8 @@ void Context::assemble(bool hessian)
     launch_fem(dev_, hessian, s_);
     prof_.end("fem", s_);
     launch_rhs(dev_, rhs_.data(), s_);
+    if(hessian)
+        coo_.build_rows(s_);
 }
 
 double Context::energy()
@@ -641,10 +643,11 @@ void Context::advance(int n, sgipc_advance_stats* st)
         *st = loc;
 }
 
-void Context::profile_enable(bool on)
+void Context::profile_enable(int every)
 {
     prof_.clear();
-    prof_.enabled = on;
+    prof_.enabled = every > 0;
+    prof_.sample  = every > 0 ? every : 1;
 }
 
 void Context::profile_query(const char* name, int32_t* launches, double* total_ms)

@@ -7,7 +7,8 @@
 //     linear_system/utils/converter.cu) -> static pattern built once from the
 //     mesh, contact blocks appended to a COO tail (no per-iteration sort);
 //   Spmv::warp_reduce_sym_spmv (linear_system/utils/spmv.cu:8-133) -> row-parallel
-//     full-storage SpMV (no atomics) + COO atomics;
+//     full-storage SpMV over the static pattern plus a row-sorted copy of the
+//     contact tail, no atomics on y;
 //   PCGSolver (linear_system/solver/pcg_solver.cu:155-304) -> Pcg below, with
 //     device-resident scalars and one host sync per convergence test;
 //   MASPreconditioner (MASPreconditioner.cu) -> Mas below.
@@ -41,27 +42,49 @@ struct Coo
     DBuf<int>      r, c;     // cap
     DBuf<double>   v;        // 9 * cap
     DBuf<unsigned> count;    // 1 (device)
-    void reserve(int n)
+    // row-sorted copy (CSR over the block rows) made by build_rows()
+    int            nb = 0;
+    DBuf<int>      rp;       // nb + 1
+    DBuf<int>      fill;     // nb + 1 (histogram / fill cursor)
+    DBuf<int>      rc;       // cap
+    DBuf<double>   rv;       // 9 * cap
+    bool           rows_valid = false;
+    void reserve(int n, int nrows)
     {
         cap = n;
+        nb  = nrows;
         r.resize(n);
         c.resize(n);
         v.resize(9 * (size_t)n);
         count.resize(1);
+        rp.resize(nrows + 1);
+        fill.resize(nrows + 1);
+        rc.resize(n);
+        rv.resize(9 * (size_t)n);
     }
+    // counting sort of the live entries by row (after every assembly)
+    void build_rows(hipStream_t s);
 };
 
 // view of A = static BSR + COO tail
 struct SysView
 {
     const Bsr* A   = nullptr;
-    const Coo* coo = nullptr;  // may be null
+    const Coo* coo = nullptr;  // may be null; its row-sorted copy must be current
 };
 
-// y = A x (FP64); optional accumulation of dot(x, y) into *dot
+// ------------------------------------------------------------ reductions
+// (slot sets: common.h)
+// host-side sum of a slot set in the device's butterfly order
+double slot_set_sum_host(const double* h);
+double read_slot_set(const double* d, hipStream_t s);
+
+// y = A x (FP64); optional dot(x, y) accumulated into the slot set `dot`
 void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t s);
 // r = b - A x
 void residual(const SysView& S, const double* b, const double* x, double* r, hipStream_t s);
+// dot(a, b) accumulated into the slot set `out` (must be zeroed)
+void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t s);
 
 // ------------------------------------------------------------------ MAS
 struct MasConfig
@@ -82,10 +105,13 @@ class Mas
     void apply(const double* r, double* z, hipStream_t s);
 
     // fused PCG pieces (see Pcg): level-0 banks with the optional update
-    // x += a p, r -= a Ap (a = rz[k-1] / pAp[k]); coarse levels; collect
+    // x += a p, r -= a Ap (a = rz[k-1] / pAp[k], both given as slot sets);
+    // coarse levels; collect with rz accumulated into the slot set rz_out and
+    // the two slot sets zero0/zero1 (may be null) cleared for later use.
+    // Every kernel returns at once when done (may be null) is non-zero.
     void apply_fused(double* x, double* r, const double* p, const double* Ap,
                      const double* rz_prev, const double* pAp, double* z, double* rz_out,
-                     hipStream_t s);
+                     double* zero0, double* zero1, const int* done, hipStream_t s);
 
     int levels() const { return L_; }
     int total_clusters() const { return total_clusters_; }
@@ -152,6 +178,10 @@ enum class Precond
 class Pcg
 {
   public:
+    Pcg() = default;
+    ~Pcg();
+    Pcg(const Pcg&)            = delete;
+    Pcg& operator=(const Pcg&) = delete;
     // solve A x = b from x = 0 with the reference's iteration structure
     PcgStats solve(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
                    const PcgConfig& cfg, hipStream_t s);
@@ -161,9 +191,19 @@ class Pcg
     Profiler* prof = nullptr;
 
   private:
-    void ensure(int n, int maxit);
-    DBuf<double> r_, z_, p_, Ap_, sc_, dinv_, tmp_;
-    int          n_ = 0, maxit_ = 0;
+    void ensure(int n);
+    // ring of kRing (pAp, rz) slot-set pairs indexed by iteration, plus aux
+    // sets for r.r, |b|^2 and the final residual
+    static constexpr int kRing = 4;
+    double* pap_set(int k) { return red_.data() + (size_t)(2 * (k % kRing)) * kSlotSet; }
+    double* rz_set(int k) { return red_.data() + (size_t)(2 * (k % kRing) + 1) * kSlotSet; }
+    double* aux_set(int i) { return red_.data() + (size_t)(2 * kRing + i) * kSlotSet; }
+    DBuf<double> r_, z_, p_[2], Ap_, red_, dinv_, tmp_;
+    DBuf<int>    ctl_;         // [0] stopping check iteration (0: running), [1] breakdown
+    DBuf<double> ctl_d_;       // [0] r.z at the last stop test
+    int*         h_ctl_ = nullptr;  // pinned: two slots of 4 ints for the stop flag
+    hipEvent_t   ev_[2] = {nullptr, nullptr};
+    int          n_ = 0;
 };
 
 }  // namespace sgipc

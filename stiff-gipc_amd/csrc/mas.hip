@@ -30,6 +30,7 @@ namespace sgipc
 
 constexpr int BANK = 16;
 constexpr int BD   = 48;  // 3 * BANK
+constexpr int ZB   = 64;  // Z entries per bank: one per lane, lanes 48..63 padding
 
 static int compute_num_levels(int nv)
 {
@@ -381,23 +382,96 @@ __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B,
 }
 
 // ---------------------------------------------------------------- apply
-// Level-0 banks (one wave each): optional PCG update of x and r, z0 = Binv r,
-// and the level-1 cluster sums of r.
+// One bank's product z = Binv R with one wave: lane u < 48 holds R_u, lane
+// t < 48 produces z_t (lanes 48..63 compute a copy of lane 0's and store it
+// into the bank's padding).  The 48 entries a lane needs (column t of the
+// stored transpose) are loaded before the products so their latencies
+// overlap; every lane stores unconditionally so the compiler keeps the loads
+// together instead of sinking them into a guarded block.
+template <typename T>
+__device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T rv)
+{
+    const int t  = __lane_id();
+    const int tc = t < BD ? t : 0;
+    const T*  P  = inv + (size_t)gb * BD * BD + tc;
+    T         pv[BD];
+#pragma unroll
+    for(int u = 0; u < BD; u++)
+        pv[u] = P[u * BD];
+    T acc = 0;
+#pragma unroll
+    for(int u = 0; u < BD; u++)
+        acc += pv[u] * __shfl(rv, u, 64);
+    return acc;
+}
+
+// Same product with R_u read from LDS (rs[0..48), a broadcast read per u).
+template <typename T>
+__device__ __forceinline__ T bank_product_lds(const T* __restrict__ inv, int gb, const T* rs)
+{
+    const int t  = __lane_id();
+    const int tc = t < BD ? t : 0;
+    const T*  P  = inv + (size_t)gb * BD * BD + tc;
+    T         pv[BD];
+#pragma unroll
+    for(int u = 0; u < BD; u++)
+        pv[u] = P[u * BD];
+    T acc = 0;
+#pragma unroll
+    for(int u = 0; u < BD; u++)
+        acc += pv[u] * rs[u];
+    return acc;
+}
+
+// Cluster sums of one bank's R into the next level (plain stores: the bank
+// owns those nodes), and optionally into the level after it (atomics: the
+// kernel two levels up reads them after this launch).  rv: lane t < 48 holds
+// component t % 3 of bank node t / 3; node_up: global id of that node's
+// parent (or -1).
+template <typename T>
+__device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, int first,
+                                              int ncl, const int* __restrict__ going_next,
+                                              T* __restrict__ R, bool up2)
+{
+    const int t = __lane_id();
+    if(t < BD)
+        Rc[t] = (T)0;
+    __syncthreads();
+    if(node_up >= 0)
+        atomicAdd(&Rc[3 * (node_up - first) + t % 3], rv);
+    __syncthreads();
+    if(t < 3 * ncl)
+    {
+        T v = Rc[t];
+        R[3 * (size_t)first + t] = v;
+        if(up2)
+            atomicAdd(&R[3 * (size_t)going_next[first + t / 3] + t % 3], v);
+    }
+}
+
+// Level-0 banks (one wave each): optional PCG update of x and r
+// (a = rz[k-1] / pAp[k] from slot sets), z0 = Binv r, and the level-1 (and,
+// with up2, level-2) cluster sums of r.
 template <typename T>
 __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_slot,
                                                double* __restrict__ x, double* __restrict__ r,
                                                const double* __restrict__ p,
                                                const double* __restrict__ Ap,
                                                const double* __restrict__ rz_prev,
-                                               const double* __restrict__ pAp,
+                                               const double* __restrict__ pap,
                                                const T* __restrict__ inv,
                                                const int* __restrict__ going_next,
                                                const int* __restrict__ bank_first,
                                                const int* __restrict__ bank_ncl, T* __restrict__ R,
-                                               T* __restrict__ Z, int has_coarse)
+                                               T* __restrict__ Z, int has_coarse, int up2,
+                                               const int* __restrict__ done)
 {
-    __shared__ T Rb[BD];
+    if(done && *done)
+        return;
     __shared__ T Rc[BD];
+    double a = 0.0;
+    if(x)
+        a = slot_set_sum(rz_prev) / slot_set_sum(pap);
     const int b  = blockIdx.x;
     const int t  = threadIdx.x;
     const int s  = b * BANK + t / 3;
@@ -410,95 +484,148 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
         rv       = r[d];
         if(x)
         {
-            double a = *rz_prev / *pAp;
             x[d] += a * p[d];
             rv -= a * Ap[d];
             r[d] = rv;
         }
     }
-    if(t < BD)
-    {
-        Rb[t] = (T)rv;
-        Rc[t] = (T)0;
-    }
-    __syncthreads();
-    if(t < BD)
-    {
-        const T* P   = inv + (size_t)b * BD * BD;
-        T        acc = 0;
-        for(int u = 0; u < BD; u++)
-            acc += P[u * BD + t] * Rb[u];
-        Z[(size_t)b * BD + t] = acc;
-        if(has_coarse && v >= 0)
-        {
-            int k = going_next[s] - bank_first[b];
-            atomicAdd(&Rc[3 * k + cc], Rb[t]);
-        }
-    }
-    __syncthreads();
-    if(has_coarse && t < 3 * bank_ncl[b])
-        R[3 * (size_t)bank_first[b] + t] = Rc[t];
+    T rt = (T)rv;
+    T z  = bank_product(inv, b, rt);
+    Z[(size_t)b * ZB + t] = z;
+    if(has_coarse)
+        restrict_bank(Rc, rt, v >= 0 ? going_next[s] : -1, bank_first[b], bank_ncl[b],
+                      going_next, R, up2 != 0);
 }
 
-// Banks of one coarse level: z = Binv R, and the next level's sums.
+// Banks of one coarse level, one wave each: z = Binv R and the next level's
+// (and with up2 the level after's) sums.
 template <typename T>
 __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
                                                const T* __restrict__ inv,
                                                const int* __restrict__ going_next,
                                                const int* __restrict__ bank_first,
                                                const int* __restrict__ bank_ncl, T* __restrict__ R,
-                                               T* __restrict__ Z, int has_next)
+                                               T* __restrict__ Z, int has_next, int up2,
+                                               const int* __restrict__ done)
 {
-    __shared__ T Rb[BD];
+    if(done && *done)
+        return;
     __shared__ T Rc[BD];
-    const int gb = bank0 + blockIdx.x;
-    const int t  = threadIdx.x;
-    const int g  = gb * BANK + t / 3;  // global node id
+    const int gb    = bank0 + blockIdx.x;
+    const int t     = threadIdx.x;
+    const int g     = gb * BANK + t / 3;  // global node id
+    const int cc    = t % 3;
+    const bool valid = (t < BD) && (g - node0 < count);
+    T         rv    = valid ? R[3 * (size_t)g + cc] : (T)0;
+    T         z     = bank_product(inv, gb, rv);
+    Z[(size_t)gb * ZB + t] = z;
+    if(has_next)
+        restrict_bank(Rc, rv, valid ? going_next[g] : -1, bank_first[gb], bank_ncl[gb],
+                      going_next, R, up2 != 0);
+}
+
+// The coarse levels left after the separate launches, in one launch.
+// Workgroups [0, nwg_b) give one wave to each bank of level lb (R complete
+// from earlier launches, no restriction needed).  The last workgroup (when
+// lt < L) owns every level from lt up: it loads R_lt (complete: stored by the
+// level lt-1 kernel, or accumulated by the level lt-2 kernel and then cleared
+// here for the next apply), sums the higher levels' R in LDS along
+// goingNext, and gives one wave to each bank of those levels.
+constexpr int kCoarseWaves = 16;
+constexpr int kTailBanks   = 64;    // a level with at most this many banks may start the tail
+constexpr int kTailNodes   = 2048;  // node span of the tail levels (a multiple of 16)
+struct LevelTable
+{
+    int count[24];
+    int off[24];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(64 * kCoarseWaves)
+    k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt, int clear_lt,
+                 const T* __restrict__ inv, const int* __restrict__ going_next,
+                 T* __restrict__ R, T* __restrict__ Z, const int* __restrict__ done)
+{
+    if(done && *done)
+        return;
+    const int w  = threadIdx.x >> 6;
+    const int t  = threadIdx.x & 63;
     const int cc = t % 3;
-    bool      valid = (t < BD) && (g - node0 < count);
-    if(t < BD)
+    if((int)blockIdx.x < nwg_b)
     {
-        Rb[t] = valid ? R[3 * (size_t)g + cc] : (T)0;
-        Rc[t] = (T)0;
+        const int node0 = lv.off[lb], count = lv.count[lb];
+        const int bk    = blockIdx.x * kCoarseWaves + w;
+        const int nbk   = (count + BANK - 1) / BANK;
+        if(bk >= nbk)
+            return;
+        const int gb = node0 / BANK + bk;
+        const int g  = gb * BANK + t / 3;
+        T         rv = (t < BD && g - node0 < count) ? R[3 * (size_t)g + cc] : (T)0;
+        T         z  = bank_product(inv, gb, rv);
+        Z[(size_t)gb * ZB + t] = z;
+        return;
     }
+    // ---- tail workgroup
+    __shared__ T Rs[3 * kTailNodes];
+    const int base = lv.off[lt];
+    const int span = lv.off[L - 1] + lv.count[L - 1] - base;
+    const int nb   = (span + BANK - 1) / BANK;  // banks of the tail levels
+    for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
+        Rs[i] = (T)0;
     __syncthreads();
-    if(t < BD)
     {
-        const T* P   = inv + (size_t)gb * BD * BD;
-        T        acc = 0;
-        for(int u = 0; u < BD; u++)
-            acc += P[u * BD + t] * Rb[u];
-        Z[(size_t)gb * BD + t] = acc;
-        if(has_next && valid)
+        const int nc = lv.count[lt];
+        for(int i = threadIdx.x; i < 3 * nc; i += blockDim.x)
         {
-            int k = going_next[g] - bank_first[gb];
-            atomicAdd(&Rc[3 * k + cc], Rb[t]);
+            Rs[i] = R[3 * (size_t)base + i];
+            if(clear_lt)
+                R[3 * (size_t)base + i] = (T)0;
         }
     }
     __syncthreads();
-    if(has_next && t < 3 * bank_ncl[gb])
-        R[3 * (size_t)bank_first[gb] + t] = Rc[t];
+    for(int l = lt; l + 1 < L; l++)
+    {
+        const int n0 = lv.off[l], nc = lv.count[l];
+        for(int i = threadIdx.x; i < 3 * nc; i += blockDim.x)
+        {
+            int g = n0 + i / 3;
+            atomicAdd(&Rs[3 * (going_next[g] - base) + i % 3], Rs[3 * (g - base) + i % 3]);
+        }
+        __syncthreads();
+    }
+    const int b0 = base / BANK;
+    for(int bi = w; bi < nb; bi += kCoarseWaves)
+    {
+        const int gb = b0 + bi;
+        T         z  = bank_product_lds(inv, gb, Rs + 3 * (gb * BANK - base));
+        Z[(size_t)gb * ZB + t] = z;
+    }
 }
 
-// z_v = Z[slot(v)] + sum_l Z[chain_l(v)];  rz += r . z
+// z_v = Z[slot(v)] + sum_l Z[chain_l(v)];  r.z into a slot set; workgroup 0
+// clears two slot sets for later iterations
 template <typename T>
-__global__ void k_mas_collect(int nv, int L, const int* __restrict__ slot_of,
-                              const int* __restrict__ chain, const T* __restrict__ Z,
-                              const double* __restrict__ r, double* __restrict__ z,
-                              double* rz_out)
+__global__ void __launch_bounds__(256)
+    k_mas_collect(int nv, int L, const int* __restrict__ slot_of, const int* __restrict__ chain,
+                  const T* __restrict__ Z, const double* __restrict__ r, double* __restrict__ z,
+                  double* rz_out, double* zero0, double* zero1, const int* __restrict__ done)
 {
+    if(done && *done)
+        return;
     int    v = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
     if(v < nv)
     {
-        int s  = slot_of[v];
-        T   z0 = Z[3 * (size_t)s], z1 = Z[3 * (size_t)s + 1], z2 = Z[3 * (size_t)s + 2];
+        int      s  = slot_of[v];
+        const T* zs = Z + (size_t)(s >> 4) * ZB + 3 * (s & 15);
+        T        z0 = zs[0], z1 = zs[1], z2 = zs[2];
         for(int l = 1; l < L; l++)
         {
-            size_t g = chain[(size_t)v * (L - 1) + l - 1];
-            z0 += Z[3 * g];
-            z1 += Z[3 * g + 1];
-            z2 += Z[3 * g + 2];
+            int      g  = chain[(size_t)v * (L - 1) + l - 1];
+            const T* zg = Z + (size_t)(g >> 4) * ZB + 3 * (g & 15);
+            z0 += zg[0];
+            z1 += zg[1];
+            z2 += zg[2];
         }
         z[3 * (size_t)v]     = (double)z0;
         z[3 * (size_t)v + 1] = (double)z1;
@@ -508,10 +635,13 @@ __global__ void k_mas_collect(int nv, int L, const int* __restrict__ slot_of,
                 + r[3 * (size_t)v + 2] * (double)z2;
     }
     if(rz_out)
+        slot_set_add(rz_out, d);
+    if(blockIdx.x == 0)
     {
-        d = wave_sum(d);
-        if(__lane_id() == 0 && d != 0.0)
-            atomicAdd(rz_out, d);
+        if(zero0)
+            slot_set_clear(zero0, threadIdx.x);
+        if(zero1)
+            slot_set_clear(zero1, threadIdx.x);
     }
 }
 
@@ -649,18 +779,23 @@ void Mas::build(const SysView& S, hipStream_t s)
         {
             inv32_.resize(nbanks * BD * BD);
             R32_.resize(3 * (size_t)total_clusters_);
-            Z32_.resize(3 * (size_t)total_clusters_);
+            Z32_.resize((size_t)ZB * nbanks);
         }
         else
         {
             inv64_.resize(nbanks * BD * BD);
             R64_.resize(3 * (size_t)total_clusters_);
-            Z64_.resize(3 * (size_t)total_clusters_);
+            Z64_.resize((size_t)ZB * nbanks);
         }
         nbanks_alloc_ = nbanks;
     }
     assemble(S, s);
     invert(s);
+    // R of a level fed by atomics from two levels below must start at zero
+    if(cfg_.fp32)
+        R32_.zero(s);
+    else
+        R64_.zero(s);
     built_ = true;
 }
 
@@ -701,52 +836,93 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
                         const int* vert_of_slot, const int* slot_of, const int* chain,
                         const int* going_next, const int* bank_first, const int* bank_ncl,
                         const T* inv, T* R, T* Z, double* x, double* r, const double* p,
-                        const double* Ap, const double* rz_prev, const double* pAp, double* z,
-                        double* rz_out, hipStream_t s)
+                        const double* Ap, const double* rz_prev, const double* pap, double* z,
+                        double* rz_out, double* zero0, double* zero1, const int* done,
+                        hipStream_t s)
 {
+    // Plan: separate launches for levels 1 .. lb-1, one k_mas_coarse launch
+    // for level lb plus the tail lt .. L-1 (lt = lb + 1).  The kernel of level
+    // lt - 2 also accumulates R_lt so the tail can start at once.
+    int lt = L, lb = -1;
+    if(L > 1)
+    {
+        for(int l = 1; l < L; l++)
+        {
+            int nbk  = (lvl[2 * l] + BANK - 1) / BANK;
+            int span = lvl[2 * (L - 1) + 1] + lvl[2 * (L - 1)] - lvl[2 * l + 1];
+            if(nbk <= kTailBanks && span <= kTailNodes)
+            {
+                lt = l;
+                break;
+            }
+        }
+        lb = (lt >= 2) ? lt - 1 : -1;
+    }
+    const int up2_level = (lb >= 1 && lt < L) ? lb - 1 : -1;  // level whose kernel feeds R_lt
     if(prof)
         prof->begin("mas_apply_l0", s);
     hipLaunchKernelGGL(k_mas_l0<T>, dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
-                       rz_prev, pAp, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0);
+                       rz_prev, pap, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0,
+                       up2_level == 0 ? 1 : 0, done);
     if(prof)
         prof->end("mas_apply_l0", s);
     SG_LAUNCH_CHECK();
-    for(int l = 1; l < L; l++)
+    if(L > 1)
     {
-        int count = lvl[2 * l], off = lvl[2 * l + 1];
-        if(count <= 0)
-            continue;
-        hipLaunchKernelGGL(k_mas_lx<T>, dim3((count + BANK - 1) / BANK), dim3(64), 0, s,
-                           off / BANK, off, count, inv, going_next, bank_first, bank_ncl, R, Z,
-                           (l + 1 < L) ? 1 : 0);
-        SG_LAUNCH_CHECK();
+        for(int l = 1; l < lb; l++)
+        {
+            int count = lvl[2 * l], off = lvl[2 * l + 1];
+            int nbk   = (count + BANK - 1) / BANK;
+            hipLaunchKernelGGL(k_mas_lx<T>, dim3(nbk), dim3(64), 0, s, off / BANK, off, count, inv,
+                               going_next, bank_first, bank_ncl, R, Z, 1,
+                               up2_level == l ? 1 : 0, done);
+            SG_LAUNCH_CHECK();
+        }
+        SG_CHECK(L <= 24, -1, "MAS: more than 24 levels");
+        LevelTable lv{};
+        for(int q = 0; q < L; q++)
+        {
+            lv.count[q] = lvl[2 * q];
+            lv.off[q]   = lvl[2 * q + 1];
+        }
+        int nwg_b = 0;
+        if(lb >= 1)
+            nwg_b = ((lvl[2 * lb] + BANK - 1) / BANK + kCoarseWaves - 1) / kCoarseWaves;
+        int grid = nwg_b + (lt < L ? 1 : 0);
+        if(grid > 0)
+        {
+            hipLaunchKernelGGL(k_mas_coarse<T>, dim3(grid), dim3(64 * kCoarseWaves), 0, s, lv, L,
+                               lb, nwg_b, lt, up2_level >= 0 ? 1 : 0, inv, going_next, R, Z,
+                               done);
+            SG_LAUNCH_CHECK();
+        }
     }
     hipLaunchKernelGGL(k_mas_collect<T>, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L,
-                       slot_of, chain, Z, r, z, rz_out);
+                       slot_of, chain, Z, r, z, rz_out, zero0, zero1, done);
     SG_LAUNCH_CHECK();
 }
 
 void Mas::apply_fused(double* x, double* r, const double* p, const double* Ap,
-                      const double* rz_prev, const double* pAp, double* z, double* rz_out,
-                      hipStream_t s)
+                      const double* rz_prev, const double* pap, double* z, double* rz_out,
+                      double* zero0, double* zero1, const int* done, hipStream_t s)
 {
     SG_CHECK(built_, -5, "MAS apply before build");
     if(cfg_.fp32)
         mas_apply_t<float>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
                            chain_.data(), going_next_.data(), bank_first_.data(),
                            bank_ncl_.data(), inv32_.data(), R32_.data(), Z32_.data(), x, r, p,
-                           Ap, rz_prev, pAp, z, rz_out, s);
+                           Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
     else
         mas_apply_t<double>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
                             chain_.data(), going_next_.data(), bank_first_.data(),
                             bank_ncl_.data(), inv64_.data(), R64_.data(), Z64_.data(), x, r, p,
-                            Ap, rz_prev, pAp, z, rz_out, s);
+                            Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
 }
 
 void Mas::apply(const double* r, double* z, hipStream_t s)
 {
     apply_fused(nullptr, const_cast<double*>(r), nullptr, nullptr, nullptr, nullptr, z, nullptr,
-                s);
+                nullptr, nullptr, nullptr, s);
 }
 
 std::vector<int> Mas::chain_host(hipStream_t s) const

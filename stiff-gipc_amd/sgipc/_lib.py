@@ -294,8 +294,9 @@ class Simulation:
         check(lib().sgipc_advance(self.h, int(n), C.byref(st)))
         return {f: getattr(st, f) for f, _ in st._fields_ if f != "pad"}
 
-    def profile(self, on=True):
-        check(lib().sgipc_profile_enable(self.h, int(on)))
+    def profile(self, every=1):
+        """Time every `every`-th launch of each profiled kernel (0: off)."""
+        check(lib().sgipc_profile_enable(self.h, int(every)))
 
     def profile_query(self, name):
         n = C.c_int32()

@@ -21,3 +21,46 @@ def rel(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def deep_hierarchy_system(nblob=4096, seed=0):
+    """Synthetic SPD block system whose MAS hierarchy has two large coarse
+    levels followed by small ones: 16-vertex chains ("blobs", one bank each);
+    inside each level-1 bank (16 blobs) blobs are joined in pairs, giving 8
+    level-2 nodes per level-1 bank; pairs of consecutive level-1 banks are
+    joined in a zigzag, so each level-2 bank is connected and level 3 has one
+    node per level-2 bank.  With 4096 blobs: level 1 = 256 banks, level 2 =
+    128 banks, level 3 = 8 banks.  Returns (bsr, part_id, rhs)."""
+    import pyoracle as O
+    rng = np.random.default_rng(seed)
+    nv = 16 * nblob
+    edges = []
+    for b in range(nblob):
+        v0 = 16 * b
+        edges += [(v0 + k, v0 + k + 1) for k in range(15)]
+    first = lambda blob: 16 * blob
+    for b1 in range(nblob // 16):
+        for q in range(0, 16, 2):
+            edges.append((first(16 * b1 + q), first(16 * b1 + q + 1)))
+    for b2 in range(nblob // 32):
+        lo, hi = 2 * b2, 2 * b2 + 1
+        for c in range(8):
+            edges.append((first(16 * lo + 2 * c) + 3, first(16 * hi + 2 * c) + 3))
+            if c + 1 < 8:
+                edges.append((first(16 * hi + 2 * c) + 5, first(16 * lo + 2 * c + 2) + 5))
+    e = np.sort(np.array(edges, np.int64), axis=1)
+    ne = e.shape[0]
+    w = rng.uniform(0.5, 2.0, size=ne)
+    off = -w[:, None, None] * np.eye(3) + 0.1 * rng.normal(size=(ne, 3, 3))
+    rowsum = np.zeros(nv)
+    absn = np.abs(off).sum(axis=2).max(axis=1)
+    np.add.at(rowsum, e[:, 0], absn)
+    np.add.at(rowsum, e[:, 1], absn)
+    S = rng.normal(size=(nv, 3, 3)) * 0.1
+    diag = (rowsum + 1.0)[:, None, None] * np.eye(3) + 0.5 * (S + S.transpose(0, 2, 1))
+    rows = np.concatenate([np.arange(nv), e[:, 0]])
+    cols = np.concatenate([np.arange(nv), e[:, 1]])
+    vals = np.concatenate([diag, off]).reshape(-1, 9)
+    A = O.bsr_from_upper_triplets(nv, rows, cols, vals)
+    part = (np.arange(nv) // 16).astype(np.int32)
+    return A, part, rng.normal(size=3 * nv)

@@ -75,3 +75,42 @@ def test_pcg_matches_oracle(systems, oracle, name, pc):
     assert rel(x, xo) < 1e-8
     D = A.dense()
     assert np.linalg.norm(D @ x - g) / np.linalg.norm(g) == pytest.approx(st.rel_residual, rel=1e-6)
+
+
+@pytest.fixture(scope="module")
+def deep(oracle):
+    from helpers import deep_hierarchy_system
+    return deep_hierarchy_system()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_mas_deep_hierarchy(deep, oracle, prec):
+    """Two large coarse levels (own launches) then a small tail restricted in
+    one workgroup: hierarchy equal, apply equal to the oracle."""
+    import sgipc
+    A, part, b = deep
+    fp32 = prec == "fp32"
+    ls = sgipc.LinearSystem(A.rowptr, A.cols, A.vals, part, sgipc.SGIPC_PRECOND_MAS,
+                            sgipc.SGIPC_MAS_FP32 if fp32 else sgipc.SGIPC_MAS_FP64)
+    m = oracle.Mas(A, part, fp32=fp32)
+    L, sizes, tc = ls.mas_info()
+    print(f"[mas-deep] {prec} levels={L} sizes={sizes.tolist()}", flush=True)
+    np.testing.assert_array_equal(sizes, m.level_sizes())
+    banks = [(int(c) + 15) // 16 for c, _ in sizes[1:L]]
+    assert banks[0] > 64 and banks[1] > 64 and min(banks[2:]) <= 64
+    r = np.random.default_rng(5).normal(size=3 * A.nb)
+    assert rel(ls.precond(r), m.apply(r)) < (2e-5 if fp32 else 1e-10)
+
+
+def test_pcg_deep_hierarchy(deep, oracle):
+    import sgipc
+    A, part, b = deep
+    ls = sgipc.LinearSystem(A.rowptr, A.cols, A.vals, part, sgipc.SGIPC_PRECOND_MAS,
+                            sgipc.SGIPC_MAS_FP64)
+    x, st = ls.pcg(b, rel_tol=1e-6)
+    m = oracle.Mas(A, part, fp32=False)
+    xo, ito, _, _ = oracle.pcg(A, b, m, 1, rel_tol=1e-6)
+    print(f"[pcg-deep] it={st.iterations} oracle_it={ito} relres={st.rel_residual:.3e}",
+          flush=True)
+    assert st.iterations == ito
+    assert rel(x, xo) < 1e-8
