@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--mas-fp64", action="store_true", help="all-FP64 MAS instead of FP32 apply")
     ap.add_argument("--cpu-iters", type=int, default=1, help="oracle Newton iterations timed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="all", choices=["all", "ref"],
+                    help="time all solver modes or only the reference FP64 one")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -94,29 +96,55 @@ def main():
                 f"({time.time() - t0:.1f} s)")
     state = (sim.positions(), sim.velocities(), sim.xtilde(), sim.kappa)
 
-    # ---- warmup + timed Newton iterations
-    if args.warmup:
-        sim.advance(args.warmup)
-    # HIP-event timing of every 10th launch of the profiled kernels during the
-    # timed region (one event pair adds a few microseconds of stream time)
-    sim.profile(10)
-    t0 = time.perf_counter()
-    adv = sim.advance(args.steps)
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    # ---- timed Newton iterations, each solver mode from the same settled state.
+    # fp64_ref (the value): the reference's plain FP64 solve, stop test on the
+    # preconditioned norm r.z every 10 iterations.  fp64_true / mixed_true: stop
+    # on the FP64 true residual ||b - A x|| / ||b|| <= 1e-4, FP64 PCG versus FP32
+    # inner PCG with FP64 refinement.
+    modes = [
+        ("fp64_ref", dict(rel_tol=1e-4, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)),
+        ("fp64_true", dict(rel_tol=1e-4, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_FP64)),
+        ("mixed_true", dict(rel_tol=1e-4, use_rz=False,
+                            solver_precision=sgipc.SGIPC_SOLVE_MIXED)),
+    ]
+    results = {}
+    for name, cfg in modes:
+        if args.modes != "all" and name != "fp64_ref":
+            continue
+        sim.set_state(*state)
+        sim.set_pcg(**cfg)
+        if args.warmup:
+            sim.advance(args.warmup)
+        # HIP-event timing of every 10th launch of the profiled kernels during
+        # the timed region (one event pair adds a few microseconds of stream time)
+        sim.profile(10 if name == "fp64_ref" else 0)
+        t0 = time.perf_counter()
+        adv = sim.advance(args.steps)
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
+        results[name] = {"ms_per_newton": round(ms, 4),
+                         "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
+                         "max_rel_residual": adv["max_pcg_rel_residual"],
+                         "fp64_fallbacks": adv["solver_fallbacks"]}
+        log(f"timed [{name}]: {args.steps} Newton iterations in {(t1 - t0):.3f} s -> "
+            f"{ms:.3f} ms/iter; pcg/iter={adv['pcg_iterations'] / args.steps:.1f} "
+            f"max_rel_res={adv['max_pcg_rel_residual']:.2e} "
+            f"steps_begun={adv['time_steps_begun']} fallbacks={adv['solver_fallbacks']}")
+        if name == "fp64_ref":
+            kern = {}
+            for kname in ("spmv", "mas_apply_l0", "fem", "mas_invert"):
+                n, tot = sim.profile_query(kname)
+                if n:
+                    kern[kname] = (n, tot)
+                    log(f"kernel {kname}: {n} launches, {tot:.3f} ms total, "
+                        f"{tot / n * 1e3:.2f} us avg")
+            ref_adv, ref_ms = adv, ms
+            sim.profile(0)
+    adv, ms = ref_adv, ref_ms
     sz = sim.sizes()
-    log(f"timed: {args.steps} Newton iterations in {(t1 - t0):.3f} s -> {ms:.3f} ms/iter; "
-        f"pcg/iter={adv['pcg_iterations'] / args.steps:.1f} "
-        f"max_rel_res={adv['max_pcg_rel_residual']:.2e} steps_begun={adv['time_steps_begun']} "
-        f"sizes={sz}")
+    log(f"sizes={sz}")
 
     # ---- roofline of the dominant profiled kernel (HIP events on the library stream)
-    kern = {}
-    for name in ("spmv", "mas_apply_l0", "fem", "mas_invert"):
-        n, tot = sim.profile_query(name)
-        if n:
-            kern[name] = (n, tot)
-            log(f"kernel {name}: {n} launches, {tot:.3f} ms total, {tot / n * 1e3:.2f} us avg")
     hbm_kernels = {k: v for k, v in kern.items() if k in ("spmv", "mas_apply_l0", "fem")}
     dom = max(hbm_kernels, key=lambda k: hbm_kernels[k][1]) if hbm_kernels else None
     roof = None
@@ -164,6 +192,7 @@ def main():
         "data": "synthetic",
         "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
         "max_pcg_rel_residual": adv["max_pcg_rel_residual"],
+        "modes": results,
         "config": {
             "workload": f"{args.n_side ** 3} tet cubes of {args.cells}^3 hex cells "
                         f"({sc.n_tets} tets, {sc.n_verts} verts), E log-uniform [1e5,1e9] Pa, "

@@ -55,7 +55,7 @@ typedef struct sgipc_pcg_cfg {
 
 typedef struct sgipc_pcg_stats {
     int32_t iterations;       /* PCG iterations (reference count semantics)    */
-    int32_t outer_iterations; /* refinement sweeps (mixed mode), 1 otherwise   */
+    int32_t outer_iterations; /* 1; 2 when the mixed solve fell back to FP64    */
     int32_t converged;
     int32_t pad;
     double  rel_residual;     /* ||b - A x||_2 / ||b||_2 recomputed in FP64    */
@@ -108,6 +108,15 @@ int sgipc_set_xtilde(sgipc_t ctx, const double* xt);     /* [3N] */
 int sgipc_get_xtilde(sgipc_t ctx, double* xt);           /* [3N] */
 int sgipc_get_kappa(sgipc_t ctx, double* kappa);
 int sgipc_set_kappa(sgipc_t ctx, double kappa);
+/* Replace the whole integration state at a time-step boundary: positions
+ * (also the step-start positions), velocities, x_tilde and kappa
+ * (GIPC's vertexes / o_vertexes / velocities / xTilta / Kappa). */
+int sgipc_set_state(sgipc_t ctx, const double* x, const double* v,
+                    const double* xt, double kappa);
+/* Replace the PCG settings of the Newton solve (PCGSolverConfig,
+ * linear_system/solver/pcg_solver.h:6-15; wired at gipc/gipc.cu:52-57):
+ * tolerances, stop norm and solver precision (FP64 or mixed). */
+int sgipc_set_pcg(sgipc_t ctx, const sgipc_pcg_cfg* cfg);
 
 /* LBVH over the surface faces (which = 0) or edges (which = 1) at the current
  * positions (lbvh_f/lbvh_e::Construct, mlbvh.cu:2074-2157).
@@ -144,7 +153,7 @@ typedef struct sgipc_iter_stats {
     int32_t n_ground;         /* ground contacts                       */
     int32_t line_search_steps;
     int32_t converged;        /* Newton stop test met before solving   */
-    int32_t pad;
+    int32_t solver_fallback;  /* 1: the mixed solve was redone in FP64 */
     double  pcg_rel_residual; /* FP64 ||b-Ax||/||b||                   */
     double  alpha;            /* accepted step                         */
     double  alpha_ccd;        /* CCD / ground bound before line search */
@@ -171,7 +180,7 @@ typedef struct sgipc_advance_stats {
     int32_t newton_iterations; /* Newton solves performed (== n requested)  */
     int32_t time_steps_begun;
     int32_t pcg_iterations;
-    int32_t pad;
+    int32_t solver_fallbacks;  /* mixed-mode solves redone in FP64          */
     double  max_pcg_rel_residual;
     double  ms_total;
 } sgipc_advance_stats;

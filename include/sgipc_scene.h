@@ -70,7 +70,8 @@ typedef struct sgipc_scene_desc {
     double  ground_normal[3];
     double  ground_offset;
     double  kappa;          /* barrier stiffness; <= 0 -> adaptive (suggestKappa) */
-    double  friction_mu;    /* reserved: lagged friction                          */
+    double  friction_mu;    /* self-contact friction coefficient (friction_rate)   */
+    double  ground_friction_mu; /* ground friction coefficient (gd_friction_rate)   */
     /* ---- Newton / line search (reference JSON "settings") ---- */
     double  abs_xdelta_tol; /* stop when |dx|_inf < scene_diag * abs_xdelta_tol  */
     double  rel_xdelta_tol; /* ... or < rel_xdelta_tol * |dx0|_inf               */

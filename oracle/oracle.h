@@ -138,6 +138,32 @@ int orc_pcg(const orc_bsr* A, const orc_mas* mas, int precond,
             int max_iter, int use_rz, int check_every, double* rz0_out,
             double* rz_out);
 
+/* ---------------- lagged friction (orc_friction.c) ---------------- */
+/* lagged normal force lam, closest-point coordinates and 3x2 row-major
+ * tangent basis of a contact code at x; returns the kind (0 PP, 1 PE, 2 PT,
+ * 3 EE) or -1 when the code carries no friction */
+int    orc_friction_lag(const double* x, const int32_t* code, double dhat2,
+                        double kappa, double* lam, double coord[2],
+                        double basis[6]);
+double orc_friction_lag_ground(const double* x, const double n[3], double off,
+                               double dhat2, double kappa);
+/* unscaled energy lam f0(|u|) of one lagged pair (u: tangential relative
+ * displacement since xp) */
+double orc_friction_energy(const double* x, const double* xp,
+                           const int32_t* code, const double coord[2],
+                           const double basis[6], double lam, double eps2);
+/* coef-scaled gradient (3m) and PSD Hessian (3m x 3m, row-major); returns m */
+int    orc_friction_grad_hess(const double* x, const double* xp,
+                              const int32_t* code, const double coord[2],
+                              const double basis[6], double lam, double eps2,
+                              double coef, int32_t ids[4], double g[12],
+                              double H[144]);
+double orc_friction_ground_energy(const double* x, const double* xp,
+                                  const double n[3], double lam, double eps2);
+void   orc_friction_ground_grad_hess(const double* x, const double* xp,
+                                     const double n[3], double lam, double eps2,
+                                     double coef, double g[3], double H[9]);
+
 /* ---------------- world / Newton (orc_world.c) ---------------- */
 typedef struct orc_world orc_world;
 orc_world* orc_world_create(const sgipc_scene_desc* s);

@@ -13,7 +13,10 @@
  *   _edgeTriIntersectionQuery, segTriIntersect :8188-8302, :961-1013
  *   _stepForward/_updateVelocities/_computeXTilta :7880-7920, :8016-8038
  * and StiffGIPC/ACCD.cu point_triangle_ccd / edge_edge_ccd :373-524.
- * Friction is not part of this restatement (friction_mu must be 0).
+ * Lagged friction (orc_friction.c) is rebuilt at every step start from the
+ * current contact sets (IPC_Solver's buildFrictionSets, GIPC.cu:11016) and
+ * enters the gradient, Hessian and energy like the reference's USE_FRICTION
+ * path (GIPC.cu:10043-10055, 10442-10445).
  * postLineSearch's kappa doubling is not restated: in the reference its host
  * counters h_close_gpNum / h_close_cpNum are never read back from the device
  * (GIPC.cuh:69-70, GIPC.cu:8737, 9176), so the doubling never fires.
@@ -37,6 +40,12 @@ struct orc_world
     int32_t   counts[5];
     uint32_t* gp;
     double *  gcontact, *gshape;
+    /* lagged friction sets */
+    int       nfr, nfg;
+    int32_t*  fr_code;
+    double *  fr_lam, *fr_coord, *fr_basis;
+    uint32_t* fg_ids;
+    double*   fg_lam;
     int       newton_k;
     double    xdelta0;
     int       in_step;
@@ -676,6 +685,12 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     w->gp    = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf_verts + 1));
     w->gcontact = (double*)calloc(3 * nv, sizeof(double));
     w->gshape   = (double*)calloc(3 * nv, sizeof(double));
+    w->fr_code  = (int32_t*)malloc(sizeof(int32_t) * 4 * w->cap);
+    w->fr_lam   = (double*)malloc(sizeof(double) * w->cap);
+    w->fr_coord = (double*)malloc(sizeof(double) * 2 * w->cap);
+    w->fr_basis = (double*)malloc(sizeof(double) * 6 * w->cap);
+    w->fg_ids   = (uint32_t*)malloc(sizeof(uint32_t) * (s->n_surf_verts + 1));
+    w->fg_lam   = (double*)malloc(sizeof(double) * (s->n_surf_verts + 1));
     /* x_tilde for the first step (main.cu:1037-1045 computeXTilta) */
     for(int v = 0; v < nv; v++)
         for(int c = 0; c < 3; c++)
@@ -695,6 +710,8 @@ void orc_world_free(orc_world* w)
     free(w->tets); free(w->faces); free(w->edges); free(w->surf); free(w->dminv);
     free(w->vol); free(w->mu); free(w->lam); free(w->pairs); free(w->ccd); free(w->gp);
     free(w->gcontact); free(w->gshape);
+    free(w->fr_code); free(w->fr_lam); free(w->fr_coord); free(w->fr_basis);
+    free(w->fg_ids); free(w->fg_lam);
     free(w);
 }
 
@@ -774,6 +791,12 @@ static void push_element(trip_t* t, int m, const int* ids, const double* H, int 
         }
 }
 
+/* eps_v^2 = fDhat dt^2, fDhat = 1e-4 bboxDiag^2 (GIPC.cu:8544, :9128) */
+static double friction_eps2(const orc_world* w)
+{
+    return 1e-4 * w->bbox_diag2 * w->s.dt * w->s.dt;
+}
+
 static void compute_grad_hess(orc_world* w, trip_t* T)
 {
     int nv = w->nv;
@@ -793,6 +816,35 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
                 w->gcontact[3 * ids[a] + c] += g[3 * a + c];
         if(T)
             push_element(T, m, ids, H, 1, w);
+    }
+    /* lagged friction (GIPC.cu:10043-10055: calFrictionGradient, calFrictionHessian) */
+    double eps2 = friction_eps2(w);
+    for(int i = 0; i < w->nfr; i++)
+    {
+        int32_t ids[4];
+        double  g[12], H[144];
+        int     m = orc_friction_grad_hess(w->x, w->xprev, w->fr_code + 4 * i, w->fr_coord + 2 * i,
+                                           w->fr_basis + 6 * i, w->fr_lam[i], eps2, w->s.friction_mu,
+                                           ids, g, H);
+        for(int a = 0; a < m; a++)
+            for(int c = 0; c < 3; c++)
+                w->gcontact[3 * ids[a] + c] += g[3 * a + c];
+        if(T)
+            push_element(T, m, ids, H, 1, w);
+    }
+    for(int i = 0; i < w->nfg; i++)
+    {
+        int    v = (int)w->fg_ids[i];
+        double g[3], H[9];
+        orc_friction_ground_grad_hess(w->x + 3 * v, w->xprev + 3 * v, w->s.ground_normal,
+                                      w->fg_lam[i], eps2, w->s.ground_friction_mu, g, H);
+        for(int c = 0; c < 3; c++)
+            w->gcontact[3 * v + c] += g[c];
+        if(T)
+        {
+            int ids[1] = {v};
+            push_element(T, 1, ids, H, 1, w);
+        }
     }
     /* ground */
     for(int i = 0; i < w->ngp; i++)
@@ -878,12 +930,20 @@ void orc_world_energy(orc_world* w, double* terms)
         gnd += orc_ground_energy(d * d, w->dhat2);
     }
     gnd *= w->kappa;
-    terms[0] = kin + ela + bar + gnd;
+    double eps2 = friction_eps2(w), fs = 0.0, fg = 0.0;
+    for(int i = 0; i < w->nfr; i++)
+        fs += orc_friction_energy(w->x, w->xprev, w->fr_code + 4 * i, w->fr_coord + 2 * i,
+                                  w->fr_basis + 6 * i, w->fr_lam[i], eps2);
+    for(int i = 0; i < w->nfg; i++)
+        fg += orc_friction_ground_energy(w->x + 3 * w->fg_ids[i], w->xprev + 3 * w->fg_ids[i], n,
+                                         w->fg_lam[i], eps2);
+    double fric = w->s.friction_mu * fs + w->s.ground_friction_mu * fg;
+    terms[0] = kin + ela + bar + gnd + fric;
     terms[1] = kin;
     terms[2] = ela;
     terms[3] = bar;
     terms[4] = gnd;
-    terms[5] = 0.0;
+    terms[5] = fric;
     terms[6] = 0.0;
     terms[7] = 0.0;
 }
@@ -1155,12 +1215,40 @@ int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_o
     return 0;
 }
 
+/* buildFrictionSets (GIPC.cu:8601-8637) on the current contact sets */
+static void build_friction_sets(orc_world* w)
+{
+    w->nfr = 0;
+    w->nfg = 0;
+    if(w->s.friction_mu != 0.0)
+        for(int i = 0; i < w->npairs; i++)
+        {
+            double lam, c[2], B[6];
+            if(orc_friction_lag(w->x, w->pairs + 4 * i, w->dhat2, w->kappa, &lam, c, B) < 0)
+                continue;
+            int k = w->nfr++;
+            memcpy(w->fr_code + 4 * k, w->pairs + 4 * i, sizeof(int32_t) * 4);
+            w->fr_lam[k] = lam;
+            memcpy(w->fr_coord + 2 * k, c, sizeof(c));
+            memcpy(w->fr_basis + 6 * k, B, sizeof(B));
+        }
+    if(w->s.ground_friction_mu != 0.0 && w->s.use_ground)
+        for(int i = 0; i < w->ngp; i++)
+        {
+            int k         = w->nfg++;
+            w->fg_ids[k]  = w->gp[i];
+            w->fg_lam[k]  = orc_friction_lag_ground(w->x + 3 * w->gp[i], w->s.ground_normal,
+                                                   w->s.ground_offset, w->dhat2, w->kappa);
+        }
+}
+
 static void begin_step(orc_world* w)
 {
     upper_bound_kappa(w, &w->kappa);
     if(w->kappa < 1e-16)
         suggest_kappa(w, &w->kappa);
     init_kappa(w);
+    build_friction_sets(w);
     memset(w->dx, 0, sizeof(double) * 3 * w->nv);
     w->newton_k = 0;
     w->xdelta0  = 0.0;

@@ -107,6 +107,17 @@ def lib():
         L.orc_world_time_step.argtypes = [C.c_void_p, ip, ip]
         L.orc_world_set_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_double]
         L.orc_world_advance.argtypes = [C.c_void_p, C.c_int, ip, ip]
+        L.orc_friction_lag.argtypes = [dp, ip, C.c_double, C.c_double, dp, dp, dp]
+        L.orc_friction_lag_ground.restype = C.c_double
+        L.orc_friction_lag_ground.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double]
+        L.orc_friction_energy.restype = C.c_double
+        L.orc_friction_energy.argtypes = [dp, dp, ip, dp, dp, C.c_double, C.c_double]
+        L.orc_friction_grad_hess.argtypes = [dp, dp, ip, dp, dp, C.c_double, C.c_double,
+                                             C.c_double, ip, dp, dp]
+        L.orc_friction_ground_energy.restype = C.c_double
+        L.orc_friction_ground_energy.argtypes = [dp, dp, dp, C.c_double, C.c_double]
+        L.orc_friction_ground_grad_hess.argtypes = [dp, dp, dp, C.c_double, C.c_double,
+                                                    C.c_double, dp, dp]
         L.orc_pt_ccd.restype = C.c_double
         L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         L.orc_ee_ccd.restype = C.c_double
@@ -386,3 +397,54 @@ def ground(x, n, off, dhat2, kappa):
     lib().orc_ground_grad_hess(P(x), P(n), off, dhat2, kappa, P(g), P(H))
     d = float(n @ x - off)
     return lib().orc_ground_energy(d * d, dhat2) * kappa, g, H.reshape(3, 3)
+
+
+# ---------------------------------------------------------------- friction
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def friction_lag(x, code, dhat2, kappa):
+    """(kind, lam, coord[2], basis 3x2) of a contact code at x; kind -1: none."""
+    x = _f64(x).ravel()
+    c = np.ascontiguousarray(code, np.int32)
+    lam = np.zeros(1)
+    coord = np.zeros(2)
+    basis = np.zeros(6)
+    kind = lib().orc_friction_lag(P(x), P(c, C.c_int32), dhat2, kappa, P(lam), P(coord),
+                                  P(basis))
+    return kind, float(lam[0]), coord, basis.reshape(3, 2)
+
+
+def friction_energy(x, xp, code, coord, basis, lam, eps2):
+    return lib().orc_friction_energy(P(_f64(x).ravel()), P(_f64(xp).ravel()),
+                                     P(np.ascontiguousarray(code, np.int32), C.c_int32),
+                                     P(_f64(coord)), P(_f64(basis).ravel()), lam, eps2)
+
+
+def friction_grad_hess(x, xp, code, coord, basis, lam, eps2, coef):
+    ids = np.zeros(4, np.int32)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    m = lib().orc_friction_grad_hess(P(_f64(x).ravel()), P(_f64(xp).ravel()),
+                                     P(np.ascontiguousarray(code, np.int32), C.c_int32),
+                                     P(_f64(coord)), P(_f64(basis).ravel()), lam, eps2, coef,
+                                     P(ids, C.c_int32), P(g), P(H))
+    n = 3 * m
+    return ids[:m].copy(), g[:n].copy(), H[:n * n].reshape(n, n).copy()
+
+
+def friction_lag_ground(x, n, off, dhat2, kappa):
+    return lib().orc_friction_lag_ground(P(_f64(x)), P(_f64(n)), off, dhat2, kappa)
+
+
+def friction_ground_energy(x, xp, n, lam, eps2):
+    return lib().orc_friction_ground_energy(P(_f64(x)), P(_f64(xp)), P(_f64(n)), lam, eps2)
+
+
+def friction_ground_grad_hess(x, xp, n, lam, eps2, coef):
+    g = np.zeros(3)
+    H = np.zeros(9)
+    lib().orc_friction_ground_grad_hess(P(_f64(x)), P(_f64(xp)), P(_f64(n)), lam, eps2, coef,
+                                        P(g), P(H))
+    return g, H.reshape(3, 3)

@@ -159,8 +159,9 @@ int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x, const sgipc_pcg_cfg*
 {
     return guard([&] {
         SG_CHECK(ls && b && x && cfg, SGIPC_EINVAL, "null argument");
-        SG_CHECK(cfg->solver_precision == SGIPC_SOLVE_FP64, SGIPC_EINVAL,
-                 "only the FP64 solver precision is available for stand-alone systems");
+        SG_CHECK(cfg->solver_precision == SGIPC_SOLVE_FP64
+                     || cfg->solver_precision == SGIPC_SOLVE_MIXED,
+                 SGIPC_EINVAL, "unknown solver precision");
         size_t n = 3 * (size_t)ls->A.nb;
         ls->b.upload(b, n, ls->s);
         ls->x.resize(n);
@@ -170,6 +171,7 @@ int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x, const sgipc_pcg_cfg*
         pc.max_iter    = cfg->max_iter;
         pc.use_rz      = cfg->use_preconditioned_norm != 0;
         pc.check_every = cfg->check_every;
+        pc.mixed       = cfg->solver_precision == SGIPC_SOLVE_MIXED;
         hipEvent_t e0, e1;
         SG_HIP(hipEventCreate(&e0));
         SG_HIP(hipEventCreate(&e1));
@@ -187,7 +189,7 @@ int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x, const sgipc_pcg_cfg*
         if(stats)
         {
             stats->iterations       = st.iterations;
-            stats->outer_iterations = 1;
+            stats->outer_iterations = st.outer_iterations;
             stats->converged        = st.converged ? 1 : 0;
             stats->rel_residual     = st.rel_residual;
             stats->rz0              = st.rz0;
@@ -316,6 +318,22 @@ int sgipc_get_kappa(sgipc_t h, double* k)
 int sgipc_set_kappa(sgipc_t h, double k)
 {
     return guard([&] { ctx_of(h).set_kappa(k); });
+}
+
+int sgipc_set_state(sgipc_t h, const double* x, const double* v, const double* xt, double kappa)
+{
+    return guard([&] {
+        SG_CHECK(x && v && xt, SGIPC_EINVAL, "null state array");
+        ctx_of(h).set_state(x, v, xt, kappa);
+    });
+}
+
+int sgipc_set_pcg(sgipc_t h, const sgipc_pcg_cfg* cfg)
+{
+    return guard([&] {
+        SG_CHECK(cfg, SGIPC_EINVAL, "null config");
+        ctx_of(h).set_pcg(*cfg);
+    });
 }
 
 int sgipc_lbvh_build(sgipc_t h, int32_t which, uint64_t* keys, uint32_t* nodes, double* aabbs)

@@ -17,6 +17,9 @@ class Context
 
     int  nv() const { return nv_; }
     void set_positions(const double* x);
+    // state at a time-step boundary (x_prev = x), as the oracle's set_state
+    void set_state(const double* x, const double* v, const double* xt, double kappa);
+    void set_pcg(const sgipc_pcg_cfg& c);
     void get(const DBuf<double>& b, double* out);
     void set_xtilde(const double* xt);
     void get_xtilde(double* out) { get(xt_, out); }
@@ -71,7 +74,14 @@ class Context
 
     DBuf<double>   xr_, xprev_, xt_, mass_, dx_, xsave_, gc_, gs_, rhs_, scratch_;
     DBuf<double>   dminv_, vol_, mu_, lam_;
-    DBuf<int>      btype_, body_, tslot_;
+    DBuf<int>      btype_, body_;
+    DBuf<int>      gh_up_, gh_lo_, gh_ptr_, gh_src_, gv_ptr_, gv_src_;  // FEM gather lists
+    DBuf<double>   fem_buf_;                                            // per-tet blocks
+    DBuf<int4>     fr_code_;                                            // lagged friction
+    DBuf<double>   fr_lam_, fr_coord_, fr_basis_, fg_lam_;
+    DBuf<uint32_t> fg_ids_;
+    DBuf<unsigned> fr_count_;                                           // [0] self, [1] ground
+    int            n_up_ = 0;
     DBuf<uint32_t> tets_, faces_, edges_, surf_, gp_;
     DBuf<unsigned> gcount_, pcount_, ccount_;
     DBuf<int4>     pairs_, pairs_raw_, cand_;

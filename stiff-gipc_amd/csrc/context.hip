@@ -9,7 +9,7 @@
 //                    postLineSearch's doubling never fires in the reference
 //                    (h_close_*Num are never read back, GIPC.cuh:69-70) and is
 //                    therefore not reproduced
-//   time step        IPC_Solver (:10940-11131) without friction
+//   time step        IPC_Solver (:10940-11131) with lagged friction
 // All state stays on the device; the host reads back only the scalars that
 // steer control flow (contact counts at capacity checks, step bounds, line
 // search energies, PCG convergence tests).
@@ -44,7 +44,8 @@ Context::Context(const sgipc_scene_desc& d)
     SG_CHECK(ne_ == 0 || d.edges, SGIPC_EINVAL, "edges missing");
     SG_CHECK(ns_ == 0 || d.surf_verts, SGIPC_EINVAL, "surface vertices missing");
     SG_CHECK(d.dt > 0 && d.dhat > 0, SGIPC_EINVAL, "dt and dhat must be positive");
-    SG_CHECK(d.friction_mu == 0.0, SGIPC_EINVAL, "friction is not available (friction_mu must be 0)");
+    SG_CHECK(d.friction_mu >= 0.0 && d.ground_friction_mu >= 0.0, SGIPC_EINVAL,
+             "friction coefficients must be non-negative");
     for(int e = 0; e < 4 * nt_; e++)
         SG_CHECK(d.tets[e] < (uint32_t)nv_, SGIPC_EINVAL, "tet vertex index out of range");
     for(int e = 0; e < 3 * nf_; e++)
@@ -138,6 +139,75 @@ Context::Context(const sgipc_scene_desc& d)
     }
     A_.set_pattern(nv_, rp, cl, s_);
 
+    // ---- FEM gather lists (k_fem_gather_hess / k_fem_gather_grad): every
+    // upper block (i <= j) of the static pattern lists the tet blocks that
+    // add into it (boundary vertices carry no elastic Hessian); every vertex
+    // lists the tet gradient entries that add into it
+    {
+        std::vector<long long> key;  // upper slot << 32 | code
+        key.reserve((size_t)10 * nt_);
+        for(int e = 0; e < nt_; e++)
+        {
+            const uint32_t* t = d.tets + 4 * (size_t)e;
+            for(int pi = 0; pi < 10; pi++)
+            {
+                int a = kPairA[pi], b = kPairB[pi];
+                int va = (int)t[a], vb = (int)t[b];
+                if(bt[va] != 0 || bt[vb] != 0)
+                    continue;
+                bool tr   = va > vb;
+                int  slot = tr ? ts[16 * (size_t)e + 4 * b + a] : ts[16 * (size_t)e + 4 * a + b];
+                int  code = e * 32 + pi * 2 + (tr ? 1 : 0);
+                key.push_back(((long long)slot << 32) | (unsigned)code);
+            }
+        }
+        SG_CHECK((long long)nt_ * 32 < (1ll << 31), SGIPC_EINVAL, "too many tets for the gather codes");
+        std::sort(key.begin(), key.end());
+        std::vector<int> up, lo, ptr, src;
+        src.reserve(key.size());
+        for(size_t k = 0; k < key.size(); k++)
+        {
+            int slot = (int)(key[k] >> 32);
+            if(k == 0 || slot != (int)(key[k - 1] >> 32))
+            {
+                ptr.push_back((int)src.size());
+                up.push_back(slot);
+            }
+            src.push_back((int)(key[k] & 0xffffffffll));
+        }
+        ptr.push_back((int)src.size());
+        // transpose slot of each upper block
+        std::vector<int> rowof(rp[nv_]);
+        for(int v = 0; v < nv_; v++)
+            for(int k = rp[v]; k < rp[v + 1]; k++)
+                rowof[k] = v;
+        lo.resize(up.size());
+        for(size_t u = 0; u < up.size(); u++)
+        {
+            int i = rowof[up[u]], j = cl[up[u]];
+            auto beg = cl.begin() + rp[j], end = cl.begin() + rp[j + 1];
+            lo[u] = (int)(std::lower_bound(beg, end, i) - cl.begin());
+        }
+        std::vector<int> vptr(nv_ + 1, 0), vsrc((size_t)4 * nt_);
+        for(int e = 0; e < nt_; e++)
+            for(int a = 0; a < 4; a++)
+                vptr[d.tets[4 * (size_t)e + a] + 1]++;
+        for(int v = 0; v < nv_; v++)
+            vptr[v + 1] += vptr[v];
+        std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+        for(int e = 0; e < nt_; e++)
+            for(int a = 0; a < 4; a++)
+                vsrc[fill[d.tets[4 * (size_t)e + a]]++] = e * 4 + a;
+        gh_up_.upload(up, s_);
+        gh_lo_.upload(lo, s_);
+        gh_ptr_.upload(ptr, s_);
+        gh_src_.upload(src, s_);
+        gv_ptr_.upload(vptr, s_);
+        gv_src_.upload(vsrc, s_);
+        n_up_ = (int)up.size();
+        fem_buf_.resize((size_t)kFemStride * (nt_ > 0 ? nt_ : 1));
+    }
+
     // ---- scene extents (GIPC::init, main.cu:853-863)
     double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
     for(int f = 0; f < nf_; f++)
@@ -199,7 +269,6 @@ Context::Context(const sgipc_scene_desc& d)
     lam_.upload(lam, s_);
     btype_.upload(bt, s_);
     body_.upload(body, s_);
-    tslot_.upload(ts, s_);
     tets_.upload(d.tets, 4 * (size_t)nt_, s_);
     faces_.upload(d.faces, 3 * (size_t)nf_, s_);
     edges_.upload(d.edges, 2 * (size_t)ne_, s_);
@@ -226,7 +295,10 @@ Context::Context(const sgipc_scene_desc& d)
     dev_.gshape = gs_.data(), dev_.gcontact = gc_.data(), dev_.vals = A_.vals.data();
     dev_.dminv = dminv_.data(), dev_.vol = vol_.data(), dev_.mu = mu_.data(), dev_.lam = lam_.data();
     dev_.btype = btype_.data(), dev_.body = body_.data(), dev_.diag = A_.diag.data();
-    dev_.tslot = tslot_.data(), dev_.rowptr = A_.rowptr.data(), dev_.cols = A_.cols.data();
+    dev_.rowptr = A_.rowptr.data(), dev_.cols = A_.cols.data();
+    dev_.fem_buf = fem_buf_.data(), dev_.n_up = n_up_;
+    dev_.gh_up = gh_up_.data(), dev_.gh_lo = gh_lo_.data(), dev_.gh_ptr = gh_ptr_.data();
+    dev_.gh_src = gh_src_.data(), dev_.gv_ptr = gv_ptr_.data(), dev_.gv_src = gv_src_.data();
     dev_.tets = tets_.data(), dev_.surf = surf_.data(), dev_.faces = faces_.data();
     dev_.edges = edges_.data();
     dev_.gp = gp_.data(), dev_.gcount = gcount_.data();
@@ -237,8 +309,27 @@ Context::Context(const sgipc_scene_desc& d)
     dev_.overflow = overflow_.data();
     coo_.count.zero(s_);
 
+    // lagged friction sets (built at every step start)
+    fr_code_.resize(pair_cap_);
+    fr_lam_.resize(pair_cap_);
+    fr_coord_.resize(2 * (size_t)pair_cap_);
+    fr_basis_.resize(6 * (size_t)pair_cap_);
+    fr_count_.resize(2);
+    fr_count_.zero(s_);
+    fg_ids_.resize(ns_ + 1);
+    fg_lam_.resize(ns_ + 1);
+    FricDev& f = dev_.fric;
+    f.mu       = d.friction_mu;
+    f.mu_gd    = d.ground_friction_mu;
+    f.eps2     = 1e-4 * bbox_diag2_ * d.dt * d.dt;  // fDhat dt^2 (GIPC.cu:8544, :9128)
+    f.xprev    = xprev_.data();
+    f.cap      = pair_cap_;
+    f.code = fr_code_.data(), f.lam = fr_lam_.data(), f.coord = fr_coord_.data();
+    f.basis = fr_basis_.data(), f.count = fr_count_.data();
+    f.gids = fg_ids_.data(), f.glam = fg_lam_.data(), f.gcount = fr_count_.data() + 1;
+
     EnergyLayout L = energy_layout(dev_);
-    epart_.resize(L.off[3] + L.n[3] + 1);
+    epart_.resize(L.off[kEnergyTerms - 1] + L.n[kEnergyTerms - 1] + 1);
     eterms_.resize(8);
 
     if(d.precond == SGIPC_PRECOND_MAS)
@@ -295,6 +386,34 @@ void Context::set_positions(const double* x)
     SG_HIP(hipStreamSynchronize(s_));
 }
 
+void Context::set_state(const double* x, const double* v, const double* xt, double kappa)
+{
+    size_t n3 = 3 * (size_t)nv_;
+    x_.upload(x, n3, s_);
+    xprev_.upload(x, n3, s_);
+    v_.upload(v, n3, s_);
+    xt_.upload(xt, n3, s_);
+    kappa_   = kappa;
+    in_step_ = false;
+    newton_k_ = 0;
+    dx_.zero(s_);
+    build_bvh();
+    build_cp();
+    SG_HIP(hipStreamSynchronize(s_));
+}
+
+void Context::set_pcg(const sgipc_pcg_cfg& c)
+{
+    SG_CHECK(c.rel_tol >= 0.0 && c.abs_tol >= 0.0, SGIPC_EINVAL, "negative PCG tolerance");
+    SG_CHECK(c.solver_precision == SGIPC_SOLVE_FP64 || c.solver_precision == SGIPC_SOLVE_MIXED,
+             SGIPC_EINVAL, "unknown solver precision");
+    desc_.pcg_rel_tol                 = c.rel_tol;
+    desc_.pcg_abs_tol                 = c.abs_tol;
+    desc_.pcg_max_iter                = c.max_iter;
+    desc_.pcg_use_preconditioned_norm = c.use_preconditioned_norm;
+    desc_.solver_precision            = c.solver_precision;
+}
+
 void Context::get(const DBuf<double>& b, double* out)
 {
     b.download(out, 3 * (size_t)nv_, s_);
@@ -329,6 +448,7 @@ void Context::assemble(bool hessian)
     }
     launch_init_diag(dev_, s_);
     launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
+    launch_friction(dev_, hessian, gc_.data(), s_);
     launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
     prof_.begin("fem", s_);
     launch_fem(dev_, hessian, s_);
@@ -341,10 +461,10 @@ void Context::assemble(bool hessian)
 double Context::energy()
 {
     launch_energy(dev_, kappa_, epart_.data(), eterms_.data(), s_);
-    double t[5];
+    double t[kEnergyTerms + 1];
     SG_HIP(hipMemcpyAsync(t, eterms_.data(), sizeof(t), hipMemcpyDeviceToHost, s_));
     SG_HIP(hipStreamSynchronize(s_));
-    return t[4];
+    return t[kEnergyTerms];
 }
 
 bool Context::is_intersected()
@@ -544,9 +664,11 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     cfg.abs_tol      = desc_.pcg_abs_tol;
     cfg.max_iter     = desc_.pcg_max_iter;
     cfg.use_rz       = desc_.pcg_use_preconditioned_norm != 0;
+    cfg.mixed        = desc_.solver_precision == SGIPC_SOLVE_MIXED;
     PcgStats ps      = pcg_.solve(S, pc, &mas_, rhs_.data(), dx_.data(), cfg, s_);
     loc.pcg_iterations   = ps.iterations;
     loc.pcg_rel_residual = ps.rel_residual;
+    loc.solver_fallback  = ps.outer_iterations > 1 ? 1 : 0;
     SG_HIP(hipEventRecord(ev[2], s_));
     // ---- step bounds
     double alpha = 1.0;
@@ -594,6 +716,8 @@ void Context::begin_step()
     if(kappa_ < 1e-16)
         suggest_kappa(kappa_);
     init_kappa();
+    // buildFrictionSets on the current contact sets (GIPC.cu:11016)
+    launch_friction_build(dev_, kappa_, s_);
     dx_.zero(s_);
     newton_k_ = 0;
     xdelta0_  = 0;
@@ -630,6 +754,7 @@ void Context::advance(int n, sgipc_advance_stats* st)
         }
         loc.newton_iterations++;
         loc.pcg_iterations += is.pcg_iterations;
+        loc.solver_fallbacks += is.solver_fallback;
         loc.max_pcg_rel_residual = std::max(loc.max_pcg_rel_residual, is.pcg_rel_residual);
         if(newton_k_ >= cap)
             end_step();
@@ -707,15 +832,16 @@ void Context::time_step(sgipc_step_stats* st)
 void Context::energy_terms(double* terms)
 {
     launch_energy(dev_, kappa_, epart_.data(), eterms_.data(), s_);
-    double t[5];
+    double t[kEnergyTerms + 1];
     SG_HIP(hipMemcpyAsync(t, eterms_.data(), sizeof(t), hipMemcpyDeviceToHost, s_));
     SG_HIP(hipStreamSynchronize(s_));
-    terms[0] = t[4];
+    terms[0] = t[kEnergyTerms];
     terms[1] = t[0];
     terms[2] = t[1];
     terms[3] = t[2];
     terms[4] = t[3];
-    terms[5] = terms[6] = terms[7] = 0.0;
+    terms[5] = t[4] + t[5];
+    terms[6] = terms[7] = 0.0;
 }
 
 void Context::lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs)

@@ -2,9 +2,11 @@
 //
 //   k_init_diag   kinetic gradient m(x - x~) and the m I diagonal blocks
 //                 (_calKineticGradient GIPC.cu:6341, mass triplets :10201-10220)
-//   k_fem         SNK1 tet gradient and PSD Hessian, 16 blocks per tet scattered
-//                 into the static BSR (_calculate_fem_gradient_hessian,
-//                 femEnergy.cu:1789-1926; PK1 :443-486; Hessian :1706-1786)
+//   k_fem         SNK1 tet gradient and PSD Hessian (_calculate_fem_gradient_hessian,
+//                 femEnergy.cu:1789-1926; PK1 :443-486; Hessian :1706-1786),
+//                 written per tet to a buffer; k_fem_gather_* add the 10 upper
+//                 blocks into the static BSR (both orientations) and the
+//                 gradient into the vertices without atomics
 //   k_ground_*    ground IPC barrier (GIPC.cu:6435-6516)
 //   k_barrier     GIPC rank-2 barrier, gradient b'(s) ds/dx and Hessian
 //                 lambda0 ds/dx ds/dx^T, s = d/sqrt(dhat^2)
@@ -13,10 +15,10 @@
 //                 with fixed-order two-pass reductions (run-to-run identical).
 //
 // Layout: per-tet constants are SoA (dminv[k * nt + e]) so each of the nine
-// loads is a coalesced 512-byte wave access; the static BSR slot of every
-// (a, b) vertex pair of a tet is precomputed (16 ints per tet).
-#include "dmath.h"
-#include "sim_kernels.h"
+// loads is a coalesced 512-byte wave access.  FP64 atomics from every tet
+// into shared BSR blocks are avoided: each BSR block (and vertex) owns a
+// precomputed list of the tet blocks that add into it.
+#include "assembly.h"
 
 namespace sgipc
 {
@@ -39,24 +41,6 @@ __global__ void k_init_diag(int nv, const double* __restrict__ mass, const doubl
     }
 }
 
-__device__ __forceinline__ void add_block(double* dst, const double b[9], bool transpose)
-{
-    if(!transpose)
-    {
-#pragma unroll
-        for(int e = 0; e < 9; e++)
-            atomicAdd(dst + e, b[e]);
-    }
-    else
-    {
-#pragma unroll
-        for(int r = 0; r < 3; r++)
-#pragma unroll
-            for(int c = 0; c < 3; c++)
-                atomicAdd(dst + 3 * r + c, b[3 * c + r]);
-    }
-}
-
 // upper-triangle index of (p, q), p <= q < 12
 __device__ __forceinline__ int uidx(int p, int q) { return p * 12 - p * (p - 1) / 2 + (q - p); }
 
@@ -66,10 +50,7 @@ __global__ void __launch_bounds__(128) k_fem(int nt, const uint32_t* __restrict_
                                              const double* __restrict__ mu_,
                                              const double* __restrict__ lam_,
                                              const double* __restrict__ x, double dt2,
-                                             const int* __restrict__ btype,
-                                             const int* __restrict__ tslot,
-                                             double* __restrict__ vals,
-                                             double* __restrict__ gshape, int hessian)
+                                             double* __restrict__ out, int hessian)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if(e >= nt)
@@ -119,16 +100,14 @@ __global__ void __launch_bounds__(128) k_fem(int nt, const uint32_t* __restrict_
         B[6 + j]     = Dm[3 + j];
         B[9 + j]     = Dm[6 + j];
     }
-    double s = dt2 * vol[e];
+    double  s  = dt2 * vol[e];
+    double* ob = out + (size_t)kFemStride * e;
 #pragma unroll
     for(int a = 0; a < 4; a++)
 #pragma unroll
         for(int i = 0; i < 3; i++)
-        {
-            double g = s * (P[i * 3 + 0] * B[a * 3 + 0] + P[i * 3 + 1] * B[a * 3 + 1]
-                            + P[i * 3 + 2] * B[a * 3 + 2]);
-            atomicAdd(gshape + 3 * t[a] + i, g);
-        }
+            ob[90 + 3 * a + i] = s * (P[i * 3 + 0] * B[a * 3 + 0] + P[i * 3 + 1] * B[a * 3 + 1]
+                                      + P[i * 3 + 2] * B[a * 3 + 2]);
     if(!hessian)
         return;
 
@@ -204,29 +183,93 @@ __global__ void __launch_bounds__(128) k_fem(int nt, const uint32_t* __restrict_
             rank1(Q, w3[k]);
         }
     }
-    // ---- scatter 16 blocks
-    int bt[4] = {btype[t[0]], btype[t[1]], btype[t[2]], btype[t[3]]};
-    const int* sl = tslot + 16 * (size_t)e;
+    // ---- the 10 upper blocks (a <= b), row-major
 #pragma unroll
-    for(int a = 0; a < 4; a++)
+    for(int pi = 0; pi < 10; pi++)
+    {
+        const int a = kPairA[pi], b = kPairB[pi];
 #pragma unroll
-        for(int b = a; b < 4; b++)
+        for(int r = 0; r < 3; r++)
+#pragma unroll
+            for(int c = 0; c < 3; c++)
+            {
+                int p = 3 * a + r, q = 3 * b + c;
+                ob[9 * pi + 3 * r + c] = (p <= q) ? H[uidx(p, q)] : H[uidx(q, p)];
+            }
+    }
+}
+
+// One thread per upper BSR block (i <= j): sum of the tet blocks listed for
+// it (code = tet * 32 + pair * 2 + transposed) added into (i, j) and, for
+// i != j, its transpose into (j, i).
+__global__ void __launch_bounds__(256) k_fem_gather_hess(int n_up, const int* __restrict__ up,
+                                                         const int* __restrict__ lo,
+                                                         const int* __restrict__ ptr,
+                                                         const int* __restrict__ src,
+                                                         const double* __restrict__ buf,
+                                                         double* __restrict__ vals)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if(u >= n_up)
+        return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for(int k = ptr[u]; k < ptr[u + 1]; k++)
+    {
+        int           code = src[k];
+        const double* b    = buf + (size_t)kFemStride * (code >> 5) + 9 * ((code >> 1) & 15);
+        if(code & 1)
         {
-            if(bt[a] != 0 || bt[b] != 0)
-                continue;
-            double blk[9];
 #pragma unroll
             for(int r = 0; r < 3; r++)
 #pragma unroll
                 for(int c = 0; c < 3; c++)
-                {
-                    int p = 3 * a + r, q = 3 * b + c;
-                    blk[r * 3 + c] = (p <= q) ? H[uidx(p, q)] : H[uidx(q, p)];
-                }
-            add_block(vals + 9 * (size_t)sl[4 * a + b], blk, false);
-            if(a != b)
-                add_block(vals + 9 * (size_t)sl[4 * b + a], blk, true);
+                    acc[3 * r + c] += b[3 * c + r];
         }
+        else
+        {
+#pragma unroll
+            for(int q = 0; q < 9; q++)
+                acc[q] += b[q];
+        }
+    }
+    double* d = vals + 9 * (size_t)up[u];
+#pragma unroll
+    for(int q = 0; q < 9; q++)
+        d[q] += acc[q];
+    int l = lo[u];
+    if(l != up[u])
+    {
+        double* dt = vals + 9 * (size_t)l;
+#pragma unroll
+        for(int r = 0; r < 3; r++)
+#pragma unroll
+            for(int c = 0; c < 3; c++)
+                dt[3 * r + c] += acc[3 * c + r];
+    }
+}
+
+// One thread per vertex: the tet gradient entries listed for it
+// (code = tet * 4 + local vertex) added into the shape gradient.
+__global__ void __launch_bounds__(256) k_fem_gather_grad(int nv, const int* __restrict__ ptr,
+                                                         const int* __restrict__ src,
+                                                         const double* __restrict__ buf,
+                                                         double* __restrict__ gshape)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if(v >= nv)
+        return;
+    double g0 = 0, g1 = 0, g2 = 0;
+    for(int k = ptr[v]; k < ptr[v + 1]; k++)
+    {
+        int           code = src[k];
+        const double* b    = buf + (size_t)kFemStride * (code >> 2) + 90 + 3 * (code & 3);
+        g0 += b[0];
+        g1 += b[1];
+        g2 += b[2];
+    }
+    gshape[3 * (size_t)v] += g0;
+    gshape[3 * (size_t)v + 1] += g1;
+    gshape[3 * (size_t)v + 2] += g2;
 }
 
 // ----------------------------------------------------------------- ground
@@ -278,25 +321,6 @@ __global__ void k_ground(const unsigned* __restrict__ gcount, const uint32_t* __
 }
 
 // --------------------------------------------------------------- barrier
-// decode a contact code into type (0 PP, 1 PE, 2 PT, 3 EE) and vertex ids
-__device__ __forceinline__ int decode_pair(int4 c, int ids[4])
-{
-    if(c.x >= 0)
-    {
-        ids[0] = c.x, ids[1] = c.y, ids[2] = c.z, ids[3] = c.w;
-        return 3;
-    }
-    ids[0] = -c.x - 1;
-    ids[1] = c.y;
-    if(c.z < 0)
-        return 0;
-    ids[2] = c.z;
-    if(c.w < 0)
-        return 1;
-    ids[3] = c.w;
-    return 2;
-}
-
 __device__ __forceinline__ double pair_d2(int type, const double* x, const int ids[4])
 {
     V3 a = ld3(x + 3 * ids[0]), b = ld3(x + 3 * ids[1]);
@@ -369,34 +393,10 @@ __device__ __forceinline__ double barrier_lambda0(double I5, double dHat, double
            / I5;
 }
 
-__device__ __forceinline__ int find_slot(const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                         int r, int c)
-{
-    int lo = rowptr[r], hi = rowptr[r + 1] - 1;
-    while(lo <= hi)
-    {
-        int m  = (lo + hi) >> 1;
-        int cm = cols[m];
-        if(cm == c)
-            return m;
-        if(cm < c)
-            lo = m + 1;
-        else
-            hi = m - 1;
-    }
-    return -1;
-}
-
 __global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ count, int pair_cap,
                                                  const int4* __restrict__ pairs,
                                                  const double* __restrict__ x, double dHat,
-                                                 double kappa, const int* __restrict__ btype,
-                                                 const int* __restrict__ rowptr,
-                                                 const int* __restrict__ cols,
-                                                 double* __restrict__ vals, int* __restrict__ coo_r,
-                                                 int* __restrict__ coo_c, double* __restrict__ coo_v,
-                                                 unsigned* __restrict__ coo_count, int coo_cap,
-                                                 int* __restrict__ overflow,
+                                                 double kappa, BlockSink sink,
                                                  double* __restrict__ gcontact, int hessian)
 {
     unsigned pi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -428,41 +428,11 @@ __global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ co
     for(int a = 0; a < nv; a++)
         for(int b = a; b < nv; b++)
         {
-            int  ra = ids[a], rb = ids[b];
-            bool zero = btype[ra] != 0 || btype[rb] != 0;
             double blk[9];
             for(int r = 0; r < 3; r++)
                 for(int c = 0; c < 3; c++)
-                    blk[r * 3 + c] = zero ? 0.0 : lam0 * gd[3 * a + r] * gd[3 * b + c];
-            int k = find_slot(rowptr, cols, ra, rb);
-            if(k >= 0)
-            {
-                if(!zero)
-                {
-                    add_block(vals + 9 * (size_t)k, blk, false);
-                    if(a != b)
-                        add_block(vals + 9 * (size_t)find_slot(rowptr, cols, rb, ra), blk, true);
-                }
-                continue;
-            }
-            // outside the static pattern: two COO entries (both orientations);
-            // kept even when zeroed so MAS still sees the contact connectivity
-            unsigned o = atomicAdd(coo_count, 2u);
-            if((int)(o + 2) > coo_cap)
-            {
-                atomicExch(overflow, 1);
-                continue;
-            }
-            coo_r[o]     = ra;
-            coo_c[o]     = rb;
-            coo_r[o + 1] = rb;
-            coo_c[o + 1] = ra;
-            for(int r = 0; r < 3; r++)
-                for(int c = 0; c < 3; c++)
-                {
-                    coo_v[9 * (size_t)o + r * 3 + c]       = blk[r * 3 + c];
-                    coo_v[9 * (size_t)(o + 1) + c * 3 + r] = blk[r * 3 + c];
-                }
+                    blk[r * 3 + c] = lam0 * gd[3 * a + r] * gd[3 * b + c];
+            sink.put(ids[a], ids[b], blk, a == b);
         }
 }
 
@@ -476,18 +446,6 @@ __global__ void k_rhs(int nv, const int* __restrict__ btype, const double* __res
 }
 
 // --------------------------------------------------------------- energies
-// block partial sums (256 threads) written in block order
-__device__ __forceinline__ void block_partial(double v, double* out)
-{
-    __shared__ double ws[4];
-    v = wave_sum(v);
-    if(__lane_id() == 0)
-        ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if(threadIdx.x == 0)
-        out[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-}
-
 __global__ void __launch_bounds__(256) k_energy_kin(int nv, const double* __restrict__ x,
                                                     const double* __restrict__ xt,
                                                     const double* __restrict__ mass, double* out)
@@ -575,12 +533,12 @@ __global__ void __launch_bounds__(256) k_energy_ground(const unsigned* __restric
     block_partial(e, out);
 }
 
-// sum nterm partial arrays (fixed order) into terms[0..3]; terms[4] = total
+// sum the partial arrays (fixed order) into terms[0..5]; terms[6] = total
 __global__ void __launch_bounds__(256) k_energy_final(const double* __restrict__ partial,
                                                       EnergyLayout lay, double* __restrict__ terms)
 {
     __shared__ double ws[4];
-    for(int term = 0; term < 4; term++)
+    for(int term = 0; term < kEnergyTerms; term++)
     {
         double v = 0.0;
         for(int i = threadIdx.x; i < lay.n[term]; i += blockDim.x)
@@ -594,7 +552,12 @@ __global__ void __launch_bounds__(256) k_energy_final(const double* __restrict__
         __syncthreads();
     }
     if(threadIdx.x == 0)
-        terms[4] = terms[0] + terms[1] + terms[2] + terms[3];
+    {
+        double t = 0.0;
+        for(int term = 0; term < kEnergyTerms; term++)
+            t += terms[term];
+        terms[kEnergyTerms] = t;
+    }
 }
 
 // ---------------------------------------------------------------- helpers
@@ -659,9 +622,17 @@ void launch_fem(const SimDev& d, bool hessian, hipStream_t s)
     if(d.nt == 0)
         return;
     hipLaunchKernelGGL(k_fem, dim3(grid_for(d.nt, 128)), dim3(128), 0, s, d.nt, d.tets, d.dminv,
-                       d.vol, d.mu, d.lam, d.x, d.dt * d.dt, d.btype, d.tslot, d.vals, d.gshape,
-                       hessian ? 1 : 0);
+                       d.vol, d.mu, d.lam, d.x, d.dt * d.dt, d.fem_buf, hessian ? 1 : 0);
     SG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fem_gather_grad, dim3(grid_for(d.nv, 256)), dim3(256), 0, s, d.nv,
+                       d.gv_ptr, d.gv_src, d.fem_buf, d.gshape);
+    SG_LAUNCH_CHECK();
+    if(hessian && d.n_up > 0)
+    {
+        hipLaunchKernelGGL(k_fem_gather_hess, dim3(grid_for(d.n_up, 256)), dim3(256), 0, s,
+                           d.n_up, d.gh_up, d.gh_lo, d.gh_ptr, d.gh_src, d.fem_buf, d.vals);
+        SG_LAUNCH_CHECK();
+    }
 }
 
 void launch_ground_detect(const SimDev& d, hipStream_t s)
@@ -689,8 +660,7 @@ void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontac
     if(d.pair_cap == 0)
         return;
     hipLaunchKernelGGL(k_barrier, dim3(grid_for(d.pair_cap, 128)), dim3(128), 0, s, d.pcount,
-                       d.pair_cap, d.pairs, d.x, d.dHat, kappa, d.btype, d.rowptr, d.cols, d.vals, d.coo_r,
-                       d.coo_c, d.coo_v, d.coo_count, d.coo_cap, d.overflow, gcontact,
+                       d.pair_cap, d.pairs, d.x, d.dHat, kappa, block_sink(d), gcontact,
                        hessian ? 1 : 0);
     SG_LAUNCH_CHECK();
 }
@@ -709,8 +679,10 @@ EnergyLayout energy_layout(const SimDev& d)
     L.n[1]   = d.nt ? (int)grid_for(d.nt, 256) : 0;
     L.n[2]   = d.pair_cap ? (int)grid_for(d.pair_cap, 256) : 0;
     L.n[3]   = (d.use_ground && d.ns) ? (int)grid_for(d.ns, 256) : 0;
+    L.n[4]   = (d.fric.mu != 0.0 && d.fric.cap) ? (int)grid_for(d.fric.cap, 256) : 0;
+    L.n[5]   = (d.fric.mu_gd != 0.0 && d.use_ground && d.ns) ? (int)grid_for(d.ns, 256) : 0;
     L.off[0] = 0;
-    for(int k = 1; k < 4; k++)
+    for(int k = 1; k < kEnergyTerms; k++)
         L.off[k] = L.off[k - 1] + L.n[k - 1];
     return L;
 }
@@ -730,6 +702,7 @@ void launch_energy(const SimDev& d, double kappa, double* partial, double* terms
     if(L.n[3])
         hipLaunchKernelGGL(k_energy_ground, dim3(L.n[3]), dim3(256), 0, s, d.gcount, d.gp, d.x,
                            d.gn, d.goff, d.dHat, kappa, partial + L.off[3]);
+    launch_energy_friction(d, L, partial, s);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial, L, terms);
     SG_LAUNCH_CHECK();
 }

@@ -290,7 +290,8 @@ __global__ void k_leaf_nodes(int n, const uint64_t* __restrict__ keys, uint32_t*
         bvs[6 * (size_t)(i + n - 1) + c] = leaf[6 * (size_t)el + c];
 }
 
-__global__ void k_internal_nodes(int n, const uint64_t* __restrict__ code, uint32_t* __restrict__ nodes)
+__global__ void k_internal_nodes(int n, const uint64_t* __restrict__ code, uint32_t* __restrict__ nodes,
+                                 int* __restrict__ rng)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if(idx >= n - 1)
@@ -355,6 +356,8 @@ __global__ void k_internal_nodes(int n, const uint64_t* __restrict__ code, uint3
             gamma = split;
         }
     }
+    rng[2 * (size_t)idx]     = first < last ? first : last;
+    rng[2 * (size_t)idx + 1] = first < last ? last : first;
     uint32_t L = (uint32_t)gamma, R = (uint32_t)gamma + 1;
     if(min(first, last) == gamma)
         L += n - 1;
@@ -366,33 +369,110 @@ __global__ void k_internal_nodes(int n, const uint64_t* __restrict__ code, uint3
     nodes[4 * (size_t)R]       = (uint32_t)idx;
 }
 
-// bottom-up refit: the second child to arrive merges (coherent loads)
-__global__ void k_refit(int n, const uint32_t* __restrict__ nodes, double* bvs, unsigned* flags)
+// ------------------------------------------------------------------ refit
+// Internal node i covers the sorted leaves [first_i, last_i] (Karras), so its
+// box is the union of those leaf boxes.  Unions are exact (max/min), so the
+// boxes are bit-identical to a bottom-up refit whatever the order.  Instead
+// of walking up with atomics and device-wide fences, every node is computed
+// independently from leaf boxes and two levels of precomputed aggregates:
+// blocks of 64 leaves and super-blocks of 64 blocks.
+constexpr int kAgg = 64;
+
+struct Box6
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n)
-        return;
-    uint32_t p = nodes[4 * (size_t)(i + n - 1)];
-    while(p != 0xFFFFFFFFu)
+    double u[3], l[3];
+};
+
+__device__ __forceinline__ Box6 box_empty()
+{
+    Box6 b;
+    for(int c = 0; c < 3; c++)
     {
-        __threadfence();
-        if(atomicAdd(flags + p, 1u) == 0u)
-            return;
-        __threadfence();
-        uint32_t        L  = nodes[4 * (size_t)p + 1], R = nodes[4 * (size_t)p + 2];
-        volatile double* vb = bvs;
-        double           o[6];
+        b.u[c] = -INFINITY;
+        b.l[c] = INFINITY;
+    }
+    return b;
+}
+
+__device__ __forceinline__ void box_add(Box6& b, const double* p)  // p: {upper, lower}
+{
+    for(int c = 0; c < 3; c++)
+    {
+        double a = p[c], d = p[3 + c];
+        b.u[c]   = (b.u[c] < a) ? a : b.u[c];
+        b.l[c]   = (d < b.l[c]) ? d : b.l[c];
+    }
+}
+
+// one wave per group of 64 input boxes: out[g] = union of in[64 g .. 64 g + 63]
+__global__ void __launch_bounds__(256) k_box_groups(int n, const double* __restrict__ in,
+                                                    double* __restrict__ out)
+{
+    int    i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6];
+    if(i < n)
+        for(int c = 0; c < 6; c++)
+            v[c] = in[6 * (size_t)i + c];
+    else
         for(int c = 0; c < 3; c++)
         {
-            double a = vb[6 * (size_t)L + c], b = vb[6 * (size_t)R + c];
-            o[c]     = (a < b) ? b : a;
-            a        = vb[6 * (size_t)L + 3 + c];
-            b        = vb[6 * (size_t)R + 3 + c];
-            o[3 + c] = (b < a) ? b : a;
+            v[c]     = -INFINITY;
+            v[3 + c] = INFINITY;
         }
-        for(int c = 0; c < 6; c++)
-            vb[6 * (size_t)p + c] = o[c];
-        p = nodes[4 * (size_t)p];
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for(int c = 0; c < 3; c++)
+        {
+            double a = __shfl_xor(v[c], o, 64), d = __shfl_xor(v[3 + c], o, 64);
+            v[c]     = (v[c] < a) ? a : v[c];
+            v[3 + c] = (d < v[3 + c]) ? d : v[3 + c];
+        }
+    int g = i >> 6;
+    if((i & 63) < 6 && g < (n + 63) / 64)
+        out[6 * (size_t)g + (i & 63)] = v[i & 63];
+}
+
+// union of in[a .. b] (inclusive)
+__device__ __forceinline__ void box_scan(Box6& acc, const double* __restrict__ in, int a, int b)
+{
+    for(int j = a; j <= b; j++)
+        box_add(acc, in + 6 * (size_t)j);
+}
+
+__global__ void __launch_bounds__(256) k_refit_query(int n, const int* __restrict__ rng,
+                                                     const double* __restrict__ blk,
+                                                     const double* __restrict__ sup,
+                                                     double* __restrict__ bvs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n - 1)
+        return;
+    const double* leaf = bvs + 6 * (size_t)(n - 1);
+    int           f = rng[2 * (size_t)i], l = rng[2 * (size_t)i + 1];
+    Box6          acc = box_empty();
+    if(l - f < 2 * kAgg)
+        box_scan(acc, leaf, f, l);
+    else
+    {
+        int fb = (f + kAgg - 1) / kAgg, lb = (l + 1) / kAgg - 1;  // full blocks fb..lb
+        box_scan(acc, leaf, f, fb * kAgg - 1);
+        box_scan(acc, leaf, (lb + 1) * kAgg, l);
+        if(lb - fb < 2 * kAgg)
+            box_scan(acc, blk, fb, lb);
+        else
+        {
+            int fs = (fb + kAgg - 1) / kAgg, ls = (lb + 1) / kAgg - 1;  // full super-blocks
+            box_scan(acc, blk, fb, fs * kAgg - 1);
+            box_scan(acc, blk, (ls + 1) * kAgg, lb);
+            box_scan(acc, sup, fs, ls);
+        }
+    }
+    double* o = bvs + 6 * (size_t)i;
+    for(int c = 0; c < 3; c++)
+    {
+        o[c]     = acc.u[c];
+        o[3 + c] = acc.l[c];
     }
 }
 
@@ -409,6 +489,10 @@ void Lbvh::init(int n_, int arity_, const uint32_t* prims_)
     keys.resize(n);
     keys_alt.resize(n);
     flags.resize(n);
+    rng.resize(2 * (size_t)(n > 1 ? n - 1 : 1));
+    int nblk_agg = (n + kAgg - 1) / kAgg;
+    agg_blk.resize(6 * (size_t)nblk_agg);
+    agg_sup.resize(6 * (size_t)((nblk_agg + kAgg - 1) / kAgg));
     int nblk = (n + kSortTile - 1) / kSortTile;
     hist.resize((size_t)256 * nblk);
     scene.resize(6);
@@ -434,9 +518,11 @@ void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
     int nblk = (n + kSortTile - 1) / kSortTile;
     uint64_t* a = keys.data();
     uint64_t* b = keys_alt.data();
+    // the keys enter in index order and every pass is stable, so sorting the
+    // 30 Morton bits alone orders by (morton, index): the index bits need no
+    // passes
+    (void)ib;
     std::vector<int> shifts;
-    for(int sh = 0; sh < ib; sh += 8)
-        shifts.push_back(sh);
     for(int sh = 32; sh < 62; sh += 8)
         shifts.push_back(sh);
     for(int sh : shifts)
@@ -456,9 +542,14 @@ void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
     if(n > 1)
     {
         hipLaunchKernelGGL(k_internal_nodes, dim3(grid_for(n - 1, 256)), dim3(256), 0, s, n,
-                           keys.data(), nodes.data());
-        hipLaunchKernelGGL(k_refit, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nodes.data(),
-                           bvs.data(), flags.data());
+                           keys.data(), nodes.data(), rng.data());
+        int nblk_agg = (n + kAgg - 1) / kAgg;
+        hipLaunchKernelGGL(k_box_groups, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                           bvs.data() + 6 * (size_t)(n - 1), agg_blk.data());
+        hipLaunchKernelGGL(k_box_groups, dim3(grid_for(nblk_agg, 256)), dim3(256), 0, s,
+                           nblk_agg, agg_blk.data(), agg_sup.data());
+        hipLaunchKernelGGL(k_refit_query, dim3(grid_for(n - 1, 256)), dim3(256), 0, s, n,
+                           rng.data(), agg_blk.data(), agg_sup.data(), bvs.data());
     }
     SG_LAUNCH_CHECK();
 }

@@ -29,6 +29,7 @@ struct Bsr
     DBuf<int>    rows;      // nnzb, block row of each entry
     DBuf<int>    diag;      // nb, index of block (i, i)
     DBuf<double> vals;      // 9 * nnzb, row-major blocks
+    DBuf<float>  vals32;    // FP32 copy for the mixed-precision inner solve
     std::vector<int> h_rowptr, h_cols;  // host copy of the pattern
 
     // build the pattern from host CSR (cols sorted, every row has its diagonal)
@@ -46,8 +47,10 @@ struct Coo
     int            nb = 0;
     DBuf<int>      rp;       // nb + 1
     DBuf<int>      fill;     // nb + 1 (histogram / fill cursor)
+    DBuf<int>      tiles;    // scan tile offsets
     DBuf<int>      rc;       // cap
     DBuf<double>   rv;       // 9 * cap
+    DBuf<float>    rv32;     // FP32 copy (mixed precision)
     bool           rows_valid = false;
     void reserve(int n, int nrows)
     {
@@ -59,6 +62,7 @@ struct Coo
         count.resize(1);
         rp.resize(nrows + 1);
         fill.resize(nrows + 1);
+        tiles.resize((nrows + 1023) / 1024 + 2);
         rc.resize(n);
         rv.resize(9 * (size_t)n);
     }
@@ -85,6 +89,8 @@ void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t
 void residual(const SysView& S, const double* b, const double* x, double* r, hipStream_t s);
 // dot(a, b) accumulated into the slot set `out` (must be zeroed)
 void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t s);
+// refresh the FP32 copies of the matrix (static pattern and contact rows)
+void refresh_fp32(const SysView& S, hipStream_t s);
 
 // ------------------------------------------------------------------ MAS
 struct MasConfig
@@ -112,6 +118,10 @@ class Mas
     void apply_fused(double* x, double* r, const double* p, const double* Ap,
                      const double* rz_prev, const double* pAp, double* z, double* rz_out,
                      double* zero0, double* zero1, const int* done, hipStream_t s);
+    // the same on FP32 PCG vectors (inner solve of the mixed-precision mode)
+    void apply_fused(float* x, float* r, const float* p, const float* Ap,
+                     const double* rz_prev, const double* pAp, float* z, double* rz_out,
+                     double* zero0, double* zero1, const int* done, hipStream_t s);
 
     int levels() const { return L_; }
     int total_clusters() const { return total_clusters_; }
@@ -122,6 +132,10 @@ class Mas
     Profiler* prof = nullptr;
 
   private:
+    template <typename V>
+    void apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
+                       const double* pap, V* z, double* rz_out, double* zero0, double* zero1,
+                       const int* done, hipStream_t s);
     void cluster_level(int level, hipStream_t s);
     void assemble(const SysView& S, hipStream_t s);
     void invert(hipStream_t s);
@@ -159,14 +173,35 @@ struct PcgConfig
     int    max_iter    = 0;
     bool   use_rz      = true;
     int    check_every = 10;
+    // mixed precision: FP32 PCG whose periodic residual recomputation is done
+    // in FP64 (with use_rz = false, rel_tol bounds the FP64 true residual)
+    bool   mixed = false;
 };
 
 struct PcgStats
 {
-    int    iterations = 0;
-    bool   converged  = false;
+    int    iterations       = 0;  // PCG iterations (FP32 + FP64 fallback in mixed mode)
+    int    outer_iterations = 0;  // 1, or 2 when the mixed mode fell back to FP64
+    int    breakdown        = 0;  // 1 non-finite r.z, 2 diverged (mixed mode)
+    bool   converged        = false;
     double rz0 = 0, rz = 0;
     double rel_residual = 0;
+};
+
+// PCG work vectors of one precision
+template <typename V>
+struct PcgVecs
+{
+    DBuf<V> r, z, p[2], Ap, tmp, res, rhs, sol;
+    int     n = 0;
+    void    ensure(int n_)
+    {
+        if(n_ == n)
+            return;
+        for(DBuf<V>* b : {&r, &z, &p[0], &p[1], &Ap, &tmp, &res, &rhs, &sol})
+            b->resize(n_);
+        n = n_;
+    }
 };
 
 enum class Precond
@@ -191,6 +226,17 @@ class Pcg
     Profiler* prof = nullptr;
 
   private:
+    // b64/x64 non-null: FP64 residual replacement at every check (mixed mode)
+    template <typename V>
+    PcgStats solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
+                     const PcgConfig& cfg, hipStream_t s, const double* b64, double* x64);
+    PcgStats solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
+                         const PcgConfig& cfg, hipStream_t s);
+    double   true_residual(const SysView& S, const double* b, const double* x, hipStream_t s);
+    template <typename V>
+    PcgVecs<V>& vecs();
+    template <typename V>
+    const V* dinv() const;
     void ensure(int n);
     // ring of kRing (pAp, rz) slot-set pairs indexed by iteration, plus aux
     // sets for r.r, |b|^2 and the final residual
@@ -198,7 +244,10 @@ class Pcg
     double* pap_set(int k) { return red_.data() + (size_t)(2 * (k % kRing)) * kSlotSet; }
     double* rz_set(int k) { return red_.data() + (size_t)(2 * (k % kRing) + 1) * kSlotSet; }
     double* aux_set(int i) { return red_.data() + (size_t)(2 * kRing + i) * kSlotSet; }
-    DBuf<double> r_, z_, p_[2], Ap_, red_, dinv_, tmp_;
+    PcgVecs<double> v64_;
+    PcgVecs<float>  v32_;
+    DBuf<double> red_, dinv_;
+    DBuf<float>  dinv32_;
     DBuf<int>    ctl_;         // [0] stopping check iteration (0: running), [1] breakdown
     DBuf<double> ctl_d_;       // [0] r.z at the last stop test
     int*         h_ctl_ = nullptr;  // pinned: two slots of 4 ints for the stop flag

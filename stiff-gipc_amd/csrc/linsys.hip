@@ -59,36 +59,84 @@ __global__ void k_coo_hist(const unsigned* __restrict__ count, unsigned cap,
         atomicAdd(hist + r[t], 1);
 }
 
-// exclusive scan of a[0..n) into out[0..n], out[n] = total (one workgroup of
-// 1024 threads, each owning a contiguous chunk)
-__global__ void __launch_bounds__(1024) k_scan_one_block(int n, const int* __restrict__ a,
-                                                         int* __restrict__ out)
+// Exclusive scan of a[0..n) into out[0..n], out[n] = total, in three
+// launches: per-tile sums (1024 elements per workgroup), one workgroup scans
+// the tile sums, every tile scans itself and adds its offset.
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int block_incl_scan(int v, int* ws)  // 1024 threads
 {
-    __shared__ int part[1024];
-    const int t     = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int b0    = t * chunk;
-    const int b1    = (b0 + chunk < n) ? b0 + chunk : n;
-    int       sum   = 0;
-    for(int i = b0; i < b1; i++)
-        sum += a[i];
-    part[t] = sum;
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    for(int o = 1; o < 64; o <<= 1)
+    {
+        int u = __shfl_up(v, o, 64);
+        if(lane >= o)
+            v += u;
+    }
+    if(lane == 63)
+        ws[w] = v;
     __syncthreads();
-    for(int o = 1; o < 1024; o <<= 1)
+    if(threadIdx.x < 16)
     {
-        int v = (t >= o) ? part[t - o] : 0;
+        int x = ws[threadIdx.x];
+        for(int o = 1; o < 16; o <<= 1)
+        {
+            int u = __shfl_up(x, o, 16);
+            if((int)threadIdx.x >= o)
+                x += u;
+        }
+        ws[threadIdx.x] = x;
+    }
+    __syncthreads();
+    return v + (w > 0 ? ws[w - 1] : 0);
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_tiles(int n, const int* __restrict__ a,
+                                                          int* __restrict__ tsum)
+{
+    __shared__ int ws[16];
+    int i = blockIdx.x * kScanTile + threadIdx.x;
+    int v = block_incl_scan(i < n ? a[i] : 0, ws);
+    if(threadIdx.x == kScanTile - 1)
+        tsum[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_tile_sums(int nt, int* __restrict__ tsum)
+{
+    __shared__ int ws[16];
+    __shared__ int carry;
+    if(threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for(int base = 0; base < nt; base += kScanTile)
+    {
+        int i = base + threadIdx.x;
+        int v = i < nt ? tsum[i] : 0;
+        int s = block_incl_scan(v, ws);
+        int c = carry;
+        if(i < nt)
+            tsum[i] = c + s - v;  // exclusive offset of tile i
         __syncthreads();
-        part[t] += v;
+        if(threadIdx.x == kScanTile - 1)
+            carry = c + s;
         __syncthreads();
     }
-    int run = part[t] - sum;  // exclusive prefix of this chunk
-    for(int i = b0; i < b1; i++)
-    {
-        out[i] = run;
-        run += a[i];
-    }
-    if(t == 1023)
-        out[n] = part[1023];
+    if(threadIdx.x == 0)
+        tsum[nt] = carry;
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_apply(int n, const int* __restrict__ a,
+                                                          const int* __restrict__ tsum,
+                                                          int* __restrict__ out, int nt)
+{
+    __shared__ int ws[16];
+    int i = blockIdx.x * kScanTile + threadIdx.x;
+    int v = i < n ? a[i] : 0;
+    int s = block_incl_scan(v, ws);
+    if(i < n)
+        out[i] = tsum[blockIdx.x] + s - v;
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+        out[n] = tsum[nt];
 }
 
 __global__ void k_coo_scatter(const unsigned* __restrict__ count, unsigned cap,
@@ -116,7 +164,12 @@ void Coo::build_rows(hipStream_t s)
     hipLaunchKernelGGL(k_coo_hist, dim3(grid_for(cap, 256)), dim3(256), 0, s, count.data(),
                        (unsigned)cap, r.data(), fill.data());
     SG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, s, nb, fill.data(), rp.data());
+    int nt = (nb + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanTile), 0, s, nb, fill.data(),
+                       tiles.data());
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanTile), 0, s, nt, tiles.data());
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kScanTile), 0, s, nb, fill.data(),
+                       tiles.data(), rp.data(), nt);
     SG_LAUNCH_CHECK();
     SG_HIP(hipMemsetAsync(fill.data(), 0, (nb + 1) * sizeof(int), s));
     hipLaunchKernelGGL(k_coo_scatter, dim3(grid_for(cap, 256)), dim3(256), 0, s, count.data(),
@@ -128,9 +181,10 @@ void Coo::build_rows(hipStream_t s)
 
 // ------------------------------------------------------------------ SpMV
 // 8 lanes per block row; a row's blocks are contiguous, so the 8 lanes of a
-// group stream 576 contiguous bytes per step.  The static pattern and the
-// row-sorted contact blocks of the row are traversed by the same lanes, so y
-// is written once per row without atomics.
+// group stream 576 contiguous bytes per step (288 in FP32).  The static
+// pattern and the row-sorted contact blocks of the row are traversed by the
+// same lanes, so y is written once per row without atomics.  V is the
+// precision of the matrix, the vectors and the row sums.
 //
 // MODE_PLAIN: y = A x            (+ optional x.y into a slot set)
 // MODE_RESID: y = b - A x
@@ -160,17 +214,18 @@ struct PcgCtl
     const double* q_set    = nullptr;
     const double* rz_check = nullptr;
     double        thr      = 0.0;
+    double        div_thr  = 0.0;  // > 0: q above it stops the solve as diverged (ctl[1] = 2)
     int           check_k  = 0;
 };
 
-template <int MODE>
+template <int MODE, typename V>
 __global__ void __launch_bounds__(256)
     k_spmv(int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
-           const double* __restrict__ vals, const int* __restrict__ rp2,
-           const int* __restrict__ cols2, const double* __restrict__ vals2,
-           const double* __restrict__ x, const double* __restrict__ pprev,
-           double* __restrict__ pcur, const double* __restrict__ b, double* __restrict__ y,
-           const double* rz_a, const double* rz_b, double* dot, PcgCtl pc)
+           const V* __restrict__ vals, const int* __restrict__ rp2,
+           const int* __restrict__ cols2, const V* __restrict__ vals2,
+           const V* __restrict__ x, const V* __restrict__ pprev, V* __restrict__ pcur,
+           const V* __restrict__ b, V* __restrict__ y, const double* rz_a, const double* rz_b,
+           double* dot, PcgCtl pc)
 {
     if(pc.ctl && pc.ctl[0])
         return;
@@ -178,30 +233,31 @@ __global__ void __launch_bounds__(256)
     {
         double q   = slot_set_sum(pc.q_set);
         double rzc = slot_set_sum(pc.rz_check);
-        bool   bad = !(rzc == rzc);
+        bool   bad = !(rzc == rzc) || !(q == q);
+        bool   div = pc.div_thr > 0.0 && q > pc.div_thr;
         if(blockIdx.x == 0 && threadIdx.x == 0)
             pc.ctl_d[0] = rzc;
-        if(q <= pc.thr || bad)
+        if(q <= pc.thr || bad || div)
         {
             if(blockIdx.x == 0 && threadIdx.x == 0)
             {
-                pc.ctl[1] = bad ? 1 : 0;
+                pc.ctl[1] = bad ? 1 : (div ? 2 : 0);
                 pc.ctl[0] = pc.check_k;
             }
             return;
         }
     }
-    double beta = 0.0;
+    V beta = 0;
     if(MODE == MODE_PCG && pprev)
-        beta = slot_set_sum(rz_a) / slot_set_sum(rz_b);
-    auto fetch = [&](int j, double& x0, double& x1, double& x2) {
-        const double* xj = x + 3 * (size_t)j;
+        beta = (V)(slot_set_sum(rz_a) / slot_set_sum(rz_b));
+    auto fetch = [&](int j, V& x0, V& x1, V& x2) {
+        const V* xj = x + 3 * (size_t)j;
         if(MODE == MODE_PCG && pprev)
         {
-            const double* pj = pprev + 3 * (size_t)j;
-            x0               = fma(beta, pj[0], xj[0]);
-            x1               = fma(beta, pj[1], xj[1]);
-            x2               = fma(beta, pj[2], xj[2]);
+            const V* pj = pprev + 3 * (size_t)j;
+            x0          = fma(beta, pj[0], xj[0]);
+            x1          = fma(beta, pj[1], xj[1]);
+            x2          = fma(beta, pj[2], xj[2]);
         }
         else
         {
@@ -214,14 +270,14 @@ __global__ void __launch_bounds__(256)
     int  row  = gid / kRowLanes;
     int  g    = gid % kRowLanes;
     bool live = row < nb;
-    double a0 = 0, a1 = 0, a2 = 0;
+    V    a0 = 0, a1 = 0, a2 = 0;
     if(live)
     {
         int k1 = rowptr[row + 1];
         for(int k = rowptr[row] + g; k < k1; k += kRowLanes)
         {
-            const double* B = vals + 9 * (size_t)k;
-            double        x0, x1, x2;
+            const V* B = vals + 9 * (size_t)k;
+            V        x0, x1, x2;
             fetch(cols[k], x0, x1, x2);
             a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
             a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
@@ -232,8 +288,8 @@ __global__ void __launch_bounds__(256)
             int q1 = rp2[row + 1];
             for(int k = rp2[row] + g; k < q1; k += kRowLanes)
             {
-                const double* B = vals2 + 9 * (size_t)k;
-                double        x0, x1, x2;
+                const V* B = vals2 + 9 * (size_t)k;
+                V        x0, x1, x2;
                 fetch(cols2[k], x0, x1, x2);
                 a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
                 a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
@@ -265,7 +321,7 @@ __global__ void __launch_bounds__(256)
             y[i + 2] = a2;
             if(MODE == MODE_PCG || dot)
             {
-                double x0, x1, x2;
+                V x0, x1, x2;
                 fetch(row, x0, x1, x2);
                 if(MODE == MODE_PCG)
                 {
@@ -273,7 +329,7 @@ __global__ void __launch_bounds__(256)
                     pcur[i + 1] = x1;
                     pcur[i + 2] = x2;
                 }
-                d = x0 * a0 + x1 * a1 + x2 * a2;
+                d = (double)(x0 * a0 + x1 * a1 + x2 * a2);
             }
         }
     }
@@ -281,8 +337,24 @@ __global__ void __launch_bounds__(256)
         slot_set_add(dot, d);
 }
 
-static inline void contact_rows(const SysView& S, const int*& rp2, const int*& c2,
-                                const double*& v2)
+// matrix values of the requested precision (static pattern, contact rows)
+template <typename V>
+struct MatVals;
+template <>
+struct MatVals<double>
+{
+    static const double* a(const SysView& S) { return S.A->vals.data(); }
+    static const double* c(const Coo* c) { return c->rv.data(); }
+};
+template <>
+struct MatVals<float>
+{
+    static const float* a(const SysView& S) { return S.A->vals32.data(); }
+    static const float* c(const Coo* c) { return c->rv32.data(); }
+};
+
+template <typename V>
+static inline void contact_rows(const SysView& S, const int*& rp2, const int*& c2, const V*& v2)
 {
     rp2 = nullptr;
     c2  = nullptr;
@@ -292,63 +364,41 @@ static inline void contact_rows(const SysView& S, const int*& rp2, const int*& c
         SG_CHECK(S.coo->rows_valid, -5, "contact CSR is stale: build_rows() after assembly");
         rp2 = S.coo->rp.data();
         c2  = S.coo->rc.data();
-        v2  = S.coo->rv.data();
+        v2  = MatVals<V>::c(S.coo);
     }
+}
+
+template <int MODE, typename V>
+static void launch_spmv(const SysView& S, const V* x, const V* pprev, V* pcur, const V* b, V* y,
+                        const double* rz_a, const double* rz_b, double* dot, const PcgCtl& pc,
+                        hipStream_t s)
+{
+    const Bsr& A = *S.A;
+    const int* rp2;
+    const int* c2;
+    const V*   v2;
+    contact_rows<V>(S, rp2, c2, v2);
+    hipLaunchKernelGGL((k_spmv<MODE, V>), dim3(grid_for((long long)A.nb * kRowLanes, 256)),
+                       dim3(256), 0, s, A.nb, A.rowptr.data(), A.cols.data(),
+                       MatVals<V>::a(S), rp2, c2, v2, x, pprev, pcur, b, y, rz_a, rz_b, dot, pc);
+    SG_LAUNCH_CHECK();
 }
 
 void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t s)
 {
-    const Bsr&    A = *S.A;
-    const int*    rp2;
-    const int*    c2;
-    const double* v2;
-    contact_rows(S, rp2, c2, v2);
-    hipLaunchKernelGGL(k_spmv<MODE_PLAIN>, dim3(grid_for((long long)A.nb * kRowLanes, 256)),
-                       dim3(256), 0, s, A.nb, A.rowptr.data(), A.cols.data(), A.vals.data(), rp2,
-                       c2, v2, x, (const double*)nullptr, (double*)nullptr,
-                       (const double*)nullptr, y, (const double*)nullptr,
-                       (const double*)nullptr, dot, PcgCtl{});
-    SG_LAUNCH_CHECK();
-}
-
-static void residual_ctl(const SysView& S, const double* b, const double* x, double* r,
-                        const PcgCtl& pc, hipStream_t s)
-{
-    const Bsr&    A = *S.A;
-    const int*    rp2;
-    const int*    c2;
-    const double* v2;
-    contact_rows(S, rp2, c2, v2);
-    hipLaunchKernelGGL(k_spmv<MODE_RESID>, dim3(grid_for((long long)A.nb * kRowLanes, 256)),
-                       dim3(256), 0, s, A.nb, A.rowptr.data(), A.cols.data(), A.vals.data(), rp2,
-                       c2, v2, x, (const double*)nullptr, (double*)nullptr, b, r,
-                       (const double*)nullptr, (const double*)nullptr, (double*)nullptr, pc);
-    SG_LAUNCH_CHECK();
+    launch_spmv<MODE_PLAIN, double>(S, x, nullptr, nullptr, nullptr, y, nullptr, nullptr, dot,
+                                    PcgCtl{}, s);
 }
 
 void residual(const SysView& S, const double* b, const double* x, double* r, hipStream_t s)
 {
-    residual_ctl(S, b, x, r, PcgCtl{}, s);
+    launch_spmv<MODE_RESID, double>(S, x, nullptr, nullptr, b, r, nullptr, nullptr, nullptr,
+                                    PcgCtl{}, s);
 }
 
-// PCG: p = z (+ beta p_prev), Ap = A p, p.Ap -> slot set
-static void spmv_pcg(const SysView& S, const double* z, const double* pprev, double* pcur,
-                     double* Ap, const double* rz_a, const double* rz_b, double* pap,
-                     const PcgCtl& pc, hipStream_t s)
-{
-    const Bsr&    A = *S.A;
-    const int*    rp2;
-    const int*    c2;
-    const double* v2;
-    contact_rows(S, rp2, c2, v2);
-    hipLaunchKernelGGL(k_spmv<MODE_PCG>, dim3(grid_for((long long)A.nb * kRowLanes, 256)),
-                       dim3(256), 0, s, A.nb, A.rowptr.data(), A.cols.data(), A.vals.data(), rp2,
-                       c2, v2, z, pprev, pcur, (const double*)nullptr, Ap, rz_a, rz_b, pap, pc);
-    SG_LAUNCH_CHECK();
-}
-
-__global__ void __launch_bounds__(256) k_dot_slots(int n, const double* __restrict__ a,
-                                                   const double* __restrict__ b, double* out,
+template <typename V>
+__global__ void __launch_bounds__(256) k_dot_slots(int n, const V* __restrict__ a,
+                                                   const V* __restrict__ b, double* out,
                                                    const int* __restrict__ done)
 {
     if(done && *done)
@@ -356,22 +406,23 @@ __global__ void __launch_bounds__(256) k_dot_slots(int n, const double* __restri
     int    i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
     for(; i < n; i += gridDim.x * blockDim.x)
-        d += a[i] * b[i];
+        d += (double)a[i] * (double)b[i];
     slot_set_add(out, d);
 }
 
-static void dot_slots_ctl(int n, const double* a, const double* b, double* out,
-                          const int* done, hipStream_t s)
+template <typename V>
+static void dot_slots_ctl(int n, const V* a, const V* b, double* out, const int* done,
+                          hipStream_t s)
 {
     unsigned g = grid_for(n, 256);
     g          = g > 1024 ? 1024 : g;
-    hipLaunchKernelGGL(k_dot_slots, dim3(g), dim3(256), 0, s, n, a, b, out, done);
+    hipLaunchKernelGGL(k_dot_slots<V>, dim3(g), dim3(256), 0, s, n, a, b, out, done);
     SG_LAUNCH_CHECK();
 }
 
 void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t s)
 {
-    dot_slots_ctl(n, a, b, out, nullptr, s);
+    dot_slots_ctl<double>(n, a, b, out, nullptr, s);
 }
 
 // clear a slot set in stream order (a kernel, so the sequence stays free of
@@ -410,6 +461,38 @@ double read_slot_set(const double* d, hipStream_t s)
     return slot_set_sum_host(h);
 }
 
+// ------------------------------------------------------------ conversions
+template <typename A, typename B>
+__global__ void k_convert(long long n, const A* __restrict__ a, B* __restrict__ b)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if(i < n)
+        b[i] = (B)a[i];
+}
+
+template <typename A, typename B>
+static void convert(long long n, const A* a, B* b, hipStream_t s)
+{
+    if(n <= 0)
+        return;
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, b);
+    SG_LAUNCH_CHECK();
+}
+
+void refresh_fp32(const SysView& S, hipStream_t s)
+{
+    Bsr& A = *const_cast<Bsr*>(S.A);
+    A.vals32.resize((size_t)9 * A.nnzb);
+    convert<double, float>(9ll * A.nnzb, A.vals.data(), A.vals32.data(), s);
+    if(S.coo && S.coo->cap > 0)
+    {
+        Coo& c = *const_cast<Coo*>(S.coo);
+        c.rv32.resize(9 * (size_t)c.cap);
+        // the live entries are a prefix of the row-sorted copy: convert all
+        convert<double, float>(9ll * c.cap, c.rv.data(), c.rv32.data(), s);
+    }
+}
+
 // ------------------------------------------------------------ block Jacobi
 __global__ void k_diag_inverse(int nb, const int* __restrict__ diag,
                                const double* __restrict__ vals, double* __restrict__ dinv)
@@ -442,37 +525,38 @@ __global__ void k_diag_inverse(int nb, const int* __restrict__ diag,
 
 // optional update x += a p, r -= a Ap (a = rz[k-1] / pAp[k] from slot sets),
 // then z = Dinv r and r.z into a slot set; block 0 clears two later sets
+template <typename V>
 __global__ void __launch_bounds__(256)
-    k_update_diag(int nb, double* x, double* r, const double* p, const double* Ap,
-                  const double* rz_prev, const double* pap, const double* __restrict__ dinv,
-                  double* z, double* rz_out, double* zero0, double* zero1,
-                  const int* __restrict__ done)
+    k_update_diag(int nb, V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
+                  const double* pap, const V* __restrict__ dinv, V* z, double* rz_out,
+                  double* zero0, double* zero1, const int* __restrict__ done)
 {
     if(done && *done)
         return;
     double a = 0.0;
     if(x)
         a = slot_set_sum(rz_prev) / slot_set_sum(pap);
-    int    i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.0;
+    V      av = (V)a;
+    int    i  = blockIdx.x * blockDim.x + threadIdx.x;
+    double d  = 0.0;
     if(i < nb)
     {
-        double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+        V ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
         if(x)
         {
             for(int c = 0; c < 3; c++)
             {
-                x[3 * i + c] += a * p[3 * i + c];
-                ri[c] -= a * Ap[3 * i + c];
+                x[3 * i + c] += av * p[3 * i + c];
+                ri[c] -= av * Ap[3 * i + c];
                 r[3 * i + c] = ri[c];
             }
         }
-        const double* D = dinv + 9 * (size_t)i;
+        const V* D = dinv + 9 * (size_t)i;
         for(int c = 0; c < 3; c++)
         {
-            double zc    = D[3 * c] * ri[0] + D[3 * c + 1] * ri[1] + D[3 * c + 2] * ri[2];
+            V zc         = D[3 * c] * ri[0] + D[3 * c + 1] * ri[1] + D[3 * c + 2] * ri[2];
             z[3 * i + c] = zc;
-            d += ri[c] * zc;
+            d += (double)(ri[c] * zc);
         }
     }
     if(rz_out)
@@ -486,19 +570,46 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void __launch_bounds__(256) k_axpy(int n, double* x, double* r, const double* p,
-                                              const double* Ap, const double* rz_prev,
-                                              const double* pap, const int* __restrict__ done)
+template <typename V>
+__global__ void __launch_bounds__(256) k_axpy(int n, V* x, V* r, const V* p, const V* Ap,
+                                              const double* rz_prev, const double* pap,
+                                              const int* __restrict__ done)
 {
     if(done && *done)
         return;
-    double a = slot_set_sum(rz_prev) / slot_set_sum(pap);
-    int    i = blockIdx.x * blockDim.x + threadIdx.x;
+    V   a = (V)(slot_set_sum(rz_prev) / slot_set_sum(pap));
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i < n)
     {
         x[i] += a * p[i];
         r[i] -= a * Ap[i];
     }
+}
+
+// reliable updates of the mixed mode: x64 += x, x = 0  /  r = (V) r64
+template <typename V>
+__global__ void k_fold(int n, double* __restrict__ x64, V* __restrict__ x,
+                       const int* __restrict__ done)
+{
+    if(done && *done)
+        return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+    {
+        x64[i] += (double)x[i];
+        x[i] = (V)0;
+    }
+}
+
+template <typename V>
+__global__ void k_round(int n, const double* __restrict__ r64, V* __restrict__ r,
+                        const int* __restrict__ done)
+{
+    if(done && *done)
+        return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+        r[i] = (V)r64[i];
 }
 
 void Pcg::build_diag(const Bsr& A, hipStream_t s)
@@ -511,14 +622,13 @@ void Pcg::build_diag(const Bsr& A, hipStream_t s)
 
 void Pcg::apply_diag(int nb, const double* r, double* z, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_diag, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
+    hipLaunchKernelGGL(k_update_diag<double>, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
                        (double*)nullptr, const_cast<double*>(r), (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        dinv_.data(), z, (double*)nullptr, (double*)nullptr, (double*)nullptr,
                        (const int*)nullptr);
     SG_LAUNCH_CHECK();
 }
-
 
 Pcg::~Pcg()
 {
@@ -529,18 +639,29 @@ Pcg::~Pcg()
             (void)hipEventDestroy(e);
 }
 
+template <>
+PcgVecs<double>& Pcg::vecs<double>()
+{
+    return v64_;
+}
+template <>
+PcgVecs<float>& Pcg::vecs<float>()
+{
+    return v32_;
+}
+template <>
+const double* Pcg::dinv<double>() const
+{
+    return dinv_.data();
+}
+template <>
+const float* Pcg::dinv<float>() const
+{
+    return dinv32_.data();
+}
+
 void Pcg::ensure(int n)
 {
-    if(n != n_)
-    {
-        r_.resize(n);
-        z_.resize(n);
-        p_[0].resize(n);
-        p_[1].resize(n);
-        Ap_.resize(n);
-        tmp_.resize(n);
-        n_ = n;
-    }
     red_.resize((size_t)(2 * kRing + 4) * kSlotSet);
     ctl_.resize(4);
     ctl_d_.resize(4);
@@ -550,6 +671,7 @@ void Pcg::ensure(int n)
         for(auto& e : ev_)
             SG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    (void)n;
 }
 
 // Iteration k (1-based) as launched here:
@@ -566,46 +688,63 @@ void Pcg::ensure(int n)
 // after each test and reads it one check period later.  Slot sets for
 // (pAp, rz) of iteration k live in ring entry k % kRing; the collect of
 // iteration k clears entry (k + 2) % kRing, whose last reader was the SpMV
-// of iteration k.
-PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
-                    const PcgConfig& cfg, hipStream_t s)
+// of iteration k.  V: precision of the matrix, vectors and preconditioner
+// application (double, or float for the inner solve of the mixed mode).
+template <typename V>
+PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
+                      const PcgConfig& cfg, hipStream_t s, const double* b64, double* x64)
 {
-    const int nb    = S.A->nb;
-    const int n     = 3 * nb;
-    int       maxit = cfg.max_iter > 0 ? cfg.max_iter : n;
-    int       every = cfg.check_every > 0 ? cfg.check_every : 10;
+    const int   nb    = S.A->nb;
+    const int   n     = 3 * nb;
+    int         maxit = cfg.max_iter > 0 ? cfg.max_iter : n;
+    int         every = cfg.check_every > 0 ? cfg.check_every : 10;
+    PcgVecs<V>& W     = vecs<V>();
     ensure(n);
+    W.ensure(n);
     SG_HIP(hipMemsetAsync(red_.data(), 0, red_.size() * sizeof(double), s));
     SG_HIP(hipMemsetAsync(ctl_.data(), 0, ctl_.size() * sizeof(int), s));
     SG_HIP(hipMemsetAsync(ctl_d_.data(), 0, ctl_d_.size() * sizeof(double), s));
-    SG_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
-    SG_HIP(hipMemcpyAsync(r_.data(), b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SG_HIP(hipMemsetAsync(x, 0, n * sizeof(V), s));
+    SG_HIP(hipMemcpyAsync(W.r.data(), b, n * sizeof(V), hipMemcpyDeviceToDevice, s));
     if(pc == Precond::Diag)
+    {
         build_diag(*S.A, s);
+        if(sizeof(V) == sizeof(float))
+        {
+            dinv32_.resize(dinv_.size());
+            convert<double, float>((long long)dinv_.size(), dinv_.data(), dinv32_.data(), s);
+        }
+    }
     const int* done = ctl_.data();
 
     // z = M r with the optional update; rz[k] -> rz_set(k)
-    auto precond = [&](bool axpy, int k, const double* p, const int* dn) {
+    auto precond = [&](bool axpy, int k, const V* p, const int* dn) {
         double* rz_prev = axpy ? rz_set(k - 1) : nullptr;
         double* pap     = axpy ? pap_set(k) : nullptr;
         double* zero0   = pap_set(k + 2);
         double* zero1   = rz_set(k + 2);
         if(pc == Precond::Mas)
-            mas->apply_fused(axpy ? x : nullptr, r_.data(), p, Ap_.data(), rz_prev, pap,
-                             z_.data(), rz_set(k), zero0, zero1, dn, s);
+            mas->apply_fused(axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(), rz_prev, pap,
+                             W.z.data(), rz_set(k), zero0, zero1, dn, s);
         else
         {
-            hipLaunchKernelGGL(k_update_diag, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
-                               axpy ? x : nullptr, r_.data(), p, Ap_.data(), rz_prev, pap,
-                               dinv_.data(), z_.data(), rz_set(k), zero0, zero1, dn);
+            hipLaunchKernelGGL(k_update_diag<V>, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
+                               axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(), rz_prev, pap,
+                               dinv<V>(), W.z.data(), rz_set(k), zero0, zero1, dn);
             SG_LAUNCH_CHECK();
         }
     };
 
-    PcgStats st;
+    const bool reliable = (b64 != nullptr);  // FP64 residual replacement (mixed mode)
+    PcgStats   st;
     precond(false, 0, nullptr, nullptr);
     if(!cfg.use_rz)
-        dot_slots(n, r_.data(), r_.data(), aux_set(0), s);
+    {
+        if(reliable)
+            dot_slots_ctl<double>(n, b64, b64, aux_set(0), nullptr, s);
+        else
+            dot_slots_ctl<V>(n, W.r.data(), W.r.data(), aux_set(0), nullptr, s);
+    }
     double rz0 = read_slot_set(rz_set(0), s);
     double rr0 = cfg.use_rz ? 0.0 : read_slot_set(aux_set(0), s);
     st.rz0     = rz0;
@@ -633,14 +772,15 @@ PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, dou
             pcx.q_set    = cfg.use_rz ? rz_set(k - 1) : aux_set(1);
             pcx.rz_check = rz_set(k - 1);
             pcx.thr      = thr;
+            pcx.div_thr  = reliable ? 10.0 * q0 : 0.0;
             pcx.check_k  = k - 1;
         }
-        double*       pcur  = p_[k & 1].data();
-        const double* pprev = (k == 1) ? nullptr : p_[(k - 1) & 1].data();
+        V*       pcur  = W.p[k & 1].data();
+        const V* pprev = (k == 1) ? nullptr : W.p[(k - 1) & 1].data();
         if(prof)
             prof->begin("spmv", s);
-        spmv_pcg(S, z_.data(), pprev, pcur, Ap_.data(), rz_set(k - 1),
-                 k >= 2 ? rz_set(k - 2) : rz_set(k - 1), pap_set(k), pcx, s);
+        launch_spmv<MODE_PCG, V>(S, W.z.data(), pprev, pcur, nullptr, W.Ap.data(), rz_set(k - 1),
+                                 k >= 2 ? rz_set(k - 2) : rz_set(k - 1), pap_set(k), pcx, s);
         if(prof)
             prof->end("spmv", s);
         if(test)
@@ -662,36 +802,120 @@ PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, dou
             break;  // the last launch only evaluated the final stop test
         if(k % every == 0)
         {
-            hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, r_.data(),
-                               pcur, Ap_.data(), rz_set(k - 1), pap_set(k), done);
+            hipLaunchKernelGGL(k_axpy<V>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x,
+                               W.r.data(), pcur, W.Ap.data(), rz_set(k - 1), pap_set(k), done);
             SG_LAUNCH_CHECK();
-            residual_ctl(S, b, x, r_.data(), ctl_only, s);
+            if(reliable)
+            {
+                // x64 += x; x = 0; r = b - A x64 in FP64, rounded into the
+                // working residual (the search direction is kept)
+                hipLaunchKernelGGL(k_fold<V>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x64, x,
+                                   done);
+                SG_LAUNCH_CHECK();
+                launch_spmv<MODE_RESID, double>(S, x64, nullptr, nullptr, b64, v64_.res.data(),
+                                                nullptr, nullptr, nullptr, ctl_only, s);
+                hipLaunchKernelGGL(k_round<V>, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                                   v64_.res.data(), W.r.data(), done);
+                SG_LAUNCH_CHECK();
+            }
+            else
+                launch_spmv<MODE_RESID, V>(S, x, nullptr, nullptr, b, W.r.data(), nullptr,
+                                           nullptr, nullptr, ctl_only, s);
             precond(false, k, pcur, done);
             if(!cfg.use_rz)
             {
                 hipLaunchKernelGGL(k_clear_set, dim3(1), dim3(64), 0, s, aux_set(1), done);
                 SG_LAUNCH_CHECK();
-                dot_slots_ctl(n, r_.data(), r_.data(), aux_set(1), done, s);
+                if(reliable)
+                    dot_slots_ctl<double>(n, v64_.res.data(), v64_.res.data(), aux_set(1), done, s);
+                else
+                    dot_slots_ctl<V>(n, W.r.data(), W.r.data(), aux_set(1), done, s);
             }
         }
         else
             precond(true, k, pcur, done);
+    }
+    if(reliable)
+    {
+        // updates after the last fold (the loop ended without a stop)
+        hipLaunchKernelGGL(k_fold<V>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x64, x,
+                           (const int*)nullptr);
+        SG_LAUNCH_CHECK();
     }
     int    hc[2];
     double hd[1];
     SG_HIP(hipMemcpyAsync(hc, ctl_.data(), sizeof(hc), hipMemcpyDeviceToHost, s));
     SG_HIP(hipMemcpyAsync(hd, ctl_d_.data(), sizeof(hd), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
-    SG_CHECK(hc[1] == 0, -4, "PCG breakdown: non-finite r.z");
+    if(!reliable)
+        SG_CHECK(hc[1] == 0, -4, "PCG breakdown: non-finite r.z");
+    st.breakdown  = hc[1];
     st.rz         = hd[0];
-    st.converged  = hc[0] != 0;
+    st.converged  = hc[0] != 0 && hc[1] == 0;
     st.iterations = hc[0] != 0 ? hc[0] : maxit;
-    // FP64 true relative residual
-    residual(S, b, x, tmp_.data(), s);
-    dot_slots(n, tmp_.data(), tmp_.data(), aux_set(2), s);
+    return st;
+}
+
+// FP64 true relative residual ||b - A x|| / ||b|| (the FP64 operator)
+double Pcg::true_residual(const SysView& S, const double* b, const double* x, hipStream_t s)
+{
+    const int n = 3 * S.A->nb;
+    ensure(n);
+    v64_.ensure(n);
+    SG_HIP(hipMemsetAsync(aux_set(2), 0, 2 * kSlotSet * sizeof(double), s));
+    residual(S, b, x, v64_.tmp.data(), s);
+    dot_slots(n, v64_.tmp.data(), v64_.tmp.data(), aux_set(2), s);
     dot_slots(n, b, b, aux_set(3), s);
     double rr = read_slot_set(aux_set(2), s), bb = read_slot_set(aux_set(3), s);
-    st.rel_residual = bb > 0 ? sqrt(rr / bb) : 0.0;
+    return bb > 0 ? sqrt(rr / bb) : 0.0;
+}
+
+PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
+                    const PcgConfig& cfg, hipStream_t s)
+{
+    if(cfg.mixed)
+        return solve_mixed(S, pc, mas, b, x, cfg, s);
+    PcgStats st          = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
+    st.outer_iterations  = 1;
+    st.rel_residual      = true_residual(S, b, x, s);
+    return st;
+}
+
+// Mixed precision: one PCG with the FP32 copy of the matrix, FP32 vectors
+// and the FP32 MAS apply, in which the reference's every-10 residual
+// recomputation is done in FP64 ("reliable updates"): at each check the FP32
+// iterate is folded into an FP64 solution, r = b - A x is evaluated with the
+// FP64 matrix and rounded back into the working residual, while the search
+// direction is kept.  The stop test (r.r form) then measures the FP64 true
+// residual.  Iterations stay close to the FP64 solve's; each costs FP32 traffic.
+PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
+                          const PcgConfig& cfg, hipStream_t s)
+{
+    const int n = 3 * S.A->nb;
+    ensure(n);
+    refresh_fp32(S, s);
+    v64_.ensure(n);
+    v32_.ensure(n);
+    SG_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+    convert<double, float>(n, b, v32_.rhs.data(), s);
+    PcgConfig c32 = cfg;
+    if(c32.max_iter <= 0)
+        c32.max_iter = n < 20000 ? n : 20000;
+    PcgStats st = solve_t<float>(S, pc, mas, v32_.rhs.data(), v32_.sol.data(), c32, s, b, x);
+    st.outer_iterations = 1;
+    st.rel_residual     = true_residual(S, b, x, s);
+    bool ok = st.breakdown == 0 && st.rel_residual == st.rel_residual
+              && (st.converged || cfg.max_iter > 0);
+    if(!ok)
+    {
+        // the FP32 iteration broke down or diverged (FP32 rounding of a badly
+        // conditioned operator): solve again in FP64
+        PcgStats f           = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
+        f.iterations        += st.iterations;
+        f.outer_iterations   = 2;
+        f.rel_residual       = true_residual(S, b, x, s);
+        return f;
+    }
     return st;
 }
 

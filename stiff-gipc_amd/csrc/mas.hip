@@ -451,12 +451,13 @@ __device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, in
 
 // Level-0 banks (one wave each): optional PCG update of x and r
 // (a = rz[k-1] / pAp[k] from slot sets), z0 = Binv r, and the level-1 (and,
-// with up2, level-2) cluster sums of r.
-template <typename T>
+// with up2, level-2) cluster sums of r.  T: preconditioner precision, V:
+// precision of the PCG vectors.
+template <typename T, typename V>
 __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_slot,
-                                               double* __restrict__ x, double* __restrict__ r,
-                                               const double* __restrict__ p,
-                                               const double* __restrict__ Ap,
+                                               V* __restrict__ x, V* __restrict__ r,
+                                               const V* __restrict__ p,
+                                               const V* __restrict__ Ap,
                                                const double* __restrict__ rz_prev,
                                                const double* __restrict__ pap,
                                                const T* __restrict__ inv,
@@ -477,15 +478,16 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
     const int s  = b * BANK + t / 3;
     const int cc = t % 3;
     int       v  = (t < BD) ? vert_of_slot[s] : -1;
-    double    rv = 0.0;
+    V         rv = 0;
     if(v >= 0)
     {
-        size_t d = 3 * (size_t)v + cc;
-        rv       = r[d];
+        size_t d  = 3 * (size_t)v + cc;
+        V      av = (V)a;
+        rv        = r[d];
         if(x)
         {
-            x[d] += a * p[d];
-            rv -= a * Ap[d];
+            x[d] += av * p[d];
+            rv -= av * Ap[d];
             r[d] = rv;
         }
     }
@@ -526,11 +528,12 @@ __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
 
 // The coarse levels left after the separate launches, in one launch.
 // Workgroups [0, nwg_b) give one wave to each bank of level lb (R complete
-// from earlier launches, no restriction needed).  The last workgroup (when
-// lt < L) owns every level from lt up: it loads R_lt (complete: stored by the
-// level lt-1 kernel, or accumulated by the level lt-2 kernel and then cleared
-// here for the next apply), sums the higher levels' R in LDS along
-// goingNext, and gives one wave to each bank of those levels.
+// from earlier launches, no restriction needed).  The remaining workgroups
+// (when lt < L) own the levels from lt up: each loads R_lt (complete: stored
+// by the level lt-1 kernel, or accumulated by the level lt-2 kernel and
+// cleared by the collect kernel afterwards), sums the higher levels' R in
+// LDS along goingNext, and gives one wave to each of its share of the banks
+// of those levels.
 constexpr int kCoarseWaves = 16;
 constexpr int kTailBanks   = 64;    // a level with at most this many banks may start the tail
 constexpr int kTailNodes   = 2048;  // node span of the tail levels (a multiple of 16)
@@ -542,7 +545,7 @@ struct LevelTable
 
 template <typename T>
 __global__ void __launch_bounds__(64 * kCoarseWaves)
-    k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt, int clear_lt,
+    k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt,
                  const T* __restrict__ inv, const int* __restrict__ going_next,
                  T* __restrict__ R, T* __restrict__ Z, const int* __restrict__ done)
 {
@@ -576,11 +579,7 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
     {
         const int nc = lv.count[lt];
         for(int i = threadIdx.x; i < 3 * nc; i += blockDim.x)
-        {
             Rs[i] = R[3 * (size_t)base + i];
-            if(clear_lt)
-                R[3 * (size_t)base + i] = (T)0;
-        }
     }
     __syncthreads();
     for(int l = lt; l + 1 < L; l++)
@@ -593,8 +592,11 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         }
         __syncthreads();
     }
-    const int b0 = base / BANK;
-    for(int bi = w; bi < nb; bi += kCoarseWaves)
+    // the tail workgroups split the tail banks (each restricted R redundantly)
+    const int b0  = base / BANK;
+    const int tw  = blockIdx.x - nwg_b;
+    const int ntw = gridDim.x - nwg_b;
+    for(int bi = tw * kCoarseWaves + w; bi < nb; bi += ntw * kCoarseWaves)
     {
         const int gb = b0 + bi;
         T         z  = bank_product_lds(inv, gb, Rs + 3 * (gb * BANK - base));
@@ -604,15 +606,18 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
 
 // z_v = Z[slot(v)] + sum_l Z[chain_l(v)];  r.z into a slot set; workgroup 0
 // clears two slot sets for later iterations
-template <typename T>
+template <typename T, typename V>
 __global__ void __launch_bounds__(256)
     k_mas_collect(int nv, int L, const int* __restrict__ slot_of, const int* __restrict__ chain,
-                  const T* __restrict__ Z, const double* __restrict__ r, double* __restrict__ z,
-                  double* rz_out, double* zero0, double* zero1, const int* __restrict__ done)
+                  const T* __restrict__ Z, const V* __restrict__ r, V* __restrict__ z,
+                  double* rz_out, double* zero0, double* zero1, T* __restrict__ clr, int nclr,
+                  const int* __restrict__ done)
 {
     if(done && *done)
         return;
     int    v = blockIdx.x * blockDim.x + threadIdx.x;
+    if(v < nclr)
+        clr[v] = (T)0;  // R of a level accumulated by atomics, for the next apply
     double d = 0.0;
     if(v < nv)
     {
@@ -627,12 +632,12 @@ __global__ void __launch_bounds__(256)
             z1 += zg[1];
             z2 += zg[2];
         }
-        z[3 * (size_t)v]     = (double)z0;
-        z[3 * (size_t)v + 1] = (double)z1;
-        z[3 * (size_t)v + 2] = (double)z2;
+        z[3 * (size_t)v]     = (V)z0;
+        z[3 * (size_t)v + 1] = (V)z1;
+        z[3 * (size_t)v + 2] = (V)z2;
         if(rz_out)
-            d = r[3 * (size_t)v] * (double)z0 + r[3 * (size_t)v + 1] * (double)z1
-                + r[3 * (size_t)v + 2] * (double)z2;
+            d = (double)r[3 * (size_t)v] * (double)z0 + (double)r[3 * (size_t)v + 1] * (double)z1
+                + (double)r[3 * (size_t)v + 2] * (double)z2;
     }
     if(rz_out)
         slot_set_add(rz_out, d);
@@ -831,12 +836,12 @@ void Mas::invert(hipStream_t s)
     SG_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, typename V>
 static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::vector<int>& lvl,
                         const int* vert_of_slot, const int* slot_of, const int* chain,
                         const int* going_next, const int* bank_first, const int* bank_ncl,
-                        const T* inv, T* R, T* Z, double* x, double* r, const double* p,
-                        const double* Ap, const double* rz_prev, const double* pap, double* z,
+                        const T* inv, T* R, T* Z, V* x, V* r, const V* p,
+                        const V* Ap, const double* rz_prev, const double* pap, V* z,
                         double* rz_out, double* zero0, double* zero1, const int* done,
                         hipStream_t s)
 {
@@ -859,9 +864,11 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         lb = (lt >= 2) ? lt - 1 : -1;
     }
     const int up2_level = (lb >= 1 && lt < L) ? lb - 1 : -1;  // level whose kernel feeds R_lt
+    T*        clr       = nullptr;                            // R_lt, cleared by collect
+    int       nclr      = 0;
     if(prof)
         prof->begin("mas_apply_l0", s);
-    hipLaunchKernelGGL(k_mas_l0<T>, dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
+    hipLaunchKernelGGL((k_mas_l0<T, V>), dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
                        rz_prev, pap, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0,
                        up2_level == 0 ? 1 : 0, done);
     if(prof)
@@ -888,35 +895,64 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         int nwg_b = 0;
         if(lb >= 1)
             nwg_b = ((lvl[2 * lb] + BANK - 1) / BANK + kCoarseWaves - 1) / kCoarseWaves;
-        int grid = nwg_b + (lt < L ? 1 : 0);
+        int ntw = 0;
+        if(lt < L)
+        {
+            int span = lvl[2 * (L - 1) + 1] + lvl[2 * (L - 1)] - lvl[2 * lt + 1];
+            int nbt  = (span + BANK - 1) / BANK;
+            ntw      = (nbt + kCoarseWaves - 1) / kCoarseWaves;
+        }
+        int grid = nwg_b + ntw;
         if(grid > 0)
         {
             hipLaunchKernelGGL(k_mas_coarse<T>, dim3(grid), dim3(64 * kCoarseWaves), 0, s, lv, L,
-                               lb, nwg_b, lt, up2_level >= 0 ? 1 : 0, inv, going_next, R, Z,
-                               done);
+                               lb, nwg_b, lt, inv, going_next, R, Z, done);
             SG_LAUNCH_CHECK();
         }
+        if(up2_level >= 0)
+        {
+            clr  = R + 3 * (size_t)lvl[2 * lt + 1];
+            nclr = 3 * lvl[2 * lt];
+        }
     }
-    hipLaunchKernelGGL(k_mas_collect<T>, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L,
-                       slot_of, chain, Z, r, z, rz_out, zero0, zero1, done);
+    SG_CHECK(nclr <= (long long)grid_for(nv, 256) * 256, -1, "MAS: tail level larger than the vertex count");
+    hipLaunchKernelGGL((k_mas_collect<T, V>), dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L,
+                       slot_of, chain, Z, r, z, rz_out, zero0, zero1, clr, nclr, done);
     SG_LAUNCH_CHECK();
+}
+
+template <typename V>
+void Mas::apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
+                        const double* pap, V* z, double* rz_out, double* zero0, double* zero1,
+                        const int* done, hipStream_t s)
+{
+    SG_CHECK(built_, -5, "MAS apply before build");
+    if(cfg_.fp32)
+        mas_apply_t<float, V>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(),
+                              slot_of_.data(), chain_.data(), going_next_.data(),
+                              bank_first_.data(), bank_ncl_.data(), inv32_.data(), R32_.data(),
+                              Z32_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
+                              done, s);
+    else
+        mas_apply_t<double, V>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(),
+                               slot_of_.data(), chain_.data(), going_next_.data(),
+                               bank_first_.data(), bank_ncl_.data(), inv64_.data(), R64_.data(),
+                               Z64_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
+                               done, s);
 }
 
 void Mas::apply_fused(double* x, double* r, const double* p, const double* Ap,
                       const double* rz_prev, const double* pap, double* z, double* rz_out,
                       double* zero0, double* zero1, const int* done, hipStream_t s)
 {
-    SG_CHECK(built_, -5, "MAS apply before build");
-    if(cfg_.fp32)
-        mas_apply_t<float>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
-                           chain_.data(), going_next_.data(), bank_first_.data(),
-                           bank_ncl_.data(), inv32_.data(), R32_.data(), Z32_.data(), x, r, p,
-                           Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
-    else
-        mas_apply_t<double>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(), slot_of_.data(),
-                            chain_.data(), going_next_.data(), bank_first_.data(),
-                            bank_ncl_.data(), inv64_.data(), R64_.data(), Z64_.data(), x, r, p,
-                            Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
+    apply_fused_t<double>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
+}
+
+void Mas::apply_fused(float* x, float* r, const float* p, const float* Ap,
+                      const double* rz_prev, const double* pap, float* z, double* rz_out,
+                      double* zero0, double* zero1, const int* done, hipStream_t s)
+{
+    apply_fused_t<float>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
 }
 
 void Mas::apply(const double* r, double* z, hipStream_t s)

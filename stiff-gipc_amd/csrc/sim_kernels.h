@@ -8,6 +8,22 @@
 namespace sgipc
 {
 
+// lagged friction sets (rebuilt at every time step start)
+struct FricDev
+{
+    double    mu = 0.0, mu_gd = 0.0, eps2 = 0.0;
+    const double* xprev = nullptr;  // positions at the step start
+    int       cap    = 0;           // self pairs capacity
+    int4*     code   = nullptr;     // lagged contact codes
+    double*   lam    = nullptr;     // lagged normal force
+    double*   coord  = nullptr;     // 2 per pair
+    double*   basis  = nullptr;     // 6 per pair (3x2 row-major)
+    unsigned* count  = nullptr;
+    uint32_t* gids   = nullptr;     // ground contacts
+    double*   glam   = nullptr;
+    unsigned* gcount = nullptr;
+};
+
 struct SimDev
 {
     int    nv = 0, nt = 0, ns = 0, nf = 0, ne = 0;
@@ -18,7 +34,14 @@ struct SimDev
     double *x = nullptr, *xt = nullptr, *mass = nullptr, *gshape = nullptr, *gcontact = nullptr;
     double* vals = nullptr;
     const double *dminv = nullptr, *vol = nullptr, *mu = nullptr, *lam = nullptr;
-    const int *btype = nullptr, *body = nullptr, *diag = nullptr, *tslot = nullptr;
+    const int *btype = nullptr, *body = nullptr, *diag = nullptr;
+    // FEM assembly without atomics: k_fem writes each tet's 10 upper blocks
+    // and 12 gradient entries (kFemStride doubles) to fem_buf; gathers add
+    // them per upper BSR block (gh_*) and per vertex (gv_*)
+    double*    fem_buf = nullptr;
+    int        n_up    = 0;
+    const int *gh_up = nullptr, *gh_lo = nullptr, *gh_ptr = nullptr, *gh_src = nullptr;
+    const int *gv_ptr = nullptr, *gv_src = nullptr;
     const int *rowptr = nullptr, *cols = nullptr;
     const uint32_t *tets = nullptr, *surf = nullptr, *faces = nullptr, *edges = nullptr;
     // ground contacts
@@ -35,12 +58,21 @@ struct SimDev
     unsigned* coo_count = nullptr;
     int       coo_cap   = 0;
     int*      overflow  = nullptr;  // [0] pairs, [1] coo, [2] ccd
+    FricDev   fric;
 };
 
+constexpr int kFemStride = 102;  // 10 blocks x 9 + 12 gradient entries per tet
+// (a, b), a <= b, of the 10 upper vertex pairs of a tet
+constexpr int kPairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+constexpr int kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+
+// partial-sum segments of the six energy terms: kinetic, elastic, barrier,
+// ground, friction, ground friction
+constexpr int kEnergyTerms = 6;
 struct EnergyLayout
 {
-    int n[4];
-    int off[4];
+    int n[kEnergyTerms];
+    int off[kEnergyTerms];
 };
 
 // fem.hip
@@ -49,6 +81,11 @@ void launch_fem(const SimDev& d, bool hessian, hipStream_t s);
 void launch_ground_detect(const SimDev& d, hipStream_t s);
 void launch_ground(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);
 void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);
+// friction.hip
+void launch_friction_build(const SimDev& d, double kappa, hipStream_t s);
+void launch_friction(const SimDev& d, bool hessian, double* gcontact, hipStream_t s);
+void launch_energy_friction(const SimDev& d, const EnergyLayout& L, double* partial,
+                            hipStream_t s);
 void launch_rhs(const SimDev& d, double* rhs, hipStream_t s);
 EnergyLayout energy_layout(const SimDev& d);
 void launch_energy(const SimDev& d, double kappa, double* partial, double* terms, hipStream_t s);
@@ -68,6 +105,8 @@ struct Lbvh
     DBuf<uint32_t>    nodes;   // 4 (2n - 1) {parent, left, right, element}
     DBuf<uint64_t>    keys, keys_alt;
     DBuf<unsigned>    flags;
+    DBuf<int>         rng;                // 2 (n - 1): sorted leaf range of each internal node
+    DBuf<double>      agg_blk, agg_sup;   // unions of 64 leaves / of 64 such blocks
     DBuf<unsigned>    hist;
     DBuf<double>      scene, partial;
     void init(int n, int arity, const uint32_t* prims);

@@ -42,7 +42,8 @@ class PcgStats(C.Structure):
 
 class IterStats(C.Structure):
     _fields_ = [("pcg_iterations", C.c_int32), ("n_pairs", C.c_int32), ("n_ground", C.c_int32),
-                ("line_search_steps", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
+                ("line_search_steps", C.c_int32), ("converged", C.c_int32),
+                ("solver_fallback", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("alpha", C.c_double),
                 ("alpha_ccd", C.c_double), ("xdelta", C.c_double), ("energy0", C.c_double),
                 ("energy1", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
@@ -60,7 +61,7 @@ class StepStats(C.Structure):
 
 class AdvanceStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_int32), ("time_steps_begun", C.c_int32),
-                ("pcg_iterations", C.c_int32), ("pad", C.c_int32),
+                ("pcg_iterations", C.c_int32), ("solver_fallbacks", C.c_int32),
                 ("max_pcg_rel_residual", C.c_double), ("ms_total", C.c_double)]
 
 
@@ -80,6 +81,7 @@ EXPORTS = [
     "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
     "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step", "sgipc_advance",
     "sgipc_profile_enable", "sgipc_profile_query", "sgipc_problem_sizes",
+    "sgipc_set_state", "sgipc_set_pcg",
 ]
 
 
@@ -107,6 +109,8 @@ def lib():
                 getattr(L, f).argtypes = [C.c_void_p, dp]
             L.sgipc_get_kappa.argtypes = [C.c_void_p, dp]
             L.sgipc_set_kappa.argtypes = [C.c_void_p, C.c_double]
+            L.sgipc_set_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_double]
+            L.sgipc_set_pcg.argtypes = [C.c_void_p, C.POINTER(PcgCfg)]
             L.sgipc_lbvh_build.argtypes = [C.c_void_p, C.c_int32, u64p, up, dp]
             L.sgipc_contact_query.argtypes = [C.c_void_p, ip, C.c_int32, ip, up, C.c_int32, ip]
             L.sgipc_assemble.argtypes = [C.c_void_p, dp, ip, ip, ip, dp, C.c_int32]
@@ -293,6 +297,16 @@ class Simulation:
         st = AdvanceStats()
         check(lib().sgipc_advance(self.h, int(n), C.byref(st)))
         return {f: getattr(st, f) for f, _ in st._fields_ if f != "pad"}
+
+    def set_state(self, x, v, xt, kappa):
+        """State at a time-step boundary (x_prev = x), like the oracle's."""
+        a = [np.ascontiguousarray(q, np.float64).ravel() for q in (x, v, xt)]
+        check(lib().sgipc_set_state(self.h, P(a[0]), P(a[1]), P(a[2]), float(kappa)))
+
+    def set_pcg(self, rel_tol=1e-4, abs_tol=0.0, max_iter=0, use_rz=True, check_every=10,
+                solver_precision=SGIPC_SOLVE_FP64):
+        cfg = PcgCfg(rel_tol, abs_tol, max_iter, int(use_rz), check_every, solver_precision)
+        check(lib().sgipc_set_pcg(self.h, C.byref(cfg)))
 
     def profile(self, every=1):
         """Time every `every`-th launch of each profiled kernel (0: off)."""

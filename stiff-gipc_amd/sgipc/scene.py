@@ -46,6 +46,7 @@ class SceneDesc(C.Structure):
         ("ground_offset", C.c_double),
         ("kappa", C.c_double),
         ("friction_mu", C.c_double),
+        ("ground_friction_mu", C.c_double),
         ("abs_xdelta_tol", C.c_double),
         ("rel_xdelta_tol", C.c_double),
         ("armijo_c1", C.c_double),
@@ -90,7 +91,8 @@ class Scene:
     ground_normal: tuple = (0.0, 1.0, 0.0)
     ground_offset: float = 0.0
     kappa: float = 0.0
-    friction_mu: float = 0.0
+    friction_mu: float = 0.0          # self-contact friction (reference friction_rate)
+    ground_friction_mu: float = 0.0   # ground friction (reference gd_friction_rate)
     abs_xdelta_tol: float = 0.0
     rel_xdelta_tol: float = 0.0
     armijo_c1: float = 1e-4
@@ -161,6 +163,7 @@ class Scene:
         d.ground_offset = self.ground_offset
         d.kappa = self.kappa
         d.friction_mu = self.friction_mu
+        d.ground_friction_mu = self.ground_friction_mu
         d.abs_xdelta_tol = self.abs_xdelta_tol
         d.rel_xdelta_tol = self.rel_xdelta_tol
         d.armijo_c1 = self.armijo_c1

@@ -114,3 +114,41 @@ def test_pcg_deep_hierarchy(deep, oracle):
           flush=True)
     assert st.iterations == ito
     assert rel(x, xo) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["bar", "cubes8"])
+@pytest.mark.parametrize("pc", ["mas", "diag"])
+def test_mixed_precision_refinement(systems, oracle, name, pc):
+    """FP32 matrix/vectors/MAS PCG with its periodic residual recomputation in
+    FP64 (reliable updates) reaches the FP64 true relative residual asked for,
+    in about the FP64 solve's iteration count, and its solution agrees with the
+    dense FP64 solve (condition number permitting)."""
+    import sgipc
+    sc, g, A = systems[name]
+    pcid = sgipc.SGIPC_PRECOND_MAS if pc == "mas" else sgipc.SGIPC_PRECOND_DIAG
+    ls = sgipc.LinearSystem(A.rowptr, A.cols, A.vals, sc.part_id, pcid, sgipc.SGIPC_MAS_FP32)
+    x, st = ls.pcg(g, rel_tol=1e-6, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_MIXED)
+    D = A.dense()
+    res = np.linalg.norm(D @ x - g) / np.linalg.norm(g)
+    x64, st64 = ls.pcg(g, rel_tol=1e-6, use_rz=False)
+    print(f"[mixed] {name} {pc} inner_it={st.iterations} outer={st.outer_iterations} "
+          f"relres={st.rel_residual:.3e} (dense {res:.3e}); fp64 it={st64.iterations} "
+          f"relres={st64.rel_residual:.3e}", flush=True)
+    assert st.converged
+    assert st.rel_residual <= 1e-6
+    assert st.iterations <= 2 * st64.iterations + 20
+    assert res == pytest.approx(st.rel_residual, rel=1e-3, abs=1e-12)
+    xd = np.linalg.solve(D, g)
+    assert rel(x, xd) < 1e-3
+
+
+def test_mixed_precision_deep_hierarchy(deep, oracle):
+    import sgipc
+    A, part, b = deep
+    ls = sgipc.LinearSystem(A.rowptr, A.cols, A.vals, part, sgipc.SGIPC_PRECOND_MAS,
+                            sgipc.SGIPC_MAS_FP32)
+    x, st = ls.pcg(b, rel_tol=1e-8, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_MIXED)
+    print(f"[mixed-deep] it={st.iterations} outer={st.outer_iterations} "
+          f"relres={st.rel_residual:.3e}", flush=True)
+    assert st.converged and st.rel_residual <= 1e-8
+    assert rel(A.spmv(x), b) <= 1.01e-8
