@@ -143,8 +143,8 @@ int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
                    int32_t* cols, double* vals, int32_t cap);
 
 /* Incremental potential at the current positions, FP64 (computeEnergy,
- * GIPC.cu:10392-10451).  terms[8] = {total, kinetic, elastic, barrier,
- * ground, friction, reserved, reserved}. */
+ * GIPC.cu:10392-10451).  terms[8] = {total, kinetic, elastic (tets),
+ * barrier, ground, friction, cloth (membrane + bending), reserved}. */
 int sgipc_energy(sgipc_t ctx, double* terms);
 
 typedef struct sgipc_iter_stats {

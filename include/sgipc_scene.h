@@ -55,6 +55,16 @@ typedef struct sgipc_scene_desc {
     const double*   tet_E;   /* [n_tets] Young's modulus                        */
     const double*   tet_nu;  /* [n_tets] Poisson ratio                          */
     const double*   tet_rho; /* [n_tets] density                                */
+    /* ---- cloth (triangle shells; reference scene JSON "triangle_*" keys) ---- */
+    int32_t         n_tris;
+    const uint32_t* tris;    /* [3*n_tris] cloth triangles; their interior edges
+                                carry the bending hinges                         */
+    double          cloth_E;           /* triangle_mesh_youngs_modulus          */
+    double          cloth_nu;          /* poisson_rate used for the cloth       */
+    double          cloth_bend_E;      /* triangle_bend_youngs_modulus          */
+    double          cloth_thickness;   /* triangle_mesh_thickness               */
+    double          cloth_density;     /* triangle_mesh_density                 */
+    double          cloth_strain_rate; /* strain_rate (cubic stretch stiffening)*/
     /* ---- collision surface ---- */
     int32_t         n_faces;
     const uint32_t* faces;      /* [3*n_faces] */

@@ -164,6 +164,49 @@ void   orc_friction_ground_grad_hess(const double* x, const double* xp,
                                      const double n[3], double lam, double eps2,
                                      double coef, double g[3], double H[9]);
 
+/* ---------------- cloth (orc_cloth.c) ---------------- */
+/* stretch, shear, bend stiffnesses from the scene's cloth moduli
+ * (main.cu:303-307) */
+void   orc_cloth_params(double E, double nu, double bend_E, double thickness,
+                        double strain_rate, double* stretch, double* shear,
+                        double* bend);
+/* rest-frame 2x2 Dm^-1 (row-major) and rest area of a triangle
+ * (__calculateDm2D_double, femEnergy.cu:262-309) */
+void   orc_tri_precompute(const double* x_rest, const uint32_t t[3],
+                          double dminv[4], double* area);
+/* Baraff-Witkin energy density, PK1 (3x2 row-major) and 6x6 vec(F)
+ * Hessian (femEnergy.cu:360-412, 565-740, 1003-1041) */
+double orc_bw_psi(const double F[6], double stretch, double shear, double r);
+void   orc_bw_pk1(const double F[6], double stretch, double shear, double r,
+                  double P[6]);
+void   orc_bw_hess(const double F[6], double stretch, double shear, double r,
+                   int project, double H[36]);
+/* area-scaled (no dt^2) energy; dt^2 area-scaled gradient (9) and Hessian
+ * (9x9) (_calculate_triangle_fem_gradient_hessian, femEnergy.cu:2215-2300) */
+double orc_tri_energy(const double* x, const uint32_t t[3],
+                      const double dminv[4], double area, double stretch,
+                      double shear, double r);
+void   orc_tri_grad_hess(const double* x, const uint32_t t[3],
+                         const double dminv[4], double area, double stretch,
+                         double shear, double r, double dt, double g[9],
+                         double H[81]);
+/* signed dihedral angle of hinge (q0, q1 | q2, q3) with its gradient (12)
+ * and Hessian (12x12) when non-NULL (LibShell::edgeTheta,
+ * femEnergy.cu:142-260) */
+double orc_dihedral(const double* q0, const double* q1, const double* q2,
+                    const double* q3, double g[12], double H[144]);
+/* k |e0| (theta - theta0)^2 (no dt^2) and the dt^2-scaled gradient and PSD
+ * Hessian (femEnergy.cu:2431-2566, 2590-2630) */
+double orc_bend_energy(const double* x, const uint32_t hinge[4], double theta0,
+                       double len0, double bend);
+void   orc_bend_grad_hess(const double* x, const uint32_t hinge[4],
+                          double theta0, double len0, double bend, double dt,
+                          double g[12], double H[144]);
+/* interior edges of a triangle list as hinges {a, b, opp0, opp1}, a < b,
+ * in increasing (a, b) order (load_mesh.cpp:320-379); returns the count and
+ * writes hinges when non-NULL */
+int    orc_bend_hinges(int ntri, const uint32_t* tris, uint32_t* hinges);
+
 /* ---------------- world / Newton (orc_world.c) ---------------- */
 typedef struct orc_world orc_world;
 orc_world* orc_world_create(const sgipc_scene_desc* s);

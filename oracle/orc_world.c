@@ -12,6 +12,8 @@
  *   ground/self/CFL step bounds      :6904-6980, 7054-7217, 8821-8968
  *   _edgeTriIntersectionQuery, segTriIntersect :8188-8302, :961-1013
  *   _stepForward/_updateVelocities/_computeXTilta :7880-7920, :8016-8038
+ *   cloth membrane / bending terms (orc_cloth.c) in the assembly
+ *                                    :10142-10173 and energy :10411-10416
  * and StiffGIPC/ACCD.cu point_triangle_ccd / edge_edge_ccd :373-524.
  * Lagged friction (orc_friction.c) is rebuilt at every step start from the
  * current contact sets (IPC_Solver's buildFrictionSets, GIPC.cu:11016) and
@@ -33,6 +35,11 @@ struct orc_world
     double *x, *xr, *xprev, *v, *xt, *mass, *dminv, *vol, *mu, *lam, *dx, *xsave;
     int32_t *btype, *body, *part;
     uint32_t *tets, *faces, *edges, *surf;
+    /* cloth: triangles (Dm^-1 2x2, area) and bending hinges */
+    int       ntri, nh;
+    uint32_t *tris, *hinges;
+    double *  tri_dminv, *tri_area, *h_theta0, *h_len0;
+    double    c_stretch, c_shear, c_bend, c_rate;
     double    dhat2, kappa, bbox_diag2, scene_diag, mean_mass;
     int       cap, npairs, nccd, ngp;
     int32_t*  pairs;
@@ -644,6 +651,37 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     w->lam    = (double*)malloc(sizeof(double) * (nt ? nt : 1));
     orc_fem_precompute(nv, w->xr, nt, w->tets, s->tet_E, s->tet_nu, s->tet_rho, w->dminv,
                        w->vol, w->mu, w->lam, w->mass);
+    /* cloth (orc_cloth.c header: area and lumped-mass conventions) */
+    w->ntri      = s->n_tris;
+    w->tris      = (uint32_t*)dup(s->tris, sizeof(uint32_t) * 3 * w->ntri);
+    w->s.tris    = w->tris;
+    w->tri_dminv = (double*)malloc(sizeof(double) * 4 * (w->ntri ? w->ntri : 1));
+    w->tri_area  = (double*)malloc(sizeof(double) * (w->ntri ? w->ntri : 1));
+    orc_cloth_params(s->cloth_E, s->cloth_nu, s->cloth_bend_E, s->cloth_thickness,
+                     s->cloth_strain_rate, &w->c_stretch, &w->c_shear, &w->c_bend);
+    w->c_rate = s->cloth_strain_rate;
+    for(int f = 0; f < w->ntri; f++)
+    {
+        orc_tri_precompute(w->xr, w->tris + 3 * f, w->tri_dminv + 4 * f, w->tri_area + f);
+        double m = w->tri_area[f] * s->cloth_thickness * s->cloth_density / 3.0;
+        for(int k = 0; k < 3; k++)
+            w->mass[w->tris[3 * f + k]] += m;
+    }
+    w->nh       = orc_bend_hinges(w->ntri, w->tris, NULL);
+    w->hinges   = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (w->nh ? w->nh : 1));
+    w->h_theta0 = (double*)malloc(sizeof(double) * (w->nh ? w->nh : 1));
+    w->h_len0   = (double*)malloc(sizeof(double) * (w->nh ? w->nh : 1));
+    orc_bend_hinges(w->ntri, w->tris, w->hinges);
+    for(int h = 0; h < w->nh; h++)
+    {
+        const uint32_t* q = w->hinges + 4 * h;
+        w->h_theta0[h]    = orc_dihedral(w->xr + 3 * q[0], w->xr + 3 * q[1], w->xr + 3 * q[2],
+                                         w->xr + 3 * q[3], NULL, NULL);
+        double d2         = 0.0;
+        for(int c = 0; c < 3; c++)
+            d2 += (w->xr[3 * q[1] + c] - w->xr[3 * q[0] + c]) * (w->xr[3 * q[1] + c] - w->xr[3 * q[0] + c]);
+        w->h_len0[h] = sqrt(d2);
+    }
     double msum = 0.0;
     for(int v = 0; v < nv; v++)
         msum += w->mass[v];
@@ -712,6 +750,8 @@ void orc_world_free(orc_world* w)
     free(w->gcontact); free(w->gshape);
     free(w->fr_code); free(w->fr_lam); free(w->fr_coord); free(w->fr_basis);
     free(w->fg_ids); free(w->fg_lam);
+    free(w->tris); free(w->tri_dminv); free(w->tri_area); free(w->hinges);
+    free(w->h_theta0); free(w->h_len0);
     free(w);
 }
 
@@ -877,6 +917,36 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
             push_element(T, 4, ids, H, 1, w);
         }
     }
+    /* cloth membrane (6 blocks per triangle) and bending (10 per hinge) */
+    for(int f = 0; f < w->ntri; f++)
+    {
+        const uint32_t* t = w->tris + 3 * f;
+        double          g[9], H[81];
+        orc_tri_grad_hess(w->x, t, w->tri_dminv + 4 * f, w->tri_area[f], w->c_stretch,
+                          w->c_shear, w->c_rate, w->s.dt, g, H);
+        for(int a = 0; a < 3; a++)
+            for(int c = 0; c < 3; c++)
+                w->gshape[3 * t[a] + c] += g[3 * a + c];
+        if(T)
+        {
+            int ids[3] = {(int)t[0], (int)t[1], (int)t[2]};
+            push_element(T, 3, ids, H, 1, w);
+        }
+    }
+    for(int h = 0; h < w->nh; h++)
+    {
+        const uint32_t* q = w->hinges + 4 * h;
+        double          g[12], H[144];
+        orc_bend_grad_hess(w->x, q, w->h_theta0[h], w->h_len0[h], w->c_bend, w->s.dt, g, H);
+        for(int a = 0; a < 4; a++)
+            for(int c = 0; c < 3; c++)
+                w->gshape[3 * q[a] + c] += g[3 * a + c];
+        if(T)
+        {
+            int ids[4] = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+            push_element(T, 4, ids, H, 1, w);
+        }
+    }
     /* mass diagonal (never zeroed) */
     if(T)
         for(int v = 0; v < nv; v++)
@@ -938,13 +1008,20 @@ void orc_world_energy(orc_world* w, double* terms)
         fg += orc_friction_ground_energy(w->x + 3 * w->fg_ids[i], w->xprev + 3 * w->fg_ids[i], n,
                                          w->fg_lam[i], eps2);
     double fric = w->s.friction_mu * fs + w->s.ground_friction_mu * fg;
-    terms[0] = kin + ela + bar + gnd + fric;
+    double cloth = 0.0;
+    for(int f = 0; f < w->ntri; f++)
+        cloth += orc_tri_energy(w->x, w->tris + 3 * f, w->tri_dminv + 4 * f, w->tri_area[f],
+                                w->c_stretch, w->c_shear, w->c_rate);
+    for(int h = 0; h < w->nh; h++)
+        cloth += orc_bend_energy(w->x, w->hinges + 4 * h, w->h_theta0[h], w->h_len0[h], w->c_bend);
+    cloth *= w->s.dt * w->s.dt;
+    terms[0] = kin + ela + bar + gnd + fric + cloth;
     terms[1] = kin;
     terms[2] = ela;
     terms[3] = bar;
     terms[4] = gnd;
     terms[5] = fric;
-    terms[6] = 0.0;
+    terms[6] = cloth;
     terms[7] = 0.0;
 }
 

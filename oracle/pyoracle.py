@@ -118,6 +118,23 @@ def lib():
         L.orc_friction_ground_energy.argtypes = [dp, dp, dp, C.c_double, C.c_double]
         L.orc_friction_ground_grad_hess.argtypes = [dp, dp, dp, C.c_double, C.c_double,
                                                     C.c_double, dp, dp]
+        d = C.c_double
+        L.orc_cloth_params.argtypes = [d, d, d, d, d, dp, dp, dp]
+        L.orc_tri_precompute.argtypes = [dp, up, dp, dp]
+        L.orc_bw_psi.restype = d
+        L.orc_bw_psi.argtypes = [dp, d, d, d]
+        L.orc_bw_pk1.argtypes = [dp, d, d, d, dp]
+        L.orc_bw_hess.argtypes = [dp, d, d, d, C.c_int, dp]
+        L.orc_tri_energy.restype = d
+        L.orc_tri_energy.argtypes = [dp, up, dp, d, d, d, d]
+        L.orc_tri_grad_hess.argtypes = [dp, up, dp, d, d, d, d, d, dp, dp]
+        L.orc_dihedral.restype = d
+        L.orc_dihedral.argtypes = [dp, dp, dp, dp, dp, dp]
+        L.orc_bend_energy.restype = d
+        L.orc_bend_energy.argtypes = [dp, up, d, d, d]
+        L.orc_bend_grad_hess.argtypes = [dp, up, d, d, d, d, dp, dp]
+        L.orc_bend_hinges.restype = C.c_int
+        L.orc_bend_hinges.argtypes = [C.c_int, up, up]
         L.orc_pt_ccd.restype = C.c_double
         L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         L.orc_ee_ccd.restype = C.c_double
@@ -448,3 +465,82 @@ def friction_ground_grad_hess(x, xp, n, lam, eps2, coef):
     lib().orc_friction_ground_grad_hess(P(_f64(x)), P(_f64(xp)), P(_f64(n)), lam, eps2, coef,
                                         P(g), P(H))
     return g, H.reshape(3, 3)
+
+
+# ------------------------------------------------------------------- cloth
+def cloth_params(E, nu, bend_E, thickness, strain_rate):
+    """(stretch, shear, bend) stiffnesses"""
+    o = [C.c_double() for _ in range(3)]
+    lib().orc_cloth_params(E, nu, bend_E, thickness, strain_rate, *[C.byref(v) for v in o])
+    return tuple(v.value for v in o)
+
+
+def tri_precompute(x_rest, tri):
+    Dm = np.zeros(4)
+    a = np.zeros(1)
+    lib().orc_tri_precompute(P(_f64(x_rest).ravel()), P(np.ascontiguousarray(tri, np.uint32),
+                             C.c_uint32), P(Dm), P(a))
+    return Dm.reshape(2, 2), float(a[0])
+
+
+def bw_psi(F, stretch, shear, r):
+    return lib().orc_bw_psi(P(_f64(F).ravel()), stretch, shear, r)
+
+
+def bw_pk1(F, stretch, shear, r):
+    Pk = np.zeros(6)
+    lib().orc_bw_pk1(P(_f64(F).ravel()), stretch, shear, r, P(Pk))
+    return Pk.reshape(3, 2)
+
+
+def bw_hess(F, stretch, shear, r, project=True):
+    """6x6 Hessian in vec(F) = (F[:,0], F[:,1]) order"""
+    H = np.zeros(36)
+    lib().orc_bw_hess(P(_f64(F).ravel()), stretch, shear, r, int(project), P(H))
+    return H.reshape(6, 6)
+
+
+def tri_energy(x, tri, dminv, area, stretch, shear, r):
+    return lib().orc_tri_energy(P(_f64(x).ravel()), P(np.ascontiguousarray(tri, np.uint32),
+                                C.c_uint32), P(_f64(dminv).ravel()), area, stretch, shear, r)
+
+
+def tri_grad_hess(x, tri, dminv, area, stretch, shear, r, dt):
+    g = np.zeros(9)
+    H = np.zeros(81)
+    lib().orc_tri_grad_hess(P(_f64(x).ravel()), P(np.ascontiguousarray(tri, np.uint32),
+                            C.c_uint32), P(_f64(dminv).ravel()), area, stretch, shear, r, dt,
+                            P(g), P(H))
+    return g, H.reshape(9, 9)
+
+
+def dihedral(q0, q1, q2, q3, derivs=True):
+    """(theta, grad 12, Hessian 12x12) or theta"""
+    q = [_f64(v) for v in (q0, q1, q2, q3)]
+    if not derivs:
+        return lib().orc_dihedral(*[P(v) for v in q], None, None)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    t = lib().orc_dihedral(*[P(v) for v in q], P(g), P(H))
+    return t, g, H.reshape(12, 12)
+
+
+def bend_energy(x, hinge, theta0, len0, bend):
+    return lib().orc_bend_energy(P(_f64(x).ravel()), P(np.ascontiguousarray(hinge, np.uint32),
+                                 C.c_uint32), theta0, len0, bend)
+
+
+def bend_grad_hess(x, hinge, theta0, len0, bend, dt):
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().orc_bend_grad_hess(P(_f64(x).ravel()), P(np.ascontiguousarray(hinge, np.uint32),
+                             C.c_uint32), theta0, len0, bend, dt, P(g), P(H))
+    return g, H.reshape(12, 12)
+
+
+def bend_hinges(tris):
+    t = np.ascontiguousarray(tris, np.uint32)
+    n = lib().orc_bend_hinges(int(t.shape[0]), P(t, C.c_uint32), None)
+    h = np.zeros((max(n, 1), 4), np.uint32)
+    lib().orc_bend_hinges(int(t.shape[0]), P(t, C.c_uint32), P(h, C.c_uint32))
+    return h[:n]

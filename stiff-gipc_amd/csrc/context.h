@@ -65,7 +65,7 @@ class Context
 
     hipStream_t s_ = nullptr;
     sgipc_scene_desc desc_{};
-    int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0;
+    int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0, ntri_ = 0, nh_ = 0;
     double dHat_ = 0, kappa_ = 0, bbox_diag2_ = 0, scene_diag_ = 0, mean_mass_ = 0;
     int    newton_k_ = 0;
     bool   in_step_  = false;
@@ -76,7 +76,9 @@ class Context
     DBuf<double>   dminv_, vol_, mu_, lam_;
     DBuf<int>      btype_, body_;
     DBuf<int>      gh_up_, gh_lo_, gh_ptr_, gh_src_, gv_ptr_, gv_src_;  // FEM gather lists
-    DBuf<double>   fem_buf_;                                            // per-tet blocks
+    DBuf<double>   fem_buf_;                                            // element records
+    DBuf<uint32_t> tri_, hinge_;                                        // cloth
+    DBuf<double>   tri_dminv_, tri_area_, hinge_th0_, hinge_len0_;
     DBuf<int4>     fr_code_;                                            // lagged friction
     DBuf<double>   fr_lam_, fr_coord_, fr_basis_, fg_lam_;
     DBuf<uint32_t> fg_ids_;

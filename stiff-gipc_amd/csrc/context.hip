@@ -17,6 +17,7 @@
 #include <array>
 #include <cmath>
 
+#include "cloth.h"
 #include "context.h"
 
 namespace sgipc
@@ -97,23 +98,118 @@ Context::Context(const sgipc_scene_desc& d)
         mu[e]    = E / (2 * (1 + nu));
         lam[e]   = E * nu / ((1 + nu) * (1 - 2 * nu));
     }
-    double msum = 0;
-    for(double m : mass)
-        msum += m;
-    mean_mass_ = msum / nv_;
+    // ---- cloth constants: rest-frame Dm^-1 (2x2), area, lumped mass
+    // area * thickness * density / 3 per vertex, interior-edge hinges with
+    // their rest angle and length (femEnergy.cu:262-309, load_mesh.cpp:320-379;
+    // the reference's initFEM leaves the cloth set-up out, main.cu:351-352)
+    ntri_ = d.n_tris;
+    SG_CHECK(ntri_ == 0 || d.tris, SGIPC_EINVAL, "cloth triangles missing");
+    for(int e = 0; e < 3 * ntri_; e++)
+        SG_CHECK(d.tris[e] < (uint32_t)nv_, SGIPC_EINVAL, "cloth vertex index out of range");
+    std::vector<double>   tdm((size_t)4 * ntri_), tarea(ntri_);
+    std::vector<uint32_t> hinges;
+    std::vector<double>   th0, len0;
+    for(int f = 0; f < ntri_; f++)
+    {
+        const uint32_t* t = d.tris + 3 * (size_t)f;
+        double          u[3], v[3], n[3];
+        for(int c = 0; c < 3; c++)
+        {
+            u[c] = xr[3 * t[1] + c] - xr[3 * t[0] + c];
+            v[c] = xr[3 * t[2] + c] - xr[3 * t[0] + c];
+        }
+        n[0]      = u[1] * v[2] - u[2] * v[1];
+        n[1]      = u[2] * v[0] - u[0] * v[2];
+        n[2]      = u[0] * v[1] - u[1] * v[0];
+        double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        SG_CHECK(nn > 0.0, SGIPC_EINVAL, "degenerate rest cloth triangle");
+        tarea[f] = 0.5 * nn;
+        for(int c = 0; c < 3; c++)
+            n[c] /= nn;
+        // minimal rotation taking n to +z: I + [r]x + [r]x^2 / (1 + cos), r = n x z
+        double r[3] = {n[1], -n[0], 0.0}, cs = n[2];
+        double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if(cs + 1.0 == 0.0)
+        {
+            R[0] = -1.0;
+            R[4] = -1.0;
+        }
+        else
+        {
+            double K[9] = {0, -r[2], r[1], r[2], 0, -r[0], -r[1], r[0], 0};
+            for(int i = 0; i < 3; i++)
+                for(int j = 0; j < 3; j++)
+                {
+                    double k2 = K[i * 3] * K[j] + K[i * 3 + 1] * K[3 + j] + K[i * 3 + 2] * K[6 + j];
+                    R[i * 3 + j] += K[i * 3 + j] + k2 * (1.0 / (1.0 + cs));
+                }
+        }
+        double pr[3][2];
+        for(int k = 0; k < 3; k++)
+            for(int i = 0; i < 2; i++)
+                pr[k][i] = R[i * 3] * xr[3 * t[k]] + R[i * 3 + 1] * xr[3 * t[k] + 1]
+                           + R[i * 3 + 2] * xr[3 * t[k] + 2];
+        double D[4] = {pr[1][0] - pr[0][0], pr[2][0] - pr[0][0], pr[1][1] - pr[0][1],
+                       pr[2][1] - pr[0][1]};
+        double det  = D[0] * D[3] - D[1] * D[2];
+        double inv[4] = {D[3] / det, -D[1] / det, -D[2] / det, D[0] / det};
+        for(int k = 0; k < 4; k++)
+            tdm[(size_t)k * ntri_ + f] = inv[k];
+        double m = tarea[f] * d.cloth_thickness * d.cloth_density / 3.0;
+        for(int k = 0; k < 3; k++)
+            mass[t[k]] += m;
+    }
+    {
+        // half-edges (a < b, opposite vertex, order) sorted: interior edges
+        // are the keys seen exactly twice, opposite vertices in triangle order
+        std::vector<std::array<uint32_t, 4>> he;
+        he.reserve(3 * (size_t)ntri_);
+        for(int f = 0; f < ntri_; f++)
+            for(int k = 0; k < 3; k++)
+            {
+                uint32_t v0 = d.tris[3 * (size_t)f + k], v1 = d.tris[3 * (size_t)f + (k + 1) % 3];
+                he.push_back({std::min(v0, v1), std::max(v0, v1), (uint32_t)(3 * f + k),
+                              d.tris[3 * (size_t)f + (k + 2) % 3]});
+            }
+        std::sort(he.begin(), he.end());
+        for(size_t i = 0; i < he.size();)
+        {
+            size_t j = i;
+            while(j < he.size() && he[j][0] == he[i][0] && he[j][1] == he[i][1])
+                j++;
+            if(j - i == 2)
+            {
+                uint32_t q[4] = {he[i][0], he[i][1], he[i][3], he[i + 1][3]};
+                hinges.insert(hinges.end(), q, q + 4);
+                const double* P[4] = {xr + 3 * q[0], xr + 3 * q[1], xr + 3 * q[2], xr + 3 * q[3]};
+                th0.push_back(hinge_theta(P[0], P[1], P[2], P[3]));
+                double l2 = 0;
+                for(int c = 0; c < 3; c++)
+                    l2 += (P[1][c] - P[0][c]) * (P[1][c] - P[0][c]);
+                len0.push_back(std::sqrt(l2));
+            }
+            i = j;
+        }
+    }
+    nh_ = (int)th0.size();
 
-    // ---- static BSR pattern: tet adjacency + every diagonal
+    // ---- static BSR pattern: element adjacency (tets, cloth triangles,
+    // hinges) + every diagonal
     std::vector<std::vector<int>> adj(nv_);
     for(int v = 0; v < nv_; v++)
         adj[v].push_back(v);
-    for(int e = 0; e < nt_; e++)
-    {
-        const uint32_t* t = d.tets + 4 * (size_t)e;
-        for(int a = 0; a < 4; a++)
-            for(int b = 0; b < 4; b++)
+    auto link = [&](const uint32_t* ids, int m) {
+        for(int a = 0; a < m; a++)
+            for(int b = 0; b < m; b++)
                 if(a != b)
-                    adj[t[a]].push_back((int)t[b]);
-    }
+                    adj[ids[a]].push_back((int)ids[b]);
+    };
+    for(int e = 0; e < nt_; e++)
+        link(d.tets + 4 * (size_t)e, 4);
+    for(int f = 0; f < ntri_; f++)
+        link(d.tris + 3 * (size_t)f, 3);
+    for(int h = 0; h < nh_; h++)
+        link(hinges.data() + 4 * (size_t)h, 4);
     std::vector<int> rp(nv_ + 1, 0), cl;
     for(int v = 0; v < nv_; v++)
     {
@@ -125,43 +221,50 @@ Context::Context(const sgipc_scene_desc& d)
     cl.reserve(rp[nv_]);
     for(int v = 0; v < nv_; v++)
         cl.insert(cl.end(), adj[v].begin(), adj[v].end());
-    std::vector<int> ts((size_t)16 * nt_);
-    for(int e = 0; e < nt_; e++)
-    {
-        const uint32_t* t = d.tets + 4 * (size_t)e;
-        for(int a = 0; a < 4; a++)
-            for(int b = 0; b < 4; b++)
-            {
-                auto beg = cl.begin() + rp[t[a]], end = cl.begin() + rp[t[a] + 1];
-                ts[16 * (size_t)e + 4 * a + b] =
-                    (int)(std::lower_bound(beg, end, (int)t[b]) - cl.begin());
-            }
-    }
     A_.set_pattern(nv_, rp, cl, s_);
+    auto slot_of = [&](int r, int c) {
+        auto beg = cl.begin() + rp[r], end = cl.begin() + rp[r + 1];
+        return (int)(std::lower_bound(beg, end, c) - cl.begin());
+    };
 
-    // ---- FEM gather lists (k_fem_gather_hess / k_fem_gather_grad): every
-    // upper block (i <= j) of the static pattern lists the tet blocks that
-    // add into it (boundary vertices carry no elastic Hessian); every vertex
-    // lists the tet gradient entries that add into it
+    // ---- element gather lists (k_fem_gather_hess / k_fem_gather_grad):
+    // every upper block (i <= j) of the static pattern lists the element
+    // blocks that add into it (boundary vertices carry no elastic Hessian);
+    // every vertex lists the element gradient entries that add into it.
+    // Records: tets at 0 (kFemStride each), triangles at off_tri
+    // (kTriStride), hinges at off_hinge (kHingeStride)
+    size_t off_tri = (size_t)kFemStride * nt_;
+    size_t off_hinge = off_tri + (size_t)kTriStride * ntri_;
+    size_t buf_len   = off_hinge + (size_t)kHingeStride * nh_;
+    SG_CHECK(buf_len / 3 < (size_t)1 << 30, SGIPC_EINVAL, "too many elements for the gather codes");
     {
         std::vector<long long> key;  // upper slot << 32 | code
-        key.reserve((size_t)10 * nt_);
-        for(int e = 0; e < nt_; e++)
-        {
-            const uint32_t* t = d.tets + 4 * (size_t)e;
-            for(int pi = 0; pi < 10; pi++)
+        key.reserve((size_t)10 * nt_ + 6 * (size_t)ntri_ + 10 * (size_t)nh_);
+        std::vector<std::pair<int, int>> vkey;  // (vertex, gradient code)
+        vkey.reserve((size_t)4 * nt_ + 3 * (size_t)ntri_ + 4 * (size_t)nh_);
+        auto add_elem = [&](const uint32_t* ids, int m, size_t base, int npairs, const int* PA,
+                            const int* PB) {
+            for(int pi = 0; pi < npairs; pi++)
             {
-                int a = kPairA[pi], b = kPairB[pi];
-                int va = (int)t[a], vb = (int)t[b];
+                int va = (int)ids[PA[pi]], vb = (int)ids[PB[pi]];
                 if(bt[va] != 0 || bt[vb] != 0)
                     continue;
                 bool tr   = va > vb;
-                int  slot = tr ? ts[16 * (size_t)e + 4 * b + a] : ts[16 * (size_t)e + 4 * a + b];
-                int  code = e * 32 + pi * 2 + (tr ? 1 : 0);
-                key.push_back(((long long)slot << 32) | (unsigned)code);
+                int  slot = tr ? slot_of(vb, va) : slot_of(va, vb);
+                long long code = (long long)(((base + 9 * (size_t)pi) / 3) << 1) | (tr ? 1 : 0);
+                key.push_back(((long long)slot << 32) | code);
             }
-        }
-        SG_CHECK((long long)nt_ * 32 < (1ll << 31), SGIPC_EINVAL, "too many tets for the gather codes");
+            for(int a = 0; a < m; a++)
+                vkey.push_back({(int)ids[a], (int)((base + 9 * (size_t)npairs + 3 * (size_t)a) / 3)});
+        };
+        for(int e = 0; e < nt_; e++)
+            add_elem(d.tets + 4 * (size_t)e, 4, (size_t)kFemStride * e, 10, kPairA, kPairB);
+        for(int f = 0; f < ntri_; f++)
+            add_elem(d.tris + 3 * (size_t)f, 3, off_tri + (size_t)kTriStride * f, 6, kTriPairA,
+                     kTriPairB);
+        for(int h = 0; h < nh_; h++)
+            add_elem(hinges.data() + 4 * (size_t)h, 4, off_hinge + (size_t)kHingeStride * h, 10,
+                     kPairA, kPairB);
         std::sort(key.begin(), key.end());
         std::vector<int> up, lo, ptr, src;
         src.reserve(key.size());
@@ -183,21 +286,19 @@ Context::Context(const sgipc_scene_desc& d)
                 rowof[k] = v;
         lo.resize(up.size());
         for(size_t u = 0; u < up.size(); u++)
+            lo[u] = slot_of(cl[up[u]], rowof[up[u]]);
+        std::stable_sort(vkey.begin(), vkey.end(),
+                         [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                             return a.first < b.first;
+                         });
+        std::vector<int> vptr(nv_ + 1, 0), vsrc(vkey.size());
+        for(size_t k = 0; k < vkey.size(); k++)
         {
-            int i = rowof[up[u]], j = cl[up[u]];
-            auto beg = cl.begin() + rp[j], end = cl.begin() + rp[j + 1];
-            lo[u] = (int)(std::lower_bound(beg, end, i) - cl.begin());
+            vptr[vkey[k].first + 1]++;
+            vsrc[k] = vkey[k].second;
         }
-        std::vector<int> vptr(nv_ + 1, 0), vsrc((size_t)4 * nt_);
-        for(int e = 0; e < nt_; e++)
-            for(int a = 0; a < 4; a++)
-                vptr[d.tets[4 * (size_t)e + a] + 1]++;
         for(int v = 0; v < nv_; v++)
             vptr[v + 1] += vptr[v];
-        std::vector<int> fill(vptr.begin(), vptr.end() - 1);
-        for(int e = 0; e < nt_; e++)
-            for(int a = 0; a < 4; a++)
-                vsrc[fill[d.tets[4 * (size_t)e + a]]++] = e * 4 + a;
         gh_up_.upload(up, s_);
         gh_lo_.upload(lo, s_);
         gh_ptr_.upload(ptr, s_);
@@ -205,8 +306,30 @@ Context::Context(const sgipc_scene_desc& d)
         gv_ptr_.upload(vptr, s_);
         gv_src_.upload(vsrc, s_);
         n_up_ = (int)up.size();
-        fem_buf_.resize((size_t)kFemStride * (nt_ > 0 ? nt_ : 1));
+        fem_buf_.resize(buf_len > 0 ? buf_len : 1);
     }
+    tri_.upload(d.tris, 3 * (size_t)ntri_, s_);
+    tri_dminv_.upload(tdm, s_);
+    tri_area_.upload(tarea, s_);
+    hinge_.upload(hinges, s_);
+    hinge_th0_.upload(th0, s_);
+    hinge_len0_.upload(len0, s_);
+    {
+        ClothDev& c = dev_.cloth;
+        c.nf = ntri_, c.nh = nh_;
+        c.tris = tri_.data(), c.dminv = tri_dminv_.data(), c.area = tri_area_.data();
+        c.hinges = hinge_.data(), c.theta0 = hinge_th0_.data(), c.len0 = hinge_len0_.data();
+        double nu  = d.cloth_nu, h = d.cloth_thickness;
+        c.stretch  = d.cloth_E / (2.0 * (1.0 + nu));           // main.cu:303-307
+        c.bend     = d.cloth_bend_E * h * h * h / (24.0 * (1.0 - nu * nu));
+        c.shear    = 0.03 * c.stretch * d.cloth_strain_rate;
+        c.rate     = d.cloth_strain_rate;
+        c.off_tri = off_tri, c.off_hinge = off_hinge;
+    }
+    double msum = 0;
+    for(double m : mass)
+        msum += m;
+    mean_mass_ = msum / nv_;
 
     // ---- scene extents (GIPC::init, main.cu:853-863)
     double up[3] = {-1e32, -1e32, -1e32}, lo[3] = {1e32, 1e32, 1e32};
@@ -330,7 +453,7 @@ Context::Context(const sgipc_scene_desc& d)
 
     EnergyLayout L = energy_layout(dev_);
     epart_.resize(L.off[kEnergyTerms - 1] + L.n[kEnergyTerms - 1] + 1);
-    eterms_.resize(8);
+    eterms_.resize(kEnergyTerms + 1);
 
     if(d.precond == SGIPC_PRECOND_MAS)
     {
@@ -841,7 +964,8 @@ void Context::energy_terms(double* terms)
     terms[3] = t[2];
     terms[4] = t[3];
     terms[5] = t[4] + t[5];
-    terms[6] = terms[7] = 0.0;
+    terms[6] = t[6] + t[7];
+    terms[7] = 0.0;
 }
 
 void Context::lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs)

@@ -199,9 +199,9 @@ __global__ void __launch_bounds__(128) k_fem(int nt, const uint32_t* __restrict_
     }
 }
 
-// One thread per upper BSR block (i <= j): sum of the tet blocks listed for
-// it (code = tet * 32 + pair * 2 + transposed) added into (i, j) and, for
-// i != j, its transpose into (j, i).
+// One thread per upper BSR block (i <= j): sum of the element blocks listed
+// for it (code = (buffer offset / 3) << 1 | transposed) added into (i, j)
+// and, for i != j, its transpose into (j, i).
 __global__ void __launch_bounds__(256) k_fem_gather_hess(int n_up, const int* __restrict__ up,
                                                          const int* __restrict__ lo,
                                                          const int* __restrict__ ptr,
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(256) k_fem_gather_hess(int n_up, const int* __
     for(int k = ptr[u]; k < ptr[u + 1]; k++)
     {
         int           code = src[k];
-        const double* b    = buf + (size_t)kFemStride * (code >> 5) + 9 * ((code >> 1) & 15);
+        const double* b    = buf + 3 * (size_t)(code >> 1);
         if(code & 1)
         {
 #pragma unroll
@@ -248,8 +248,8 @@ __global__ void __launch_bounds__(256) k_fem_gather_hess(int n_up, const int* __
     }
 }
 
-// One thread per vertex: the tet gradient entries listed for it
-// (code = tet * 4 + local vertex) added into the shape gradient.
+// One thread per vertex: the element gradient entries listed for it
+// (code = buffer offset / 3) added into the shape gradient.
 __global__ void __launch_bounds__(256) k_fem_gather_grad(int nv, const int* __restrict__ ptr,
                                                          const int* __restrict__ src,
                                                          const double* __restrict__ buf,
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_fem_gather_grad(int nv, const int* __re
     for(int k = ptr[v]; k < ptr[v + 1]; k++)
     {
         int           code = src[k];
-        const double* b    = buf + (size_t)kFemStride * (code >> 2) + 90 + 3 * (code & 3);
+        const double* b    = buf + 3 * (size_t)code;
         g0 += b[0];
         g1 += b[1];
         g2 += b[2];
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(256) k_energy_ground(const unsigned* __restric
     block_partial(e, out);
 }
 
-// sum the partial arrays (fixed order) into terms[0..5]; terms[6] = total
+// sum the partial arrays (fixed order) into terms[0..7]; terms[8] = total
 __global__ void __launch_bounds__(256) k_energy_final(const double* __restrict__ partial,
                                                       EnergyLayout lay, double* __restrict__ terms)
 {
@@ -619,11 +619,16 @@ void launch_init_diag(const SimDev& d, hipStream_t s)
 
 void launch_fem(const SimDev& d, bool hessian, hipStream_t s)
 {
-    if(d.nt == 0)
+    if(d.nt == 0 && d.cloth.nf == 0 && d.cloth.nh == 0)
         return;
-    hipLaunchKernelGGL(k_fem, dim3(grid_for(d.nt, 128)), dim3(128), 0, s, d.nt, d.tets, d.dminv,
-                       d.vol, d.mu, d.lam, d.x, d.dt * d.dt, d.fem_buf, hessian ? 1 : 0);
-    SG_LAUNCH_CHECK();
+    if(d.nt)
+    {
+        hipLaunchKernelGGL(k_fem, dim3(grid_for(d.nt, 128)), dim3(128), 0, s, d.nt, d.tets,
+                           d.dminv, d.vol, d.mu, d.lam, d.x, d.dt * d.dt, d.fem_buf,
+                           hessian ? 1 : 0);
+        SG_LAUNCH_CHECK();
+    }
+    launch_cloth(d, hessian, s);
     hipLaunchKernelGGL(k_fem_gather_grad, dim3(grid_for(d.nv, 256)), dim3(256), 0, s, d.nv,
                        d.gv_ptr, d.gv_src, d.fem_buf, d.gshape);
     SG_LAUNCH_CHECK();
@@ -681,6 +686,8 @@ EnergyLayout energy_layout(const SimDev& d)
     L.n[3]   = (d.use_ground && d.ns) ? (int)grid_for(d.ns, 256) : 0;
     L.n[4]   = (d.fric.mu != 0.0 && d.fric.cap) ? (int)grid_for(d.fric.cap, 256) : 0;
     L.n[5]   = (d.fric.mu_gd != 0.0 && d.use_ground && d.ns) ? (int)grid_for(d.ns, 256) : 0;
+    L.n[6]   = d.cloth.nf ? (int)grid_for(d.cloth.nf, 256) : 0;
+    L.n[7]   = d.cloth.nh ? (int)grid_for(d.cloth.nh, 256) : 0;
     L.off[0] = 0;
     for(int k = 1; k < kEnergyTerms; k++)
         L.off[k] = L.off[k - 1] + L.n[k - 1];
@@ -703,6 +710,7 @@ void launch_energy(const SimDev& d, double kappa, double* partial, double* terms
         hipLaunchKernelGGL(k_energy_ground, dim3(L.n[3]), dim3(256), 0, s, d.gcount, d.gp, d.x,
                            d.gn, d.goff, d.dHat, kappa, partial + L.off[3]);
     launch_energy_friction(d, L, partial, s);
+    launch_energy_cloth(d, L, partial, s);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial, L, terms);
     SG_LAUNCH_CHECK();
 }

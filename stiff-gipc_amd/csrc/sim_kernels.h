@@ -24,6 +24,20 @@ struct FricDev
     unsigned* gcount = nullptr;
 };
 
+// cloth triangles (membrane) and bending hinges
+struct ClothDev
+{
+    int             nf = 0, nh = 0;
+    const uint32_t* tris   = nullptr;  // 3 per triangle
+    const double*   dminv  = nullptr;  // 2x2 rest Dm^-1, SoA: dminv[k * nf + f]
+    const double*   area   = nullptr;
+    const uint32_t* hinges = nullptr;  // 4 per hinge {a, b, opp0, opp1}
+    const double*   theta0 = nullptr;  // rest dihedral angle
+    const double*   len0   = nullptr;  // rest hinge-edge length
+    double          stretch = 0, shear = 0, bend = 0, rate = 0;
+    size_t          off_tri = 0, off_hinge = 0;  // record offsets in the element buffer
+};
+
 struct SimDev
 {
     int    nv = 0, nt = 0, ns = 0, nf = 0, ne = 0;
@@ -35,9 +49,12 @@ struct SimDev
     double* vals = nullptr;
     const double *dminv = nullptr, *vol = nullptr, *mu = nullptr, *lam = nullptr;
     const int *btype = nullptr, *body = nullptr, *diag = nullptr;
-    // FEM assembly without atomics: k_fem writes each tet's 10 upper blocks
-    // and 12 gradient entries (kFemStride doubles) to fem_buf; gathers add
-    // them per upper BSR block (gh_*) and per vertex (gv_*)
+    // element assembly without atomics: k_fem writes each tet's 10 upper
+    // blocks and 12 gradient entries (kFemStride doubles) to fem_buf, the
+    // cloth kernels append their triangle and hinge records after the tets;
+    // gathers add them per upper BSR block (gh_*) and per vertex (gv_*).
+    // Gather codes address the buffer in units of 3 doubles: a Hessian code
+    // is (offset / 3) << 1 | transposed, a gradient code is offset / 3
     double*    fem_buf = nullptr;
     int        n_up    = 0;
     const int *gh_up = nullptr, *gh_lo = nullptr, *gh_ptr = nullptr, *gh_src = nullptr;
@@ -59,6 +76,7 @@ struct SimDev
     int       coo_cap   = 0;
     int*      overflow  = nullptr;  // [0] pairs, [1] coo, [2] ccd
     FricDev   fric;
+    ClothDev  cloth;
 };
 
 constexpr int kFemStride = 102;  // 10 blocks x 9 + 12 gradient entries per tet
@@ -66,9 +84,9 @@ constexpr int kFemStride = 102;  // 10 blocks x 9 + 12 gradient entries per tet
 constexpr int kPairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
 constexpr int kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
-// partial-sum segments of the six energy terms: kinetic, elastic, barrier,
-// ground, friction, ground friction
-constexpr int kEnergyTerms = 6;
+// partial-sum segments of the energy terms: kinetic, elastic, barrier,
+// ground, friction, ground friction, cloth membrane, cloth bending
+constexpr int kEnergyTerms = 8;
 struct EnergyLayout
 {
     int n[kEnergyTerms];
@@ -86,6 +104,9 @@ void launch_friction_build(const SimDev& d, double kappa, hipStream_t s);
 void launch_friction(const SimDev& d, bool hessian, double* gcontact, hipStream_t s);
 void launch_energy_friction(const SimDev& d, const EnergyLayout& L, double* partial,
                             hipStream_t s);
+// cloth.hip
+void launch_cloth(const SimDev& d, bool hessian, hipStream_t s);
+void launch_energy_cloth(const SimDev& d, const EnergyLayout& L, double* partial, hipStream_t s);
 void launch_rhs(const SimDev& d, double* rhs, hipStream_t s);
 EnergyLayout energy_layout(const SimDev& d);
 void launch_energy(const SimDev& d, double kappa, double* partial, double* terms, hipStream_t s);

@@ -32,6 +32,14 @@ class SceneDesc(C.Structure):
         ("tet_E", C.POINTER(C.c_double)),
         ("tet_nu", C.POINTER(C.c_double)),
         ("tet_rho", C.POINTER(C.c_double)),
+        ("n_tris", C.c_int32),
+        ("tris", C.POINTER(C.c_uint32)),
+        ("cloth_E", C.c_double),
+        ("cloth_nu", C.c_double),
+        ("cloth_bend_E", C.c_double),
+        ("cloth_thickness", C.c_double),
+        ("cloth_density", C.c_double),
+        ("cloth_strain_rate", C.c_double),
         ("n_faces", C.c_int32),
         ("faces", C.POINTER(C.c_uint32)),
         ("n_edges", C.c_int32),
@@ -84,6 +92,13 @@ class Scene:
     surf_verts: np.ndarray           # (S,) uint32
     v: np.ndarray | None = None
     x_rest: np.ndarray | None = None
+    tris: np.ndarray | None = None   # (M,3) uint32 cloth triangles
+    cloth_E: float = 0.0
+    cloth_nu: float = 0.0
+    cloth_bend_E: float = 0.0
+    cloth_thickness: float = 0.0
+    cloth_density: float = 0.0
+    cloth_strain_rate: float = 0.0
     dt: float = 0.01
     dhat: float = 1e-3
     gravity: tuple = (0.0, -9.8, 0.0)
@@ -134,6 +149,7 @@ class Scene:
             np.ascontiguousarray(self.faces, np.uint32),
             np.ascontiguousarray(self.edges, np.uint32),
             np.ascontiguousarray(self.surf_verts, np.uint32),
+            None if self.tris is None else np.ascontiguousarray(self.tris, np.uint32),
         ]
         k = self._keep
         d = SceneDesc()
@@ -149,6 +165,14 @@ class Scene:
         d.tet_E = _ptr(k[7], C.c_double)
         d.tet_nu = _ptr(k[8], C.c_double)
         d.tet_rho = _ptr(k[9], C.c_double)
+        d.n_tris = 0 if self.tris is None else int(self.tris.shape[0])
+        d.tris = _ptr(k[13], C.c_uint32)
+        d.cloth_E = self.cloth_E
+        d.cloth_nu = self.cloth_nu
+        d.cloth_bend_E = self.cloth_bend_E
+        d.cloth_thickness = self.cloth_thickness
+        d.cloth_density = self.cloth_density
+        d.cloth_strain_rate = self.cloth_strain_rate
         d.n_faces = int(self.faces.shape[0])
         d.faces = _ptr(k[10], C.c_uint32)
         d.n_edges = int(self.edges.shape[0])
@@ -239,6 +263,41 @@ def surface_of(tets):
     return faces.astype(np.uint32), e.astype(np.uint32), sv.astype(np.uint32)
 
 
+def cloth_grid(n, size, origin=(0.0, 0.0, 0.0), bank=(4, 4), normal_axis=1):
+    """Square sheet of n x n vertices spanning size metres in the plane normal
+    to normal_axis, 2 (n-1)^2 triangles (alternating diagonals).  Vertices
+    are ordered bank by bank (4x4 blocks = one 16-vertex MAS bank, part ids
+    non-decreasing).  Returns (x, tris, part_id)."""
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    bi, bj = bank
+    nbj = -(-n // bj)
+    blk = (ii // bi) * nbj + (jj // bj)
+    local = (ii % bi) * bj + (jj % bj)
+    order = np.lexsort((local, blk))
+    lattice_to_new = np.empty_like(order)
+    lattice_to_new[order] = np.arange(order.size)
+    _, part_id = np.unique(blk[order], return_inverse=True)
+    h = size / (n - 1)
+    a, b = [k for k in range(3) if k != normal_axis]
+    x = np.zeros((n * n, 3))
+    x[:, a] = ii[order] * h
+    x[:, b] = jj[order] * h
+    x += np.asarray(origin, np.float64)
+
+    def vid(i, j):
+        return lattice_to_new[i * n + j]
+
+    ci, cj = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    ci, cj = ci.ravel(), cj.ravel()
+    v00, v10, v01, v11 = vid(ci, cj), vid(ci + 1, cj), vid(ci, cj + 1), vid(ci + 1, cj + 1)
+    flip = (ci + cj) % 2 == 1
+    t1 = np.where(flip[:, None], np.stack([v00, v10, v01], 1), np.stack([v00, v10, v11], 1))
+    t2 = np.where(flip[:, None], np.stack([v10, v11, v01], 1), np.stack([v00, v11, v01], 1))
+    tris = np.concatenate([t1, t2], 0).astype(np.uint32)
+    return x, tris, part_id.astype(np.int32)
+
+
 def random_rotation(rng):
     q = rng.normal(size=4)
     q /= np.linalg.norm(q)
@@ -250,10 +309,13 @@ def random_rotation(rng):
     ])
 
 
-def assemble_bodies(bodies, **kw):
-    """Concatenate bodies = [(x, tets, part_id, E, nu, rho, btype)] into a Scene
-    with per-body ids and offset part ids."""
-    xs, ts, ps, Es, nus, rhos, bts, bids = [], [], [], [], [], [], [], []
+def assemble_bodies(bodies, cloths=(), **kw):
+    """Concatenate bodies = [(x, tets, part_id, E, nu, rho, btype)] and
+    cloths = [(x, tris, part_id, btype)] into a Scene with per-body ids and
+    offset part ids.  btype is a scalar or a per-vertex array.  Cloth
+    vertices get body id -1 (self contact allowed); the collision surface
+    is the tet boundary plus every cloth triangle, edge and vertex."""
+    xs, ts, ps, Es, nus, rhos, bts, bids, trs = [], [], [], [], [], [], [], [], []
     voff = 0
     poff = 0
     for b, (x, t, p, E, nu, rho, bt) in enumerate(bodies):
@@ -263,17 +325,34 @@ def assemble_bodies(bodies, **kw):
         Es.append(np.full(t.shape[0], E))
         nus.append(np.full(t.shape[0], nu))
         rhos.append(np.full(t.shape[0], rho))
-        bts.append(np.full(x.shape[0], bt, np.int32))
+        bts.append(np.broadcast_to(np.asarray(bt, np.int32), (x.shape[0],)).copy())
         bids.append(np.full(x.shape[0], b, np.int32))
         voff += x.shape[0]
         poff = int(ps[-1].max()) + 1
-    tets = np.concatenate(ts).astype(np.uint32)
+    for (x, tr, p, bt) in cloths:
+        xs.append(x)
+        trs.append(tr.astype(np.int64) + voff)
+        ps.append(p + poff)
+        bts.append(np.broadcast_to(np.asarray(bt, np.int32), (x.shape[0],)).copy())
+        bids.append(np.full(x.shape[0], -1, np.int32))
+        voff += x.shape[0]
+        poff = int(ps[-1].max()) + 1
+    tets = np.concatenate(ts).astype(np.uint32) if ts else np.zeros((0, 4), np.uint32)
     faces, edges, sv = surface_of(tets)
+    tris = None
+    if trs:
+        tris = np.concatenate(trs).astype(np.uint32)
+        te = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]], 0)
+        faces = np.concatenate([faces, tris], 0).astype(np.uint32)
+        edges = np.unique(np.concatenate([edges.astype(np.int64), np.sort(te, 1)], 0),
+                          axis=0).astype(np.uint32)
+        sv = np.unique(np.concatenate([sv, tris.ravel()])).astype(np.uint32)
+    cat = (lambda a: np.concatenate(a) if a else np.zeros(0))
     return Scene(
-        x=np.concatenate(xs), tets=tets, E=np.concatenate(Es), nu=np.concatenate(nus),
-        rho=np.concatenate(rhos), part_id=np.concatenate(ps).astype(np.int32),
+        x=np.concatenate(xs), tets=tets, E=cat(Es), nu=cat(nus),
+        rho=cat(rhos), part_id=np.concatenate(ps).astype(np.int32),
         body_id=np.concatenate(bids), btype=np.concatenate(bts), faces=faces, edges=edges,
-        surf_verts=sv, **kw)
+        surf_verts=sv, tris=tris, **kw)
 
 
 # --------------------------------------------------------------------------
@@ -338,5 +417,36 @@ def stack_scene(n=2, cells=3, size=0.1, gap=0.5e-3, rot_deg=7.0, seed=0, E=1e6, 
     defaults = dict(dt=0.01, dhat=1e-3, name=f"stack{n}x{cells}^3")
     defaults.update(kw)
     sc = assemble_bodies(bodies, **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
+
+
+def cloth_scene(n=16, size=0.3, height=0.15, n_boxes=1, box_size=0.1, box_cells=2, seed=0,
+                E=1e6, nu=0.3, bend_stiffness=1e-4, thickness=1e-3, density=200.0,
+                strain_rate=0.1, box_E=1e9, box_rho=1000.0, fixed_boxes=True, **kw):
+    """Cloth sheet of n x n vertices (size metres square, 2 (n-1)^2 triangles)
+    held horizontally at 'height' above n_boxes tet boxes of box_size metres
+    resting on the ground at seeded random positions under the sheet (one
+    box: centred).  Membrane E, nu; the bending stiffness k = E_b h^3 /
+    (24 (1 - nu^2)) is set directly through E_b = 24 (1 - nu^2) k / h^3.
+    Boxes are fixed obstacles (btype 1) unless fixed_boxes is False."""
+    rng = np.random.default_rng(seed)
+    xc, tc, pc = cloth_grid(n, size, origin=(-size / 2, height, -size / 2))
+    xb0, tb0, pb0 = hex_grid((box_cells,) * 3, box_size / box_cells)
+    xb0 = xb0 - xb0.mean(0)
+    bodies = []
+    for k in range(n_boxes):  # n_boxes = 0: the sheet and the ground only
+        if n_boxes == 1:
+            ctr = np.array([0.0, 0.0, 0.0])
+        else:
+            ctr = np.array([rng.uniform(-0.4, 0.4) * size, 0.0, rng.uniform(-0.4, 0.4) * size])
+        ctr[1] = box_size / 2 + 1e-3 * (1 + k % 3)
+        bodies.append((xb0 + ctr, tb0, pb0, box_E, 0.3, box_rho, 1 if fixed_boxes else 0))
+    bend_E = 24.0 * (1.0 - nu * nu) * bend_stiffness / thickness ** 3
+    defaults = dict(dt=0.01, dhat=1e-3, name=f"cloth{n}x{n}_{n_boxes}boxes", cloth_E=E,
+                    cloth_nu=nu, cloth_bend_E=bend_E, cloth_thickness=thickness,
+                    cloth_density=density, cloth_strain_rate=strain_rate)
+    defaults.update(kw)
+    sc = assemble_bodies(bodies, cloths=[(xc, tc, pc, 0)], **defaults)
     sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
     return sc
