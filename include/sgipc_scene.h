@@ -65,6 +65,16 @@ typedef struct sgipc_scene_desc {
     double          cloth_thickness;   /* triangle_mesh_thickness               */
     double          cloth_density;     /* triangle_mesh_density                 */
     double          cloth_strain_rate; /* strain_rate (cubic stretch stiffening)*/
+    /* ---- affine bodies (ABD; reference abd_system/) ----
+     * A vertex with abd_body[v] = b >= 0 moves with affine body b:
+     * x = p_b + A_b (x0 - c_b), c_b the body's mass centre at creation.
+     * ABD vertices come after every other vertex, grouped by body in
+     * increasing body order; a tet is either all-ABD (one body: it only
+     * supplies mass and volume) or all-FEM.  A body is fixed when its
+     * vertices have btype != 0.                                            */
+    int32_t         n_abd;
+    const int32_t*  abd_body;  /* [n_verts] body index or -1 (NULL -> no ABD)  */
+    double          abd_kappa; /* shape stiffness kappa (reference 1e8)        */
     /* ---- collision surface ---- */
     int32_t         n_faces;
     const uint32_t* faces;      /* [3*n_faces] */

@@ -207,6 +207,17 @@ void   orc_bend_grad_hess(const double* x, const uint32_t hinge[4],
  * writes hinges when non-NULL */
 int    orc_bend_hinges(int ntri, const uint32_t* tris, uint32_t* hinges);
 
+/* ---------------- affine bodies (orc_abd.c) ---------------- */
+/* ||A A^T - I||_F^2 of q = (p, a1, a2, a3), its gradient over (a1, a2, a3)
+ * and unprojected 9x9 Hessian (details/abd_energy.inl:9-127) */
+double orc_abd_shape_energy(const double q[12]);
+void   orc_abd_shape_grad(const double q[12], double g[9]);
+void   orc_abd_shape_hess(const double q[12], double H[81]);
+/* x = p + A xbar; J^T v (12); J (3x12 row-major) (abd_jacobi_matrix.h:8-62) */
+void   orc_abd_x(const double xbar[3], const double q[12], double x[3]);
+void   orc_abd_jt(const double xbar[3], const double v[3], double out[12]);
+void   orc_abd_jmat(const double xbar[3], double J[36]);
+
 /* ---------------- world / Newton (orc_world.c) ---------------- */
 typedef struct orc_world orc_world;
 orc_world* orc_world_create(const sgipc_scene_desc* s);
@@ -217,7 +228,9 @@ void       orc_world_set_xtilde(orc_world* w, const double* xt);
 void       orc_world_get_xtilde(const orc_world* w, double* xt);
 double     orc_world_get_kappa(const orc_world* w);
 void       orc_world_set_kappa(orc_world* w, double k);
-/* assembled RHS gradient (3N) and full BSR Hessian (caller frees) */
+/* system nodes: FEM vertices, then 4 per affine body (p, a1, a2, a3) */
+int  orc_world_nodes(const orc_world* w);
+/* assembled RHS gradient (3 nodes) and full node BSR Hessian (caller frees) */
 int  orc_world_assemble(orc_world* w, double* grad, orc_bsr* H);
 /* terms[8] as sgipc_energy */
 void orc_world_energy(orc_world* w, double* terms);

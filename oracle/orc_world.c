@@ -14,6 +14,14 @@
  *   _stepForward/_updateVelocities/_computeXTilta :7880-7920, :8016-8038
  *   cloth membrane / bending terms (orc_cloth.c) in the assembly
  *                                    :10142-10173 and energy :10411-10416
+ * and the affine-body system (abd_system/): body state q, dyadic mass, q~,
+ * shape energy (orc_abd.c); vertex-level contact blocks mapped to the body
+ * nodes by J^T H J (write_barrier_hessian and the ABD-FEM coupling,
+ * setup_abd_system_gradient_and_hessian.cu:84-124, 419-601), applied here to
+ * both orientations of every block so the reduced matrix is J^T H J exactly;
+ * dq mapped back to vertex displacements by J (cal_dx_from_dq); line search
+ * on q with x = J q (step_forward, cal_x_from_q); q velocity update
+ * (update_velocity, cal_q_tilde).
  * and StiffGIPC/ACCD.cu point_triangle_ccd / edge_edge_ccd :373-524.
  * Lagged friction (orc_friction.c) is rebuilt at every step start from the
  * current contact sets (IPC_Solver's buildFrictionSets, GIPC.cu:11016) and
@@ -40,6 +48,16 @@ struct orc_world
     uint32_t *tris, *hinges;
     double *  tri_dminv, *tri_area, *h_theta0, *h_len0;
     double    c_stretch, c_shear, c_bend, c_rate;
+    /* affine bodies (orc_abd.c): system nodes are [FEM vertices | 4 per body]
+     * (p, a1, a2, a3); ABD vertices are the suffix [nfem, nv) */
+    int       nfem, nabd, nn;
+    int32_t*  abd_of;                         /* per vertex: body or -1 */
+    int*      bstart;                         /* body vertex ranges     */
+    int*      bfix;
+    int*      tet_abd;                        /* mass-only tets         */
+    double *  xbar, *bM, *bq, *bqp, *bqv, *bqt, *bqs, *bG, *bV;
+    int32_t*  npart;                          /* MAS part id per node   */
+    double*   dy;                             /* solution, 3 nn         */
     double    dhat2, kappa, bbox_diag2, scene_diag, mean_mass;
     int       cap, npairs, nccd, ngp;
     int32_t*  pairs;
@@ -71,6 +89,82 @@ static void* dup(const void* p, size_t bytes)
 }
 
 static int bt(const orc_world* w, int v) { return w->btype ? w->btype[v] : 0; }
+
+/* ---------------------------------------------------------------- bodies */
+/* q from a vertex field f by mass-weighted least squares: M^-1 sum m J^T f */
+static void body_fit(const orc_world* w, int b, const double* f, double q[12])
+{
+    double rhs[12] = {0}, Mi[144];
+    for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+    {
+        double jt[12];
+        orc_abd_jt(w->xbar + 3 * v, f + 3 * v, jt);
+        for(int k = 0; k < 12; k++)
+            rhs[k] += w->mass[v] * jt[k];
+    }
+    memcpy(Mi, w->bM + 144 * b, sizeof(Mi));
+    orc_inverse(12, Mi);
+    for(int i = 0; i < 12; i++)
+    {
+        double a = 0.0;
+        for(int k = 0; k < 12; k++)
+            a += Mi[i * 12 + k] * rhs[k];
+        q[i] = a;
+    }
+}
+
+/* vertex field of body b from q: f = J q */
+static void body_apply(const orc_world* w, int b, const double q[12], double* f)
+{
+    for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+        orc_abd_x(w->xbar + 3 * v, q, f + 3 * v);
+}
+
+/* x, x_prev, v, x~ of the ABD vertices from the body state */
+static void bodies_to_vertices(orc_world* w)
+{
+    for(int b = 0; b < w->nabd; b++)
+    {
+        body_apply(w, b, w->bq + 12 * b, w->x);
+        body_apply(w, b, w->bqp + 12 * b, w->xprev);
+        body_apply(w, b, w->bqt + 12 * b, w->xt);
+        body_apply(w, b, w->bqv + 12 * b, w->v);
+    }
+}
+
+/* body state from vertex positions x (step boundary: q_prev = q), velocities
+ * v and predictions xt */
+static void vertices_to_bodies(orc_world* w, const double* v, const double* xt)
+{
+    for(int b = 0; b < w->nabd; b++)
+    {
+        body_fit(w, b, w->x, w->bq + 12 * b);
+        memcpy(w->bqp + 12 * b, w->bq + 12 * b, sizeof(double) * 12);
+        body_fit(w, b, v, w->bqv + 12 * b);
+        body_fit(w, b, xt, w->bqt + 12 * b);
+    }
+    bodies_to_vertices(w);
+}
+
+/* node map of a vertex: FEM -> itself with C = I; ABD -> the 4 body nodes with
+ * C_0 = I, C_(1+k) = xbar e_k^T (node a_k collects f_k xbar) */
+static int node_map(const orc_world* w, int v, double C[4][9])
+{
+    memset(C, 0, sizeof(double) * 36);
+    for(int i = 0; i < 3; i++)
+        C[0][4 * i] = 1.0;
+    if(w->abd_of[v] < 0)
+        return 1;
+    const double* xb = w->xbar + 3 * v;
+    for(int k = 0; k < 3; k++)
+        for(int l = 0; l < 3; l++)
+            C[1 + k][l * 3 + k] = xb[l];
+    return 4;
+}
+static int node_base(const orc_world* w, int v)
+{
+    return w->abd_of[v] < 0 ? v : w->nfem + 4 * w->abd_of[v];
+}
 
 /* ---------------------------------------------------------------- contacts */
 static void build_cp(orc_world* w)
@@ -682,6 +776,100 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
             d2 += (w->xr[3 * q[1] + c] - w->xr[3 * q[0] + c]) * (w->xr[3 * q[1] + c] - w->xr[3 * q[0] + c]);
         w->h_len0[h] = sqrt(d2);
     }
+    /* affine bodies: mass centre, body-frame coordinates, dyadic mass,
+     * volume, q = (c, I), q' fitted to v, affine gravity G = M^-1 sum m J^T g
+     * (abd_system.cu _setup_abd_state, _setup_J, _setup_abd_dyadic_mass,
+     * _setup_abd_gravity; cal_q_tilde) */
+    w->abd_of  = (int32_t*)malloc(sizeof(int32_t) * nv);
+    w->nabd    = (s->abd_body && s->n_abd > 0) ? s->n_abd : 0;
+    w->nfem    = nv;
+    for(int v = 0; v < nv; v++)
+    {
+        w->abd_of[v] = w->nabd ? s->abd_body[v] : -1;
+        if(w->abd_of[v] >= 0 && w->nfem == nv)
+            w->nfem = v;
+    }
+    w->nn      = w->nfem + 4 * w->nabd;
+    w->tet_abd = (int*)calloc(nt ? nt : 1, sizeof(int));
+    for(int e = 0; e < nt; e++)
+        w->tet_abd[e] = w->abd_of[w->tets[4 * e]] >= 0;
+    int nb1    = w->nabd ? w->nabd : 1;
+    w->bstart  = (int*)calloc(nb1 + 1, sizeof(int));
+    w->bfix    = (int*)calloc(nb1, sizeof(int));
+    w->xbar    = (double*)calloc(3 * nv, sizeof(double));
+    w->bM      = (double*)calloc(144 * nb1, sizeof(double));
+    w->bq      = (double*)calloc(12 * nb1, sizeof(double));
+    w->bqp     = (double*)calloc(12 * nb1, sizeof(double));
+    w->bqv     = (double*)calloc(12 * nb1, sizeof(double));
+    w->bqt     = (double*)calloc(12 * nb1, sizeof(double));
+    w->bqs     = (double*)calloc(12 * nb1, sizeof(double));
+    w->bG      = (double*)calloc(12 * nb1, sizeof(double));
+    w->bV      = (double*)calloc(nb1, sizeof(double));
+    for(int v = w->nfem; v < nv; v++)
+        w->bstart[w->abd_of[v] + 1] = v + 1;
+    w->bstart[0] = w->nfem;
+    for(int e = 0; e < nt; e++)
+        if(w->tet_abd[e])
+            w->bV[w->abd_of[w->tets[4 * e]]] += w->vol[e];
+    for(int b = 0; b < w->nabd; b++)
+    {
+        double ms = 0.0, c[3] = {0, 0, 0};
+        for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+        {
+            ms += w->mass[v];
+            for(int k = 0; k < 3; k++)
+                c[k] += w->mass[v] * w->x[3 * v + k];
+        }
+        for(int k = 0; k < 3; k++)
+            c[k] /= ms;
+        w->bfix[b] = bt(w, w->bstart[b]) != 0;
+        double* M  = w->bM + 144 * b;
+        for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+        {
+            double J[36];
+            for(int k = 0; k < 3; k++)
+                w->xbar[3 * v + k] = w->x[3 * v + k] - c[k];
+            orc_abd_jmat(w->xbar + 3 * v, J);
+            for(int i = 0; i < 12; i++)
+                for(int j = 0; j < 12; j++)
+                    M[i * 12 + j] += w->mass[v]
+                                     * (J[i] * J[j] + J[12 + i] * J[12 + j] + J[24 + i] * J[24 + j]);
+        }
+        double* q = w->bq + 12 * b;
+        memset(q, 0, sizeof(double) * 12);
+        for(int k = 0; k < 3; k++)
+        {
+            q[k]         = c[k];
+            q[3 + 4 * k] = 1.0;
+        }
+        memcpy(w->bqp + 12 * b, q, sizeof(double) * 12);
+        if(!w->bfix[b])
+        {
+            double* gv = (double*)calloc(3 * nv, sizeof(double));
+            for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+                for(int k = 0; k < 3; k++)
+                    gv[3 * v + k] = s->gravity[k];
+            body_fit(w, b, gv, w->bG + 12 * b);
+            free(gv);
+            body_fit(w, b, w->v, w->bqv + 12 * b);
+        }
+        for(int k = 0; k < 12; k++)
+            w->bqt[12 * b + k] = q[k] + (w->bqv[12 * b + k] * s->dt
+                                         + w->bG[12 * b + k] * (s->dt * s->dt));
+    }
+    /* MAS parts per node: FEM vertices keep theirs, each body is one part */
+    w->npart  = (int32_t*)malloc(sizeof(int32_t) * (w->nn ? w->nn : 1));
+    int maxp  = -1;
+    for(int v = 0; v < w->nfem; v++)
+    {
+        w->npart[v] = w->part ? w->part[v] : v / 16;
+        maxp        = w->npart[v] > maxp ? w->npart[v] : maxp;
+    }
+    for(int b = 0; b < w->nabd; b++)
+        for(int k = 0; k < 4; k++)
+            w->npart[w->nfem + 4 * b + k] = maxp + 1 + b;
+    w->dy = (double*)calloc(3 * (w->nn ? w->nn : 1), sizeof(double));
+
     double msum = 0.0;
     for(int v = 0; v < nv; v++)
         msum += w->mass[v];
@@ -735,6 +923,7 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
             w->xt[3 * v + c] = w->xprev[3 * v + c]
                                + (w->v[3 * v + c] * s->dt
                                   + (bt(w, v) == 0 ? s->gravity[c] * (s->dt * s->dt) : 0.0));
+    bodies_to_vertices(w);
     build_cp(w);
     return w;
 }
@@ -752,14 +941,23 @@ void orc_world_free(orc_world* w)
     free(w->fg_ids); free(w->fg_lam);
     free(w->tris); free(w->tri_dminv); free(w->tri_area); free(w->hinges);
     free(w->h_theta0); free(w->h_len0);
+    free(w->abd_of); free(w->tet_abd); free(w->bstart); free(w->bfix); free(w->xbar);
+    free(w->bM); free(w->bq); free(w->bqp); free(w->bqv); free(w->bqt); free(w->bqs);
+    free(w->bG); free(w->bV); free(w->npart); free(w->dy);
     free(w);
 }
 
 void orc_world_set_positions(orc_world* w, const double* x)
 {
     memcpy(w->x, x, sizeof(double) * 3 * w->nv);
+    for(int b = 0; b < w->nabd; b++)   /* ABD vertices stay affine: x = J q */
+    {
+        body_fit(w, b, w->x, w->bq + 12 * b);
+        body_apply(w, b, w->bq + 12 * b, w->x);
+    }
     build_cp(w);
 }
+int orc_world_nodes(const orc_world* w) { return w->nn; }
 void orc_world_get_positions(const orc_world* w, double* x)
 {
     memcpy(x, w->x, sizeof(double) * 3 * w->nv);
@@ -841,9 +1039,11 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
 {
     int nv = w->nv;
     memset(w->gcontact, 0, sizeof(double) * 3 * nv);
+    /* kinetic on FEM vertices; affine bodies carry M (q - q~) on their nodes */
     for(int v = 0; v < nv; v++)
         for(int c = 0; c < 3; c++)
-            w->gshape[3 * v + c] = (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v];
+            w->gshape[3 * v + c] =
+                v < w->nfem ? (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v] : 0.0;
     /* barrier */
     for(int i = 0; i < w->npairs; i++)
     {
@@ -904,6 +1104,8 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
     /* FEM */
     for(int e = 0; e < w->nt; e++)
     {
+        if(w->tet_abd[e])
+            continue;
         const uint32_t* t = w->tets + 4 * e;
         double          g[12], H[144];
         orc_tet_grad_hess(w->x, t, w->dminv + 9 * e, w->vol[e], w->mu[e], w->lam[e], w->s.dt, g,
@@ -947,26 +1149,122 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
             push_element(T, 4, ids, H, 1, w);
         }
     }
-    /* mass diagonal (never zeroed) */
+    /* mass diagonal of the FEM vertices (never zeroed) */
     if(T)
-        for(int v = 0; v < nv; v++)
+        for(int v = 0; v < w->nfem; v++)
         {
             double blk[9] = {w->mass[v], 0, 0, 0, w->mass[v], 0, 0, 0, w->mass[v]};
             trip_push(T, v, v, blk);
         }
 }
 
+/* dst = C_a B C_b^T (3x3 row-major) */
+static void cbc(const double* Ca, const double* B, const double* Cb, double* dst)
+{
+    double t[9];
+    for(int i = 0; i < 3; i++)
+        for(int j = 0; j < 3; j++)
+            t[i * 3 + j] = Ca[i * 3] * B[j] + Ca[i * 3 + 1] * B[3 + j] + Ca[i * 3 + 2] * B[6 + j];
+    for(int i = 0; i < 3; i++)
+        for(int j = 0; j < 3; j++)
+            dst[i * 3 + j] = t[i * 3] * Cb[j * 3] + t[i * 3 + 1] * Cb[j * 3 + 1] + t[i * 3 + 2] * Cb[j * 3 + 2];
+}
+
+/* the node system: vertex-level upper blocks mapped through the node maps
+ * (both orientations of off-diagonal blocks, keeping node blocks with
+ * row <= col), the body blocks M + PSD(kappa V dt^2 d2 shape), the node
+ * gradient [g_FEM | sum J^T g_contact + M (q - q~) + shape] */
+static void reduce_to_nodes(orc_world* w, const trip_t* T, double* grad, trip_t* Y)
+{
+    for(int v = 0; v < w->nfem; v++)
+        for(int c = 0; c < 3; c++)
+            grad[3 * v + c] = bt(w, v) != 0 ? 0.0 : w->gcontact[3 * v + c] + w->gshape[3 * v + c];
+    double dt2 = w->s.dt * w->s.dt;
+    for(int b = 0; b < w->nabd; b++)
+    {
+        double* g = grad + 3 * (w->nfem + 4 * b);
+        memset(g, 0, sizeof(double) * 12);
+        if(w->bfix[b])
+            continue;
+        for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+        {
+            double jt[12];
+            orc_abd_jt(w->xbar + 3 * v, w->gcontact + 3 * v, jt);
+            for(int k = 0; k < 12; k++)
+                g[k] += jt[k];
+        }
+        const double* M = w->bM + 144 * b;
+        for(int i = 0; i < 12; i++)
+            for(int k = 0; k < 12; k++)
+                g[i] += M[i * 12 + k] * (w->bq[12 * b + k] - w->bqt[12 * b + k]);
+        double sg[9], kv = w->s.abd_kappa * w->bV[b] * dt2;
+        orc_abd_shape_grad(w->bq + 12 * b, sg);
+        for(int k = 0; k < 9; k++)
+            g[3 + k] += kv * sg[k];
+    }
+    if(!T)
+        return;
+    for(int t = 0; t < T->n; t++)
+    {
+        int    r = T->r[t], c = T->c[t];
+        double Cr[4][9], Cc[4][9], blk[9], bt_[9];
+        int    mr = node_map(w, r, Cr), mc = node_map(w, c, Cc);
+        int    br = node_base(w, r), bc = node_base(w, c);
+        for(int a = 0; a < mr; a++)
+            for(int b = 0; b < mc; b++)
+            {
+                cbc(Cr[a], T->v + 9 * t, Cc[b], blk);
+                int R = br + a, Cn = bc + b;
+                if(R <= Cn)
+                    trip_push(Y, R, Cn, blk);
+                if(r != c && Cn <= R)
+                {
+                    for(int i = 0; i < 3; i++)
+                        for(int j = 0; j < 3; j++)
+                            bt_[i * 3 + j] = blk[j * 3 + i];
+                    trip_push(Y, Cn, R, bt_);
+                }
+            }
+    }
+    for(int b = 0; b < w->nabd; b++)
+    {
+        double H[144];
+        memcpy(H, w->bM + 144 * b, sizeof(H));
+        if(!w->bfix[b])
+        {
+            double S[81], kv = w->s.abd_kappa * w->bV[b] * dt2;
+            orc_abd_shape_hess(w->bq + 12 * b, S);
+            for(int k = 0; k < 81; k++)
+                S[k] *= kv;
+            orc_psd_project(9, S);
+            for(int i = 0; i < 9; i++)
+                for(int j = 0; j < 9; j++)
+                    H[(3 + i) * 12 + 3 + j] += S[i * 9 + j];
+        }
+        int base = w->nfem + 4 * b;
+        for(int m = 0; m < 4; m++)
+            for(int n = m; n < 4; n++)
+            {
+                double blk[9];
+                for(int i = 0; i < 3; i++)
+                    for(int j = 0; j < 3; j++)
+                        blk[i * 3 + j] = H[(3 * m + i) * 12 + 3 * n + j];
+                trip_push(Y, base + m, base + n, blk);
+            }
+    }
+}
+
 int orc_world_assemble(orc_world* w, double* grad, orc_bsr* H)
 {
-    trip_t T = {0};
+    trip_t T = {0}, Y = {0};
     compute_grad_hess(w, H ? &T : NULL);
-    for(int v = 0; v < w->nv; v++)
-        for(int c = 0; c < 3; c++)
-            grad[3 * v + c] =
-                bt(w, v) != 0 ? 0.0 : w->gcontact[3 * v + c] + w->gshape[3 * v + c];
+    reduce_to_nodes(w, H ? &T : NULL, grad, &Y);
     int rc = 0;
     if(H)
-        rc = orc_bsr_from_upper_triplets(w->nv, T.n, T.r, T.c, T.v, H);
+        rc = orc_bsr_from_upper_triplets(w->nn, Y.n, Y.r, Y.c, Y.v, H);
+    free(Y.r);
+    free(Y.c);
+    free(Y.v);
     free(T.r);
     free(T.c);
     free(T.v);
@@ -975,8 +1273,22 @@ int orc_world_assemble(orc_world* w, double* grad, orc_bsr* H)
 
 void orc_world_energy(orc_world* w, double* terms)
 {
-    double kin = 0.0, ela = 0.0, bar = 0.0, gnd = 0.0;
-    for(int v = 0; v < w->nv; v++)
+    double kin = 0.0, ela = 0.0, bar = 0.0, gnd = 0.0, shape = 0.0;
+    for(int b = 0; b < w->nabd; b++)
+    {
+        if(w->bfix[b])
+            continue;
+        const double* M = w->bM + 144 * b;
+        double        d[12];
+        for(int k = 0; k < 12; k++)
+            d[k] = w->bq[12 * b + k] - w->bqt[12 * b + k];
+        for(int i = 0; i < 12; i++)
+            for(int k = 0; k < 12; k++)
+                kin += 0.5 * d[i] * M[i * 12 + k] * d[k];
+        shape += w->s.abd_kappa * w->bV[b] * orc_abd_shape_energy(w->bq + 12 * b);
+    }
+    shape *= w->s.dt * w->s.dt;
+    for(int v = 0; v < w->nfem; v++)
     {
         double d2 = 0.0;
         for(int c = 0; c < 3; c++)
@@ -987,7 +1299,8 @@ void orc_world_energy(orc_world* w, double* terms)
         kin += d2 * w->mass[v] * 0.5;
     }
     for(int e = 0; e < w->nt; e++)
-        ela += orc_tet_energy(w->x, w->tets + 4 * e, w->dminv + 9 * e, w->vol[e], w->mu[e],
+        if(!w->tet_abd[e])
+            ela += orc_tet_energy(w->x, w->tets + 4 * e, w->dminv + 9 * e, w->vol[e], w->mu[e],
                               w->lam[e]);
     ela *= w->s.dt * w->s.dt;
     for(int i = 0; i < w->npairs; i++)
@@ -1015,14 +1328,14 @@ void orc_world_energy(orc_world* w, double* terms)
     for(int h = 0; h < w->nh; h++)
         cloth += orc_bend_energy(w->x, w->hinges + 4 * h, w->h_theta0[h], w->h_len0[h], w->c_bend);
     cloth *= w->s.dt * w->s.dt;
-    terms[0] = kin + ela + bar + gnd + fric + cloth;
+    terms[0] = kin + ela + bar + gnd + fric + cloth + shape;
     terms[1] = kin;
     terms[2] = ela;
     terms[3] = bar;
     terms[4] = gnd;
     terms[5] = fric;
     terms[6] = cloth;
-    terms[7] = 0.0;
+    terms[7] = shape;
 }
 
 static double energy(orc_world* w)
@@ -1070,6 +1383,8 @@ static void init_kappa(orc_world* w)
             GE[3 * v + c] = (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v];
     for(int e = 0; e < w->nt; e++)
     {
+        if(w->tet_abd[e])
+            continue;
         const uint32_t* t = w->tets + 4 * e;
         double          g[12], H[144];
         orc_tet_grad_hess(w->x, t, w->dminv + 9 * e, w->vol[e], w->mu[e], w->lam[e], w->s.dt, g,
@@ -1116,10 +1431,19 @@ static void init_kappa(orc_world* w)
 /* ------------------------------------------------------------------ Newton */
 static void step_forward(orc_world* w, double alpha)
 {
-    for(int v = 0; v < w->nv; v++)
+    for(int v = 0; v < w->nfem; v++)
         if(bt(w, v) == 0)
             for(int c = 0; c < 3; c++)
                 w->x[3 * v + c] = w->xsave[3 * v + c] - w->dx[3 * v + c] * alpha;
+    for(int b = 0; b < w->nabd; b++)
+    {
+        if(w->bfix[b])
+            continue;
+        const double* dq = w->dy + 3 * (w->nfem + 4 * b);
+        for(int k = 0; k < 12; k++)
+            w->bq[12 * b + k] = w->bqs[12 * b + k] - dq[k] * alpha;
+        body_apply(w, b, w->bq + 12 * b, w->x);
+    }
 }
 
 static void line_search(orc_world* w, double* alpha, double cfl_alpha, int* nls)
@@ -1130,6 +1454,7 @@ static void line_search(orc_world* w, double* alpha, double cfl_alpha, int* nls)
         gdx += w->gcontact[i] * w->dx[i];
     double c1m = w->s.armijo_c1 * gdx;
     memcpy(w->xsave, w->x, sizeof(double) * 3 * w->nv);
+    memcpy(w->bqs, w->bq, sizeof(double) * 12 * (w->nabd ? w->nabd : 1));
     step_forward(w, *alpha);
     while(is_intersected(w))
     {
@@ -1227,7 +1552,7 @@ int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_o
                                double* xdelta_out, int* converged_out)
 {
     int      nv   = w->nv;
-    double*  grad = (double*)malloc(sizeof(double) * 3 * nv);
+    double*  grad = (double*)malloc(sizeof(double) * 3 * (w->nn ? w->nn : 1));
     orc_bsr  H    = {0};
     orc_world_assemble(w, grad, &H);
     double xdelta = 0.0;
@@ -1251,11 +1576,24 @@ int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_o
         *converged_out = 0;
     orc_mas* mas = NULL;
     if(w->s.precond == SGIPC_PRECOND_MAS)
-        mas = orc_mas_create(&H, w->part, w->s.mas_precision == SGIPC_MAS_FP32);
+        mas = orc_mas_create(&H, w->npart, w->s.mas_precision == SGIPC_MAS_FP32);
     double rz0, rz;
-    int    it = orc_pcg(&H, mas, w->s.precond == SGIPC_PRECOND_MAS ? 1 : 0, grad, w->dx,
+    int    it = orc_pcg(&H, mas, w->s.precond == SGIPC_PRECOND_MAS ? 1 : 0, grad, w->dy,
                         w->s.pcg_rel_tol, w->s.pcg_abs_tol, w->s.pcg_max_iter,
                         w->s.pcg_use_preconditioned_norm, 10, &rz0, &rz);
+    /* vertex displacements: FEM from the solution, ABD vertices J dq */
+    memcpy(w->dx, w->dy, sizeof(double) * 3 * w->nfem);
+    for(int b = 0; b < w->nabd; b++)
+    {
+        const double* dq = w->dy + 3 * (w->nfem + 4 * b);
+        for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+        {
+            double d[3];
+            orc_abd_x(w->xbar + 3 * v, dq, d);
+            for(int c = 0; c < 3; c++)
+                w->dx[3 * v + c] = d[c];
+        }
+    }
     if(pcg_iters_out)
         *pcg_iters_out = it;
     orc_mas_free(mas);
@@ -1327,6 +1665,7 @@ static void begin_step(orc_world* w)
     init_kappa(w);
     build_friction_sets(w);
     memset(w->dx, 0, sizeof(double) * 3 * w->nv);
+    memset(w->dy, 0, sizeof(double) * 3 * (w->nn ? w->nn : 1));
     w->newton_k = 0;
     w->xdelta0  = 0.0;
     w->in_step  = 1;
@@ -1334,7 +1673,23 @@ static void begin_step(orc_world* w)
 
 static void end_step(orc_world* w)
 {
-    for(int v = 0; v < w->nv; v++)
+    /* bodies: q' = (q - q_prev) / dt, q_prev = q, q~ = q + dt q' + dt^2 G
+     * (update_velocity, cal_q_tilde) */
+    for(int b = 0; b < w->nabd; b++)
+    {
+        double* q  = w->bq + 12 * b;
+        double* qp = w->bqp + 12 * b;
+        double* qv = w->bqv + 12 * b;
+        double* qt = w->bqt + 12 * b;
+        for(int k = 0; k < 12; k++)
+        {
+            qv[k] = w->bfix[b] ? 0.0 : (q[k] - qp[k]) * (1 / w->s.dt);
+            qp[k] = q[k];
+            qt[k] = w->bfix[b] ? q[k]
+                               : q[k] + (qv[k] * w->s.dt + w->bG[12 * b + k] * (w->s.dt * w->s.dt));
+        }
+    }
+    for(int v = 0; v < w->nfem; v++)
         for(int c = 0; c < 3; c++)
         {
             if(bt(w, v) == 0)
@@ -1343,11 +1698,12 @@ static void end_step(orc_world* w)
                 w->v[3 * v + c] = 0.0;
             w->xprev[3 * v + c] = w->x[3 * v + c];
         }
-    for(int v = 0; v < w->nv; v++)
+    for(int v = 0; v < w->nfem; v++)
         for(int c = 0; c < 3; c++)
             w->xt[3 * v + c] = w->xprev[3 * v + c]
                                + (w->v[3 * v + c] * w->s.dt
                                   + (bt(w, v) == 0 ? w->s.gravity[c] * (w->s.dt * w->s.dt) : 0.0));
+    bodies_to_vertices(w);
     w->in_step = 0;
 }
 
@@ -1396,6 +1752,7 @@ void orc_world_set_state(orc_world* w, const double* x, const double* v, const d
     memcpy(w->xprev, x, sizeof(double) * 3 * w->nv);
     memcpy(w->v, v, sizeof(double) * 3 * w->nv);
     memcpy(w->xt, xt, sizeof(double) * 3 * w->nv);
+    vertices_to_bodies(w, v, xt);
     w->kappa    = kappa;
     w->in_step  = 0;
     build_cp(w);

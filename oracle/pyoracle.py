@@ -135,6 +135,15 @@ def lib():
         L.orc_bend_grad_hess.argtypes = [dp, up, d, d, d, d, dp, dp]
         L.orc_bend_hinges.restype = C.c_int
         L.orc_bend_hinges.argtypes = [C.c_int, up, up]
+        L.orc_abd_shape_energy.restype = d
+        L.orc_abd_shape_energy.argtypes = [dp]
+        L.orc_abd_shape_grad.argtypes = [dp, dp]
+        L.orc_abd_shape_hess.argtypes = [dp, dp]
+        L.orc_abd_x.argtypes = [dp, dp, dp]
+        L.orc_abd_jt.argtypes = [dp, dp, dp]
+        L.orc_abd_jmat.argtypes = [dp, dp]
+        L.orc_world_nodes.restype = C.c_int
+        L.orc_world_nodes.argtypes = [C.c_void_p]
         L.orc_pt_ccd.restype = C.c_double
         L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         L.orc_ee_ccd.restype = C.c_double
@@ -326,8 +335,12 @@ class World:
     def kappa(self, k):
         lib().orc_world_set_kappa(self.h, k)
 
+    @property
+    def nodes(self):
+        return lib().orc_world_nodes(self.h)
+
     def assemble(self):
-        g = np.zeros(3 * self.nv)
+        g = np.zeros(3 * self.nodes)
         raw = Bsr()
         lib().orc_world_assemble(self.h, P(g), C.byref(raw))
         return g, OBsr(raw)
@@ -544,3 +557,20 @@ def bend_hinges(tris):
     h = np.zeros((max(n, 1), 4), np.uint32)
     lib().orc_bend_hinges(int(t.shape[0]), P(t, C.c_uint32), P(h, C.c_uint32))
     return h[:n]
+
+
+# ------------------------------------------------------------- affine bodies
+def abd_shape(q):
+    """(energy, gradient 9, unprojected Hessian 9x9) of ||A A^T - I||_F^2"""
+    q = _f64(q)
+    g = np.zeros(9)
+    H = np.zeros(81)
+    lib().orc_abd_shape_grad(P(q), P(g))
+    lib().orc_abd_shape_hess(P(q), P(H))
+    return lib().orc_abd_shape_energy(P(q)), g, H.reshape(9, 9)
+
+
+def abd_jmat(xbar):
+    J = np.zeros(36)
+    lib().orc_abd_jmat(P(_f64(xbar)), P(J))
+    return J.reshape(3, 12)

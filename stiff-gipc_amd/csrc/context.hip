@@ -726,9 +726,10 @@ void Context::init_kappa()
 {
     if(read_u(pcount_.data()) == 0)
         return;
-    // GE = kinetic + elastic gradient (gs), gc = unit-kappa ground + barrier
+    // GE = kinetic + tet elastic gradient (gs; the reference leaves the cloth
+    // gradient out, GIPC.cu:9809), gc = unit-kappa ground + barrier
     launch_init_diag(dev_, s_);
-    launch_fem(dev_, false, s_);
+    launch_fem(dev_, false, s_, false);
     launch_ground(dev_, 1.0, false, gc_.data(), s_);
     launch_barrier(dev_, 1.0, false, gc_.data(), s_);
     SG_HIP(hipMemsetAsync(scratch_.data() + 4, 0, 2 * sizeof(double), s_));

@@ -18,6 +18,8 @@
 // loads is a coalesced 512-byte wave access.  FP64 atomics from every tet
 // into shared BSR blocks are avoided: each BSR block (and vertex) owns a
 // precomputed list of the tet blocks that add into it.
+#include <climits>
+
 #include "assembly.h"
 
 namespace sgipc
@@ -249,11 +251,13 @@ __global__ void __launch_bounds__(256) k_fem_gather_hess(int n_up, const int* __
 }
 
 // One thread per vertex: the element gradient entries listed for it
-// (code = buffer offset / 3) added into the shape gradient.
+// (code = buffer offset / 3, those below code_lim) added into the shape
+// gradient.
 __global__ void __launch_bounds__(256) k_fem_gather_grad(int nv, const int* __restrict__ ptr,
                                                          const int* __restrict__ src,
                                                          const double* __restrict__ buf,
-                                                         double* __restrict__ gshape)
+                                                         double* __restrict__ gshape,
+                                                         int code_lim)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if(v >= nv)
@@ -262,6 +266,8 @@ __global__ void __launch_bounds__(256) k_fem_gather_grad(int nv, const int* __re
     for(int k = ptr[v]; k < ptr[v + 1]; k++)
     {
         int           code = src[k];
+        if(code >= code_lim)
+            continue;
         const double* b    = buf + 3 * (size_t)code;
         g0 += b[0];
         g1 += b[1];
@@ -617,9 +623,9 @@ void launch_init_diag(const SimDev& d, hipStream_t s)
     SG_LAUNCH_CHECK();
 }
 
-void launch_fem(const SimDev& d, bool hessian, hipStream_t s)
+void launch_fem(const SimDev& d, bool hessian, hipStream_t s, bool cloth)
 {
-    if(d.nt == 0 && d.cloth.nf == 0 && d.cloth.nh == 0)
+    if(d.nt == 0 && (!cloth || (d.cloth.nf == 0 && d.cloth.nh == 0)))
         return;
     if(d.nt)
     {
@@ -628,9 +634,11 @@ void launch_fem(const SimDev& d, bool hessian, hipStream_t s)
                            hessian ? 1 : 0);
         SG_LAUNCH_CHECK();
     }
-    launch_cloth(d, hessian, s);
+    if(cloth)
+        launch_cloth(d, hessian, s);
     hipLaunchKernelGGL(k_fem_gather_grad, dim3(grid_for(d.nv, 256)), dim3(256), 0, s, d.nv,
-                       d.gv_ptr, d.gv_src, d.fem_buf, d.gshape);
+                       d.gv_ptr, d.gv_src, d.fem_buf, d.gshape,
+                       cloth ? INT_MAX : (int)(d.cloth.off_tri / 3));
     SG_LAUNCH_CHECK();
     if(hessian && d.n_up > 0)
     {

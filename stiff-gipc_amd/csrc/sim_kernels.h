@@ -95,7 +95,8 @@ struct EnergyLayout
 
 // fem.hip
 void launch_init_diag(const SimDev& d, hipStream_t s);
-void launch_fem(const SimDev& d, bool hessian, hipStream_t s);
+// cloth = false leaves the cloth records out of the gathered gradient
+void launch_fem(const SimDev& d, bool hessian, hipStream_t s, bool cloth = true);
 void launch_ground_detect(const SimDev& d, hipStream_t s);
 void launch_ground(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);
 void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontact, hipStream_t s);

@@ -40,6 +40,9 @@ class SceneDesc(C.Structure):
         ("cloth_thickness", C.c_double),
         ("cloth_density", C.c_double),
         ("cloth_strain_rate", C.c_double),
+        ("n_abd", C.c_int32),
+        ("abd_body", C.POINTER(C.c_int32)),
+        ("abd_kappa", C.c_double),
         ("n_faces", C.c_int32),
         ("faces", C.POINTER(C.c_uint32)),
         ("n_edges", C.c_int32),
@@ -99,6 +102,8 @@ class Scene:
     cloth_thickness: float = 0.0
     cloth_density: float = 0.0
     cloth_strain_rate: float = 0.0
+    abd_body: np.ndarray | None = None  # (N,) int32 affine body per vertex, -1 = FEM
+    abd_kappa: float = 1e8
     dt: float = 0.01
     dhat: float = 1e-3
     gravity: tuple = (0.0, -9.8, 0.0)
@@ -150,6 +155,7 @@ class Scene:
             np.ascontiguousarray(self.edges, np.uint32),
             np.ascontiguousarray(self.surf_verts, np.uint32),
             None if self.tris is None else np.ascontiguousarray(self.tris, np.uint32),
+            None if self.abd_body is None else np.ascontiguousarray(self.abd_body, np.int32),
         ]
         k = self._keep
         d = SceneDesc()
@@ -173,6 +179,9 @@ class Scene:
         d.cloth_thickness = self.cloth_thickness
         d.cloth_density = self.cloth_density
         d.cloth_strain_rate = self.cloth_strain_rate
+        d.n_abd = 0 if self.abd_body is None else int(self.abd_body.max() + 1)
+        d.abd_body = _ptr(k[14], C.c_int32)
+        d.abd_kappa = self.abd_kappa
         d.n_faces = int(self.faces.shape[0])
         d.faces = _ptr(k[10], C.c_uint32)
         d.n_edges = int(self.edges.shape[0])
@@ -310,15 +319,21 @@ def random_rotation(rng):
 
 
 def assemble_bodies(bodies, cloths=(), **kw):
-    """Concatenate bodies = [(x, tets, part_id, E, nu, rho, btype)] and
+    """Concatenate bodies = [(x, tets, part_id, E, nu, rho, btype[, abd])] and
     cloths = [(x, tris, part_id, btype)] into a Scene with per-body ids and
-    offset part ids.  btype is a scalar or a per-vertex array.  Cloth
-    vertices get body id -1 (self contact allowed); the collision surface
-    is the tet boundary plus every cloth triangle, edge and vertex."""
+    offset part ids.  btype is a scalar or a per-vertex array.  Bodies with
+    abd = True become affine bodies (their tets supply mass and volume only)
+    and are placed after every FEM and cloth vertex.  Cloth vertices get body
+    id -1 (self contact allowed); the collision surface is the tet boundary
+    plus every cloth triangle, edge and vertex."""
     xs, ts, ps, Es, nus, rhos, bts, bids, trs = [], [], [], [], [], [], [], [], []
     voff = 0
     poff = 0
-    for b, (x, t, p, E, nu, rho, bt) in enumerate(bodies):
+    fem = [(i, b) for i, b in enumerate(bodies) if not (len(b) > 7 and b[7])]
+    abd = [(i, b) for i, b in enumerate(bodies) if len(b) > 7 and b[7]]
+    abd_of = []
+    for b, (x, t, p, E, nu, rho, bt) in [(i, bb[:7]) for i, bb in fem]:
+        abd_of.append(np.full(x.shape[0], -1, np.int32))
         xs.append(x)
         ts.append(t.astype(np.int64) + voff)
         ps.append(p + poff)
@@ -335,6 +350,19 @@ def assemble_bodies(bodies, cloths=(), **kw):
         ps.append(p + poff)
         bts.append(np.broadcast_to(np.asarray(bt, np.int32), (x.shape[0],)).copy())
         bids.append(np.full(x.shape[0], -1, np.int32))
+        abd_of.append(np.full(x.shape[0], -1, np.int32))
+        voff += x.shape[0]
+        poff = int(ps[-1].max()) + 1
+    for k, (b, (x, t, p, E, nu, rho, bt)) in enumerate([(i, bb[:7]) for i, bb in abd]):
+        xs.append(x)
+        ts.append(t.astype(np.int64) + voff)
+        ps.append(p + poff)
+        Es.append(np.full(t.shape[0], E))
+        nus.append(np.full(t.shape[0], nu))
+        rhos.append(np.full(t.shape[0], rho))
+        bts.append(np.broadcast_to(np.asarray(bt, np.int32), (x.shape[0],)).copy())
+        bids.append(np.full(x.shape[0], b, np.int32))
+        abd_of.append(np.full(x.shape[0], k, np.int32))
         voff += x.shape[0]
         poff = int(ps[-1].max()) + 1
     tets = np.concatenate(ts).astype(np.uint32) if ts else np.zeros((0, 4), np.uint32)
@@ -352,7 +380,7 @@ def assemble_bodies(bodies, cloths=(), **kw):
         x=np.concatenate(xs), tets=tets, E=cat(Es), nu=cat(nus),
         rho=cat(rhos), part_id=np.concatenate(ps).astype(np.int32),
         body_id=np.concatenate(bids), btype=np.concatenate(bts), faces=faces, edges=edges,
-        surf_verts=sv, tris=tris, **kw)
+        surf_verts=sv, tris=tris, abd_body=np.concatenate(abd_of) if abd else None, **kw)
 
 
 # --------------------------------------------------------------------------
@@ -448,5 +476,30 @@ def cloth_scene(n=16, size=0.3, height=0.15, n_boxes=1, box_size=0.1, box_cells=
                     cloth_density=density, cloth_strain_rate=strain_rate)
     defaults.update(kw)
     sc = assemble_bodies(bodies, cloths=[(xc, tc, pc, 0)], **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
+
+
+def abd_scene(n_abd=2, cells=2, size=0.1, gap=0.5e-3, fem_base=True, base_E=1e6, rot_deg=9.0,
+              abd_kappa=1e8, rho=1000.0, seed=0, **kw):
+    """A soft FEM box on the ground (fem_base) with n_abd affine boxes
+    stacked on it, each rotated by rot_deg about the vertical axis
+    (alternating) and 'gap' above its support: contacts between affine
+    bodies, between an affine body and the FEM box, and with the ground."""
+    x0, t0, p0 = hex_grid((cells,) * 3, size / cells)
+    x0 = x0 - x0.mean(0)
+    bodies = []
+    y = gap + size / 2
+    if fem_base:
+        bodies.append((x0 + np.array([0.0, y, 0.0]), t0, p0, base_E, 0.4, rho, 0))
+        y += size + gap
+    for k in range(n_abd):
+        a = np.deg2rad(rot_deg * (1 if k % 2 == 0 else -1))
+        R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+        bodies.append((x0 @ R.T + np.array([0.0, y, 0.0]), t0, p0, 1e9, 0.3, rho, 0, True))
+        y += size + gap
+    defaults = dict(dt=0.01, dhat=1e-3, name=f"abd{n_abd}", abd_kappa=abd_kappa)
+    defaults.update(kw)
+    sc = assemble_bodies(bodies, **defaults)
     sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
     return sc

@@ -1,0 +1,180 @@
+"""Pins of the oracle's affine-body terms (abd_system/details/abd_energy.inl,
+abd_jacobi_matrix.h, setup_abd_system_gradient_and_hessian.cu): the shape
+energy's closed form, finite differences of its gradient and Hessian, the
+Jacobian, the node-space gradient of a world against finite differences of
+its incremental potential, the body blocks against their definition, and
+rigid motion of affine boxes (free fall, spinning, resting on a soft box)."""
+import numpy as np
+import pytest
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def _q(p, A):
+    return np.concatenate([p, A[0], A[1], A[2]])
+
+
+def test_shape_energy_closed_form(oracle):
+    rng = np.random.default_rng(0)
+    R = _rot([1, 2, 3], 0.7)
+    e, g, _ = oracle.abd_shape(_q(rng.normal(size=3), R))
+    assert e == pytest.approx(0.0, abs=1e-28)
+    assert np.abs(g).max() < 1e-14
+    for _ in range(3):
+        A = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+        e, _, _ = oracle.abd_shape(_q(np.zeros(3), A))
+        assert e == pytest.approx(np.linalg.norm(A @ A.T - np.eye(3)) ** 2, rel=1e-13)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_shape_derivatives_fd(oracle, seed):
+    rng = np.random.default_rng(10 + seed)
+    q = _q(rng.normal(size=3), np.eye(3) + 0.3 * rng.normal(size=(3, 3)))
+    e, g, H = oracle.abd_shape(q)
+    h = 1e-6
+    gfd = np.zeros(9)
+    Hfd = np.zeros((9, 9))
+    for k in range(9):
+        d = np.zeros(12)
+        d[3 + k] = h
+        gfd[k] = (oracle.abd_shape(q + d)[0] - oracle.abd_shape(q - d)[0]) / (2 * h)
+        Hfd[:, k] = (oracle.abd_shape(q + d)[1] - oracle.abd_shape(q - d)[1]) / (2 * h)
+    np.testing.assert_allclose(g, gfd, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(H, Hfd, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(H, H.T, atol=1e-13)
+
+
+def test_jacobian(oracle):
+    rng = np.random.default_rng(3)
+    xb = rng.normal(size=3)
+    J = oracle.abd_jmat(xb)
+    p, A = rng.normal(size=3), rng.normal(size=(3, 3))
+    np.testing.assert_allclose(J @ _q(p, A), p + A @ xb, rtol=1e-14)
+
+
+def _falling_box(rot=0.0, omega=None, **kw):
+    from sgipc import scene as S
+    x0, t0, p0 = S.hex_grid((2, 2, 2), 0.05)
+    x0 = x0 - x0.mean(0)
+    x0 = x0 @ _rot([0, 1, 0], rot).T + np.array([0.0, 1.0, 0.0])
+    sc = S.assemble_bodies([(x0, t0, p0, 1e9, 0.3, 1000.0, 0, True)], dt=0.01, dhat=1e-3,
+                           use_ground=False, **kw)
+    if omega is not None:
+        c = x0.mean(0)
+        sc.v = np.cross(omega, x0 - c)
+    sc.abs_xdelta_tol = 1e-7
+    return sc
+
+
+def test_free_fall_is_rigid(oracle):
+    sc = _falling_box(rot=0.3)
+    w = oracle.World(sc)
+    assert w.nodes == 4
+    x0 = w.positions().copy()
+    for n in range(1, 5):
+        w.time_step()
+        drop = -9.8 * 0.01 ** 2 * n * (n + 1) / 2
+        np.testing.assert_allclose(w.positions(), x0 + np.array([0.0, drop, 0.0]), atol=1e-10)
+    assert w.energy()[7] == pytest.approx(0.0, abs=1e-20)
+
+
+def test_spinning_box_stays_rigid(oracle):
+    """an affine box spinning at 5 rad/s: the shape energy (kappa = 1e8) keeps
+    A a rotation, so every edge length is preserved to 1e-5"""
+    sc = _falling_box(omega=np.array([0.0, 5.0, 2.0]), gravity=(0.0, 0.0, 0.0))
+    w = oracle.World(sc)
+    x0 = w.positions().copy()
+    d0 = np.linalg.norm(x0[:, None] - x0[None], axis=2)
+    for _ in range(10):
+        w.time_step()
+    x = w.positions()
+    d = np.linalg.norm(x[:, None] - x[None], axis=2)
+    assert np.abs(d - d0).max() / d0.max() < 1e-5
+    assert np.abs(x - x0).max() > 0.01          # it did turn
+
+
+def test_node_gradient_is_energy_derivative(oracle):
+    """the node gradient is d(incremental potential)/dq for a body in contact
+    (kinetic M (q - q~), shape, barrier through J^T) and d/dx for FEM nodes"""
+    from sgipc import scene as S
+    sc = S.abd_scene(n_abd=2, gap=4e-4)
+    sc.kappa = 1e4
+    w = oracle.World(sc)
+    x = w.positions().copy()
+    # move inside the bodies' affine space: perturb q of body 0 (rotation + stretch)
+    abd = sc.abd_body
+    xb = x[abd == 0] - x[abd == 0].mean(0)
+    A = np.eye(3) + np.array([[0.01, 0.003, 0], [0, -0.004, 0.002], [0.001, 0, 0.006]])
+    x[abd == 0] = x[abd == 0].mean(0) + 1e-4 + xb @ A.T
+    w.set_positions(x)
+    w.kappa = 1e4
+    g, _ = w.assemble()
+    nfem = int((abd < 0).sum())
+    gb = g[3 * nfem:3 * nfem + 12]
+    xbar0 = sc.x[abd == 0] - sc.x[abd == 0].mean(0)   # body frame fixed at creation
+    J = [oracle.abd_jmat(v) for v in xbar0]
+    h = 1e-7
+    for k in (0, 1, 4, 8, 11):
+        def E(s):
+            y = x.copy()
+            dq = np.zeros(12)
+            dq[k] = s
+            y[abd == 0] = x[abd == 0] + np.array([Jm @ dq for Jm in J])
+            w.set_positions(y)
+            return w.energy()[0]
+        fd = (E(h) - E(-h)) / (2 * h)
+        assert gb[k] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(gb).max())
+    w.set_positions(x)
+
+
+def test_body_block_is_mass_plus_projected_shape(oracle):
+    sc = _falling_box(rot=0.2)
+    w = oracle.World(sc)
+    x = w.positions()
+    c = x.mean(0)   # uniform masses of a regular grid: the mass centre is the mean
+    A = np.eye(3) + np.array([[0.02, 0.01, 0], [0, -0.03, 0.01], [0.0, 0, 0.04]])
+    w.set_positions(c + (x - c) @ A.T)
+    _, H = w.assemble()
+    Hd = H.dense()
+    # M = sum m J^T J with lumped masses (a regular 2x2x2 grid of equal tets)
+    from sgipc import scene as S
+    g, _ = w.assemble()
+    xbar = x - c
+    vol = 0.05 ** 3 / 6
+    m = np.zeros(len(x))
+    for t in sc.tets:
+        m[t] += vol * 1000.0 / 4
+    M = sum(mi * oracle.abd_jmat(xb).T @ oracle.abd_jmat(xb) for mi, xb in zip(m, xbar))
+    q = np.concatenate([c, A[0], A[1], A[2]])
+    _, _, Hs = oracle.abd_shape(q)
+    Hs = Hs * 1e8 * 0.1 ** 3 * 0.01 ** 2
+    wv, V = np.linalg.eigh(Hs)
+    Hp = (V * np.maximum(wv, 0)) @ V.T
+    want = M.copy()
+    want[3:, 3:] += Hp
+    np.testing.assert_allclose(Hd, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_abd_stack_on_soft_box_rests(oracle):
+    """two affine boxes stacked on a soft FEM box settle: no interpenetration
+    (the bodies keep their gaps positive) and the affine boxes stay rigid"""
+    from sgipc import scene as S
+    sc = S.abd_scene(n_abd=2)
+    w = oracle.World(sc)
+    x0 = w.positions().copy()
+    for _ in range(12):
+        w.time_step()
+    x = w.positions()
+    abd = sc.abd_body
+    for b in (0, 1):
+        d0 = np.linalg.norm(x0[abd == b][:, None] - x0[abd == b][None], axis=2)
+        d = np.linalg.norm(x[abd == b][:, None] - x[abd == b][None], axis=2)
+        assert np.abs(d - d0).max() < 1e-6
+    fem = abd < 0
+    assert x[abd == 0, 1].min() > x[fem, 1].max() - 1e-3   # resting on the soft box
+    assert x[abd == 1, 1].min() > x[abd == 0, 1].max()
+    assert x[:, 1].min() > 0.0
