@@ -33,7 +33,7 @@ def log(*a):
 
 def algorithmic_bytes(name, sz, nparts):
     """Bytes one launch must move at minimum (each operand read or written once)."""
-    nv, nt, nnzb, ncoo = sz["n_verts"], sz["n_tets"], sz["nnzb"], sz["coo_blocks"]
+    nv, nt, nnzb, ncoo = sz["n_nodes"], sz["n_tets"], sz["nnzb"], sz["coo_blocks"]
     if name == "spmv":
         # static BSR: 72 B block + 4 B column; rowptr; x read, y write (24 B each);
         # COO tail: block + 2 ids + x gather + y read-modify-write

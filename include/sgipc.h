@@ -136,15 +136,17 @@ int sgipc_contact_query(sgipc_t ctx, int32_t* pairs, int32_t cap,
                         int32_t* n_ground);
 
 /* Gradient and Hessian at the current positions (computeGradientAndHessian,
- * GIPC.cu:10012-10228).  grad[3N] is the assembled right-hand side (zero on
- * boundary vertices).  The Hessian is returned as a full BSR; pass cap = 0
- * to query nnzb only. */
+ * GIPC.cu:10012-10228) on the system nodes (FEM vertices, then 4 nodes
+ * (p, a1, a2, a3) per affine body; n_nodes from sgipc_problem_sizes).
+ * grad[3 n_nodes] is the assembled right-hand side (zero on boundary
+ * vertices and fixed bodies); rowptr[n_nodes + 1].  The Hessian is returned
+ * as a full BSR; pass cap = 0 to query nnzb only. */
 int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
                    int32_t* cols, double* vals, int32_t cap);
 
 /* Incremental potential at the current positions, FP64 (computeEnergy,
  * GIPC.cu:10392-10451).  terms[8] = {total, kinetic, elastic (tets),
- * barrier, ground, friction, cloth (membrane + bending), reserved}. */
+ * barrier, ground, friction, cloth (membrane + bending), affine shape}. */
 int sgipc_energy(sgipc_t ctx, double* terms);
 
 typedef struct sgipc_iter_stats {
@@ -199,8 +201,9 @@ int sgipc_advance(sgipc_t ctx, int32_t n, sgipc_advance_stats* stats);
 int sgipc_profile_enable(sgipc_t ctx, int32_t every);
 int sgipc_profile_query(sgipc_t ctx, const char* kernel, int32_t* launches,
                         double* total_ms);
-/* problem sizes for algorithmic byte counts: out[8] = {n_verts, n_tets,
- * nnzb static, coo blocks, mas banks, mas levels, self pairs, ground pairs} */
+/* problem sizes for algorithmic byte counts: out[8] = {n_nodes (system block
+ * rows: FEM vertices + 4 per affine body), n_tets (elastic), nnzb static,
+ * coo blocks, mas banks, mas levels, self pairs, ground pairs} */
 int sgipc_problem_sizes(sgipc_t ctx, int64_t* out);
 
 #ifdef __cplusplus

@@ -61,11 +61,15 @@ class Context
     void   begin_step();                   // IPC_Solver prologue (kappa)
     void   end_step();                     // updateVelocities + computeXTilta
     double read_d(const double* p);
+    void   step_forward(double alpha);
+    void   body_fit(int b, const double* f, double* q) const;
+    void   body_apply(int b, const double* q, double* f) const;
     unsigned read_u(const unsigned* p);
 
     hipStream_t s_ = nullptr;
     sgipc_scene_desc desc_{};
     int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0, ntri_ = 0, nh_ = 0;
+    int nabd_ = 0, nfem_ = 0, nn_ = 0;  // affine bodies, FEM vertices, system nodes
     double dHat_ = 0, kappa_ = 0, bbox_diag2_ = 0, scene_diag_ = 0, mean_mass_ = 0;
     int    newton_k_ = 0;
     bool   in_step_  = false;
@@ -79,6 +83,10 @@ class Context
     DBuf<double>   fem_buf_;                                            // element records
     DBuf<uint32_t> tri_, hinge_;                                        // cloth
     DBuf<double>   tri_dminv_, tri_area_, hinge_th0_, hinge_len0_;
+    DBuf<int>      bslots_, node_of_, bstart_, bfix_;                   // affine bodies
+    DBuf<double>   xbar_, bM_, bV_, bq_, bqs_, gbody_, dy_;
+    std::vector<int>    h_bstart_, h_bfix_;
+    std::vector<double> h_xbar_, h_mass_, h_bMinv_, h_bG_;
     DBuf<int4>     fr_code_;                                            // lagged friction
     DBuf<double>   fr_lam_, fr_coord_, fr_basis_, fg_lam_;
     DBuf<uint32_t> fg_ids_;

@@ -23,6 +23,46 @@
 namespace sgipc
 {
 
+// Gauss-Jordan inverse with partial pivoting (row-major n x n); false when
+// singular
+static bool invert_dense(int n, const double* A, double* Ai)
+{
+    std::vector<double> a(A, A + n * n);
+    for(int i = 0; i < n * n; i++)
+        Ai[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+    for(int c = 0; c < n; c++)
+    {
+        int p = c;
+        for(int r = c + 1; r < n; r++)
+            if(std::fabs(a[r * n + c]) > std::fabs(a[p * n + c]))
+                p = r;
+        if(a[p * n + c] == 0.0)
+            return false;
+        for(int k = 0; k < n; k++)
+        {
+            std::swap(a[c * n + k], a[p * n + k]);
+            std::swap(Ai[c * n + k], Ai[p * n + k]);
+        }
+        double inv = 1.0 / a[c * n + c];
+        for(int k = 0; k < n; k++)
+        {
+            a[c * n + k] *= inv;
+            Ai[c * n + k] *= inv;
+        }
+        for(int r = 0; r < n; r++)
+            if(r != c && a[r * n + c] != 0.0)
+            {
+                double f = a[r * n + c];
+                for(int k = 0; k < n; k++)
+                {
+                    a[r * n + k] -= f * a[c * n + k];
+                    Ai[r * n + k] -= f * Ai[c * n + k];
+                }
+            }
+    }
+    return true;
+}
+
 static double now_ms(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0;
@@ -98,6 +138,152 @@ Context::Context(const sgipc_scene_desc& d)
         mu[e]    = E / (2 * (1 + nu));
         lam[e]   = E * nu / ((1 + nu) * (1 - 2 * nu));
     }
+    // ---- affine bodies: vertices [nfem, nv) grouped by body; their tets
+    // only supply mass and volume; q = (c, I) with c the mass centre, xbar =
+    // x - c, dyadic mass M = sum m J^T J (abd_system.cu _setup_abd_state,
+    // _setup_J, _setup_abd_dyadic_mass), q' and affine gravity G fitted by
+    // M^-1 sum m J^T f (_setup_abd_gravity)
+    std::vector<double>   hx(d.x, d.x + 3 * (size_t)nv_), hv(3 * (size_t)nv_, 0.0);
+    if(d.v)
+        hv.assign(d.v, d.v + 3 * (size_t)nv_);
+    std::vector<int> abd_of(nv_, -1);
+    nabd_ = (d.abd_body && d.n_abd > 0) ? d.n_abd : 0;
+    if(nabd_)
+        abd_of.assign(d.abd_body, d.abd_body + nv_);
+    nfem_ = nv_;
+    for(int v = 0; v < nv_; v++)
+        if(abd_of[v] >= 0)
+        {
+            nfem_ = v;
+            break;
+        }
+    for(int v = nfem_; v < nv_; v++)
+        SG_CHECK(abd_of[v] >= 0 && abd_of[v] < nabd_
+                     && (v == nfem_ ? abd_of[v] == 0 : (abd_of[v] - abd_of[v - 1] == 0
+                                                       || abd_of[v] - abd_of[v - 1] == 1)),
+                 SGIPC_EINVAL, "affine-body vertices must follow all others, grouped by body");
+    SG_CHECK(nabd_ == 0 || abd_of[nv_ - 1] == nabd_ - 1, SGIPC_EINVAL,
+             "affine body without vertices");
+    nn_ = nfem_ + 4 * nabd_;
+    std::vector<uint32_t> etets;
+    std::vector<double>   bvol(nabd_, 0.0);
+    {
+        std::vector<double> d2, v2, m2, l2;
+        for(int e = 0; e < nt_; e++)
+        {
+            const uint32_t* t = d.tets + 4 * (size_t)e;
+            int na = 0;
+            for(int k = 0; k < 4; k++)
+                na += abd_of[t[k]] >= 0;
+            SG_CHECK(na == 0 || (na == 4 && abd_of[t[1]] == abd_of[t[0]] && abd_of[t[2]] == abd_of[t[0]]
+                                 && abd_of[t[3]] == abd_of[t[0]]),
+                     SGIPC_EINVAL, "a tet mixes affine-body and FEM vertices");
+            if(na)
+            {
+                bvol[abd_of[t[0]]] += vol[e];
+                continue;
+            }
+            etets.insert(etets.end(), t, t + 4);
+            v2.push_back(vol[e]);
+            m2.push_back(mu[e]);
+            l2.push_back(lam[e]);
+        }
+        int ne2 = (int)v2.size();
+        d2.resize((size_t)9 * ne2);
+        for(int e = 0, j = 0; e < nt_; e++)
+        {
+            const uint32_t* t = d.tets + 4 * (size_t)e;
+            if(abd_of[t[0]] >= 0)
+                continue;
+            for(int k = 0; k < 9; k++)
+                d2[(size_t)k * ne2 + j] = dminv[(size_t)k * nt_ + e];
+            j++;
+        }
+        dminv.swap(d2);
+        vol.swap(v2);
+        mu.swap(m2);
+        lam.swap(l2);
+        nt_ = ne2;
+    }
+    std::vector<int>    bstart(nabd_ + 1, nfem_), bfix(nabd_, 0);
+    std::vector<double> xbar(3 * (size_t)nv_, 0.0), bM(144 * (size_t)nabd_, 0.0),
+        bq(12 * (size_t)nabd_, 0.0), bqv(12 * (size_t)nabd_, 0.0), bG(12 * (size_t)nabd_, 0.0);
+    h_bMinv_.assign(144 * (size_t)nabd_, 0.0);
+    for(int v = nfem_; v < nv_; v++)
+        bstart[abd_of[v] + 1] = v + 1;
+    for(int b = 0; b < nabd_; b++)
+    {
+        double ms = 0.0, c[3] = {0, 0, 0};
+        for(int v = bstart[b]; v < bstart[b + 1]; v++)
+        {
+            ms += mass[v];
+            for(int k = 0; k < 3; k++)
+                c[k] += mass[v] * hx[3 * v + k];
+            SG_CHECK((bt[v] != 0) == (bt[bstart[b]] != 0), SGIPC_EINVAL,
+                     "an affine body is partly fixed");
+        }
+        SG_CHECK(ms > 0.0, SGIPC_EINVAL, "affine body without mass (no tets)");
+        for(int k = 0; k < 3; k++)
+            c[k] /= ms;
+        bfix[b]   = bt[bstart[b]] != 0;
+        double* M = bM.data() + 144 * (size_t)b;
+        for(int v = bstart[b]; v < bstart[b + 1]; v++)
+        {
+            double* xb = xbar.data() + 3 * (size_t)v;
+            for(int k = 0; k < 3; k++)
+                xb[k] = hx[3 * v + k] - c[k];
+            // J^T J = [[I, X], [X^T, blockdiag(xb xb^T)]] with X_k = e_k xb^T
+            for(int k = 0; k < 3; k++)
+            {
+                M[12 * k + k] += mass[v];
+                for(int l = 0; l < 3; l++)
+                {
+                    M[12 * k + 3 + 3 * k + l] += mass[v] * xb[l];
+                    M[12 * (3 + 3 * k + l) + k] += mass[v] * xb[l];
+                    for(int l2 = 0; l2 < 3; l2++)
+                        M[12 * (3 + 3 * k + l) + 3 + 3 * k + l2] += mass[v] * xb[l] * xb[l2];
+                }
+            }
+        }
+        double* Mi = h_bMinv_.data() + 144 * (size_t)b;
+        SG_CHECK(invert_dense(12, M, Mi), SGIPC_EINVAL, "singular affine-body mass (flat body)");
+        for(int k = 0; k < 3; k++)
+        {
+            bq[12 * b + k]         = c[k];
+            bq[12 * b + 3 + 4 * k] = 1.0;
+        }
+    }
+    h_bstart_ = bstart;
+    h_xbar_   = xbar;
+    h_mass_   = mass;
+    h_bfix_   = bfix;
+    for(int b = 0; b < nabd_; b++)
+        if(!bfix[b])
+        {
+            std::vector<double> gv(3 * (size_t)nv_, 0.0);
+            for(int v = bstart[b]; v < bstart[b + 1]; v++)
+                for(int k = 0; k < 3; k++)
+                    gv[3 * v + k] = d.gravity[k];
+            body_fit(b, gv.data(), bG.data() + 12 * b);
+            body_fit(b, hv.data(), bqv.data() + 12 * b);
+        }
+    h_bG_ = bG;
+    // ABD vertices follow their body: x = J q, v = J q', x~ = J (q + dt q' + dt^2 G)
+    std::vector<double> hxt(3 * (size_t)nv_);
+    for(int i = 0; i < nfem_; i++)
+        for(int c = 0; c < 3; c++)
+            hxt[3 * i + c] = hx[3 * i + c]
+                             + (hv[3 * i + c] * d.dt + (bt[i] == 0 ? d.gravity[c] * (d.dt * d.dt) : 0.0));
+    for(int b = 0; b < nabd_; b++)
+    {
+        double qt[12];
+        for(int k = 0; k < 12; k++)
+            qt[k] = bq[12 * b + k] + (bqv[12 * b + k] * d.dt + bG[12 * b + k] * (d.dt * d.dt));
+        body_apply(b, bq.data() + 12 * b, hx.data());
+        body_apply(b, bqv.data() + 12 * b, hv.data());
+        body_apply(b, qt, hxt.data());
+    }
+
     // ---- cloth constants: rest-frame Dm^-1 (2x2), area, lumped mass
     // area * thickness * density / 3 per vertex, interior-edge hinges with
     // their rest angle and length (femEnergy.cu:262-309, load_mesh.cpp:320-379;
@@ -105,7 +291,8 @@ Context::Context(const sgipc_scene_desc& d)
     ntri_ = d.n_tris;
     SG_CHECK(ntri_ == 0 || d.tris, SGIPC_EINVAL, "cloth triangles missing");
     for(int e = 0; e < 3 * ntri_; e++)
-        SG_CHECK(d.tris[e] < (uint32_t)nv_, SGIPC_EINVAL, "cloth vertex index out of range");
+        SG_CHECK(d.tris[e] < (uint32_t)nfem_, SGIPC_EINVAL,
+                 "cloth vertex index out of range (or on an affine body)");
     std::vector<double>   tdm((size_t)4 * ntri_), tarea(ntri_);
     std::vector<uint32_t> hinges;
     std::vector<double>   th0, len0;
@@ -195,8 +382,8 @@ Context::Context(const sgipc_scene_desc& d)
 
     // ---- static BSR pattern: element adjacency (tets, cloth triangles,
     // hinges) + every diagonal
-    std::vector<std::vector<int>> adj(nv_);
-    for(int v = 0; v < nv_; v++)
+    std::vector<std::vector<int>> adj(nn_);
+    for(int v = 0; v < nn_; v++)
         adj[v].push_back(v);
     auto link = [&](const uint32_t* ids, int m) {
         for(int a = 0; a < m; a++)
@@ -205,23 +392,28 @@ Context::Context(const sgipc_scene_desc& d)
                     adj[ids[a]].push_back((int)ids[b]);
     };
     for(int e = 0; e < nt_; e++)
-        link(d.tets + 4 * (size_t)e, 4);
+        link(etets.data() + 4 * (size_t)e, 4);
+    for(int b = 0; b < nabd_; b++)
+        for(int m = 0; m < 4; m++)
+            for(int n = 0; n < 4; n++)
+                if(m != n)
+                    adj[nfem_ + 4 * b + m].push_back(nfem_ + 4 * b + n);
     for(int f = 0; f < ntri_; f++)
         link(d.tris + 3 * (size_t)f, 3);
     for(int h = 0; h < nh_; h++)
         link(hinges.data() + 4 * (size_t)h, 4);
-    std::vector<int> rp(nv_ + 1, 0), cl;
-    for(int v = 0; v < nv_; v++)
+    std::vector<int> rp(nn_ + 1, 0), cl;
+    for(int v = 0; v < nn_; v++)
     {
         auto& a = adj[v];
         std::sort(a.begin(), a.end());
         a.erase(std::unique(a.begin(), a.end()), a.end());
         rp[v + 1] = rp[v] + (int)a.size();
     }
-    cl.reserve(rp[nv_]);
-    for(int v = 0; v < nv_; v++)
+    cl.reserve(rp[nn_]);
+    for(int v = 0; v < nn_; v++)
         cl.insert(cl.end(), adj[v].begin(), adj[v].end());
-    A_.set_pattern(nv_, rp, cl, s_);
+    A_.set_pattern(nn_, rp, cl, s_);
     auto slot_of = [&](int r, int c) {
         auto beg = cl.begin() + rp[r], end = cl.begin() + rp[r + 1];
         return (int)(std::lower_bound(beg, end, c) - cl.begin());
@@ -258,7 +450,7 @@ Context::Context(const sgipc_scene_desc& d)
                 vkey.push_back({(int)ids[a], (int)((base + 9 * (size_t)npairs + 3 * (size_t)a) / 3)});
         };
         for(int e = 0; e < nt_; e++)
-            add_elem(d.tets + 4 * (size_t)e, 4, (size_t)kFemStride * e, 10, kPairA, kPairB);
+            add_elem(etets.data() + 4 * (size_t)e, 4, (size_t)kFemStride * e, 10, kPairA, kPairB);
         for(int f = 0; f < ntri_; f++)
             add_elem(d.tris + 3 * (size_t)f, 3, off_tri + (size_t)kTriStride * f, 6, kTriPairA,
                      kTriPairB);
@@ -280,8 +472,8 @@ Context::Context(const sgipc_scene_desc& d)
         }
         ptr.push_back((int)src.size());
         // transpose slot of each upper block
-        std::vector<int> rowof(rp[nv_]);
-        for(int v = 0; v < nv_; v++)
+        std::vector<int> rowof(rp[nn_]);
+        for(int v = 0; v < nn_; v++)
             for(int k = rp[v]; k < rp[v + 1]; k++)
                 rowof[k] = v;
         lo.resize(up.size());
@@ -307,6 +499,33 @@ Context::Context(const sgipc_scene_desc& d)
         gv_src_.upload(vsrc, s_);
         n_up_ = (int)up.size();
         fem_buf_.resize(buf_len > 0 ? buf_len : 1);
+    }
+    {
+        std::vector<int> bslots(16 * (size_t)nabd_);
+        for(int b = 0; b < nabd_; b++)
+            for(int m = 0; m < 4; m++)
+                for(int n = 0; n < 4; n++)
+                    bslots[16 * (size_t)b + 4 * m + n] = slot_of(nfem_ + 4 * b + m, nfem_ + 4 * b + n);
+        std::vector<int> node_of(nv_);
+        for(int v = 0; v < nv_; v++)
+            node_of[v] = v < nfem_ ? v : nfem_ + 4 * abd_of[v];
+        bslots_.upload(bslots, s_);
+        node_of_.upload(node_of, s_);
+        xbar_.upload(xbar, s_);
+        bstart_.upload(bstart, s_);
+        bfix_.upload(bfix, s_);
+        bM_.upload(bM, s_);
+        bV_.upload(bvol, s_);
+        bq_.upload(bq, s_);
+        bqs_.upload(bq, s_);
+        gbody_.resize(12 * (size_t)std::max(nabd_, 1));
+        gbody_.zero(s_);
+        AbdDev& a = dev_.abd;
+        a.nb = nabd_, a.nfem = nfem_;
+        a.bstart = bstart_.data(), a.fixed = bfix_.data(), a.M = bM_.data(), a.vol = bV_.data();
+        a.q = bq_.data(), a.qs = bqs_.data(), a.gbody = gbody_.data(), a.slots = bslots_.data();
+        a.kappa = d.abd_kappa;
+        dev_.nfem = nfem_, dev_.node_of = node_of_.data(), dev_.xbar = xbar_.data();
     }
     tri_.upload(d.tris, 3 * (size_t)ntri_, s_);
     tri_dminv_.upload(tdm, s_);
@@ -365,26 +584,23 @@ Context::Context(const sgipc_scene_desc& d)
 
     // ---- device state
     size_t n3 = 3 * (size_t)nv_;
-    x_.upload(d.x, n3, s_);
+    x_.upload(hx, s_);
     xr_.upload(xr, n3, s_);
-    xprev_.upload(d.x, n3, s_);
-    std::vector<double> v0(n3, 0.0);
-    if(d.v)
-        v0.assign(d.v, d.v + n3);
-    v_.upload(v0, s_);
-    std::vector<double> xt(n3);
-    for(int i = 0; i < nv_; i++)
-        for(int c = 0; c < 3; c++)
-            xt[3 * i + c] = d.x[3 * i + c]
-                            + (v0[3 * i + c] * d.dt + (bt[i] == 0 ? d.gravity[c] * (d.dt * d.dt) : 0.0));
-    xt_.upload(xt, s_);
+    xprev_.upload(hx, s_);
+    v_.upload(hv, s_);
+    xt_.upload(hxt, s_);
     mass_.upload(mass, s_);
     dx_.resize(n3);
     dx_.zero(s_);
     xsave_.resize(n3);
     gc_.resize(n3);
     gs_.resize(n3);
-    rhs_.resize(n3);
+    rhs_.resize(3 * (size_t)nn_);
+    if(nabd_)
+    {
+        dy_.resize(3 * (size_t)nn_);
+        dy_.zero(s_);
+    }
     scratch_.resize(16);
     dminv_.upload(dminv, s_);
     vol_.upload(vol, s_);
@@ -392,7 +608,7 @@ Context::Context(const sgipc_scene_desc& d)
     lam_.upload(lam, s_);
     btype_.upload(bt, s_);
     body_.upload(body, s_);
-    tets_.upload(d.tets, 4 * (size_t)nt_, s_);
+    tets_.upload(etets, s_);
     faces_.upload(d.faces, 3 * (size_t)nf_, s_);
     edges_.upload(d.edges, 2 * (size_t)ne_, s_);
     surf_.upload(d.surf_verts, ns_, s_);
@@ -407,7 +623,8 @@ Context::Context(const sgipc_scene_desc& d)
     cand_.resize(cand_cap_);
     overflow_.resize(4);
     overflow_.zero(s_);
-    coo_.reserve(6 * pair_cap_, nv_);
+    // affine-body contact blocks expand into up to 4x4 node blocks
+    coo_.reserve((nabd_ ? 48 : 6) * (size_t)pair_cap_, nn_);
 
     dev_.nv = nv_, dev_.nt = nt_, dev_.ns = ns_, dev_.nf = nf_, dev_.ne = ne_;
     dev_.dt = d.dt, dev_.dHat = dHat_, dev_.goff = d.ground_offset;
@@ -459,7 +676,18 @@ Context::Context(const sgipc_scene_desc& d)
     {
         MasConfig mc;
         mc.fp32 = d.mas_precision == SGIPC_MAS_FP32;
-        mas_.init(A_, d.part_id, mc, s_);
+        // MAS parts per node: FEM vertices keep theirs, each body is one part
+        std::vector<int> npart(nn_);
+        int              maxp = -1;
+        for(int v = 0; v < nfem_; v++)
+        {
+            npart[v] = d.part_id ? d.part_id[v] : v / 16;
+            maxp     = std::max(maxp, npart[v]);
+        }
+        for(int b = 0; b < nabd_; b++)
+            for(int k = 0; k < 4; k++)
+                npart[nfem_ + 4 * b + k] = maxp + 1 + b;
+        mas_.init(A_, npart.data(), mc, s_);
     }
     mas_.prof = &prof_;
     pcg_.prof = &prof_;
@@ -468,6 +696,43 @@ Context::Context(const sgipc_scene_desc& d)
     build_bvh();
     build_cp();
     SG_HIP(hipStreamSynchronize(s_));
+}
+
+// q = M^-1 sum m J^T f over body b's vertices (least-squares affine fit)
+void Context::body_fit(int b, const double* f, double* q) const
+{
+    double rhs[12] = {0};
+    for(int v = h_bstart_[b]; v < h_bstart_[b + 1]; v++)
+    {
+        const double* xb = h_xbar_.data() + 3 * (size_t)v;
+        for(int k = 0; k < 3; k++)
+        {
+            double mf = h_mass_[v] * f[3 * (size_t)v + k];
+            rhs[k] += mf;
+            for(int l = 0; l < 3; l++)
+                rhs[3 + 3 * k + l] += mf * xb[l];
+        }
+    }
+    const double* Mi = h_bMinv_.data() + 144 * (size_t)b;
+    for(int i = 0; i < 12; i++)
+    {
+        double a = 0.0;
+        for(int k = 0; k < 12; k++)
+            a += Mi[12 * i + k] * rhs[k];
+        q[i] = a;
+    }
+}
+
+// f = J q on body b's vertices
+void Context::body_apply(int b, const double* q, double* f) const
+{
+    for(int v = h_bstart_[b]; v < h_bstart_[b + 1]; v++)
+    {
+        const double* xb = h_xbar_.data() + 3 * (size_t)v;
+        for(int k = 0; k < 3; k++)
+            f[3 * (size_t)v + k] = q[k] + q[3 + 3 * k] * xb[0] + q[4 + 3 * k] * xb[1]
+                                   + q[5 + 3 * k] * xb[2];
+    }
 }
 
 Context::~Context()
@@ -503,6 +768,23 @@ void Context::check_overflow()
 
 void Context::set_positions(const double* x)
 {
+    if(nabd_)
+    {
+        // affine-body vertices stay affine: x = J q with q fitted to x
+        std::vector<double> hx(x, x + 3 * (size_t)nv_), q(12 * (size_t)nabd_);
+        for(int b = 0; b < nabd_; b++)
+        {
+            body_fit(b, hx.data(), q.data() + 12 * b);
+            body_apply(b, q.data() + 12 * b, hx.data());
+        }
+        bq_.upload(q, s_);
+        x_.upload(hx, s_);
+        SG_HIP(hipStreamSynchronize(s_));
+        build_bvh();
+        build_cp();
+        SG_HIP(hipStreamSynchronize(s_));
+        return;
+    }
     x_.upload(x, 3 * (size_t)nv_, s_);
     build_bvh();
     build_cp();
@@ -511,11 +793,29 @@ void Context::set_positions(const double* x)
 
 void Context::set_state(const double* x, const double* v, const double* xt, double kappa)
 {
-    size_t n3 = 3 * (size_t)nv_;
-    x_.upload(x, n3, s_);
-    xprev_.upload(x, n3, s_);
-    v_.upload(v, n3, s_);
-    xt_.upload(xt, n3, s_);
+    size_t              n3 = 3 * (size_t)nv_;
+    std::vector<double> hx(x, x + n3), hv(v, v + n3), hxt(xt, xt + n3);
+    if(nabd_)
+    {
+        std::vector<double> q(12 * (size_t)nabd_);
+        for(int b = 0; b < nabd_; b++)
+        {
+            double qv[12], qt[12];
+            body_fit(b, hx.data(), q.data() + 12 * b);
+            body_fit(b, hv.data(), qv);
+            body_fit(b, hxt.data(), qt);
+            body_apply(b, q.data() + 12 * b, hx.data());
+            body_apply(b, qv, hv.data());
+            body_apply(b, qt, hxt.data());
+        }
+        bq_.upload(q, s_);
+        bqs_.upload(q, s_);
+        dy_.zero(s_);
+    }
+    x_.upload(hx, s_);
+    xprev_.upload(hx, s_);
+    v_.upload(hv, s_);
+    xt_.upload(hxt, s_);
     kappa_   = kappa;
     in_step_ = false;
     newton_k_ = 0;
@@ -570,6 +870,7 @@ void Context::assemble(bool hessian)
         coo_.count.zero(s_);
     }
     launch_init_diag(dev_, s_);
+    launch_abd_body(dev_, hessian, s_);
     launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
     launch_friction(dev_, hessian, gc_.data(), s_);
     launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
@@ -577,6 +878,7 @@ void Context::assemble(bool hessian)
     launch_fem(dev_, hessian, s_);
     prof_.end("fem", s_);
     launch_rhs(dev_, rhs_.data(), s_);
+    launch_abd_rhs(dev_, rhs_.data(), s_);
     if(hessian)
         coo_.build_rows(s_);
 }
@@ -644,6 +946,14 @@ double Context::cfl_speed()
     return read_d(scratch_.data() + 2);
 }
 
+// x = x_s - alpha dx (FEM), q = q_s - alpha dq and x = J q (affine bodies)
+void Context::step_forward(double alpha)
+{
+    launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+    if(nabd_)
+        launch_abd_step(dev_, dy_.data(), alpha, s_);
+}
+
 void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0, double* e1)
 {
     double E0 = energy();
@@ -654,13 +964,16 @@ void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0,
     double c1m = desc_.armijo_c1 * read_d(scratch_.data() + 3);
     SG_HIP(hipMemcpyAsync(xsave_.data(), x_.data(), 3 * (size_t)nv_ * sizeof(double),
                           hipMemcpyDeviceToDevice, s_));
-    launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+    if(nabd_)
+        SG_HIP(hipMemcpyAsync(bqs_.data(), bq_.data(), 12 * (size_t)nabd_ * sizeof(double),
+                              hipMemcpyDeviceToDevice, s_));
+    step_forward(alpha);
     build_bvh();
     while(is_intersected())
     {
         alpha /= 2.0;
         alpha = std::min(cfl_alpha, alpha);
-        launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+        step_forward(alpha);
         build_bvh();
     }
     build_cp();
@@ -673,7 +986,7 @@ void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0,
             break;
         alpha *= desc_.armijo_beta;
         k++;
-        launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+        step_forward(alpha);
         build_bvh();
         build_cp();
         E = energy();
@@ -685,7 +998,7 @@ void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0,
         {
             alpha *= desc_.armijo_beta;
             alpha = std::clamp(alpha, desc_.armijo_alpha_min, cfl_alpha);
-            launch_step_forward(dev_, xsave_.data(), dx_.data(), alpha, s_);
+            step_forward(alpha);
             build_bvh();
             recompute = true;
         }
@@ -789,7 +1102,10 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     cfg.max_iter     = desc_.pcg_max_iter;
     cfg.use_rz       = desc_.pcg_use_preconditioned_norm != 0;
     cfg.mixed        = desc_.solver_precision == SGIPC_SOLVE_MIXED;
-    PcgStats ps      = pcg_.solve(S, pc, &mas_, rhs_.data(), dx_.data(), cfg, s_);
+    double*  sol     = nabd_ ? dy_.data() : dx_.data();
+    PcgStats ps      = pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+    if(nabd_)  // vertex displacements J dq of the affine-body vertices
+        launch_abd_expand(dev_, dy_.data(), dx_.data(), s_);
     loc.pcg_iterations   = ps.iterations;
     loc.pcg_rel_residual = ps.rel_residual;
     loc.solver_fallback  = ps.outer_iterations > 1 ? 1 : 0;
@@ -843,6 +1159,8 @@ void Context::begin_step()
     // buildFrictionSets on the current contact sets (GIPC.cu:11016)
     launch_friction_build(dev_, kappa_, s_);
     dx_.zero(s_);
+    if(nabd_)
+        dy_.zero(s_);
     newton_k_ = 0;
     xdelta0_  = 0;
     in_step_  = true;
@@ -912,7 +1230,7 @@ void Context::profile_query(const char* name, int32_t* launches, double* total_m
 
 void Context::problem_sizes(int64_t* out)
 {
-    out[0] = nv_;
+    out[0] = nn_;
     out[1] = nt_;
     out[2] = A_.nnzb;
     out[3] = (int64_t)read_u(coo_.count.data());
@@ -966,7 +1284,7 @@ void Context::energy_terms(double* terms)
     terms[4] = t[3];
     terms[5] = t[4] + t[5];
     terms[6] = t[6] + t[7];
-    terms[7] = 0.0;
+    terms[7] = t[8];
 }
 
 void Context::lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs)
@@ -1015,7 +1333,7 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
     assemble(true);
     check_overflow();
     if(grad)
-        rhs_.download(grad, 3 * (size_t)nv_, s_);
+        rhs_.download(grad, 3 * (size_t)nn_, s_);
     std::vector<double> v = A_.vals.to_host(s_);
     unsigned            nc = read_u(coo_.count.data());
     std::vector<int>    cr(nc), cc(nc);
@@ -1028,12 +1346,12 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
     }
     SG_HIP(hipStreamSynchronize(s_));
     // merge static pattern and COO tail row by row
-    std::vector<std::vector<std::pair<int, const double*>>> extra(nv_);
+    std::vector<std::vector<std::pair<int, const double*>>> extra(nn_);
     for(unsigned k = 0; k < nc; k++)
         extra[cr[k]].push_back({cc[k], cv.data() + 9 * (size_t)k});
-    std::vector<int>    rp(nv_ + 1, 0), cl;
+    std::vector<int>    rp(nn_ + 1, 0), cl;
     std::vector<double> vl;
-    for(int i = 0; i < nv_; i++)
+    for(int i = 0; i < nn_; i++)
     {
         std::vector<std::pair<int, std::array<double, 9>>> row;
         for(int k = A_.h_rowptr[i]; k < A_.h_rowptr[i + 1]; k++)

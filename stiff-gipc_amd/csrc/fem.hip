@@ -25,17 +25,22 @@
 namespace sgipc
 {
 
-__global__ void k_init_diag(int nv, const double* __restrict__ mass, const double* __restrict__ x,
-                            const double* __restrict__ xt, const int* __restrict__ diag,
-                            double* __restrict__ vals, double* __restrict__ gshape,
-                            double* __restrict__ gcontact)
+// kinetic gradient of every vertex (affine-body vertices included: their
+// J^T reduction is M (q - q~)); m I blocks for the FEM vertices
+__global__ void k_init_diag(int nv, int nfem, const double* __restrict__ mass,
+                            const double* __restrict__ x, const double* __restrict__ xt,
+                            const int* __restrict__ diag, double* __restrict__ vals,
+                            double* __restrict__ gshape, double* __restrict__ gcontact)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= nv)
         return;
-    double  m = mass[i];
-    double* B = vals + 9 * (size_t)diag[i];
-    B[0] = m, B[1] = 0, B[2] = 0, B[3] = 0, B[4] = m, B[5] = 0, B[6] = 0, B[7] = 0, B[8] = m;
+    double m = mass[i];
+    if(i < nfem)
+    {
+        double* B = vals + 9 * (size_t)diag[i];
+        B[0] = m, B[1] = 0, B[2] = 0, B[3] = 0, B[4] = m, B[5] = 0, B[6] = 0, B[7] = 0, B[8] = m;
+    }
     for(int c = 0; c < 3; c++)
     {
         gshape[3 * i + c]   = (x[3 * i + c] - xt[3 * i + c]) * m;
@@ -299,8 +304,8 @@ __global__ void k_ground_detect(int ns, const uint32_t* __restrict__ surf,
 
 __global__ void k_ground(const unsigned* __restrict__ gcount, const uint32_t* __restrict__ gp,
                          const double* __restrict__ x, V3 n, double off, double dHat,
-                         double kappa, const int* __restrict__ btype, const int* __restrict__ diag,
-                         double* __restrict__ vals, double* __restrict__ gcontact, int hessian)
+                         double kappa, const int* __restrict__ btype, BlockSink sink,
+                         double* __restrict__ gcontact, int hessian)
 {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= *gcount)
@@ -323,7 +328,7 @@ __global__ void k_ground(const unsigned* __restrict__ gcount, const uint32_t* __
     for(int r = 0; r < 3; r++)
         for(int c = 0; c < 3; c++)
             blk[r * 3 + c] = p * nn[r] * nn[c];
-    add_block(vals + 9 * (size_t)diag[v], blk, false);
+    sink.put((int)v, (int)v, blk, true);
 }
 
 // --------------------------------------------------------------- barrier
@@ -539,7 +544,7 @@ __global__ void __launch_bounds__(256) k_energy_ground(const unsigned* __restric
     block_partial(e, out);
 }
 
-// sum the partial arrays (fixed order) into terms[0..7]; terms[8] = total
+// sum the partial arrays (fixed order) into terms[0..8]; terms[9] = total
 __global__ void __launch_bounds__(256) k_energy_final(const double* __restrict__ partial,
                                                       EnergyLayout lay, double* __restrict__ terms)
 {
@@ -618,8 +623,8 @@ __global__ void k_update_velocity(int nv, const int* __restrict__ btype, const d
 // ----------------------------------------------------------------- launch
 void launch_init_diag(const SimDev& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init_diag, dim3(grid_for(d.nv, 256)), dim3(256), 0, s, d.nv, d.mass,
-                       d.x, d.xt, d.diag, d.vals, d.gshape, d.gcontact);
+    hipLaunchKernelGGL(k_init_diag, dim3(grid_for(d.nv, 256)), dim3(256), 0, s, d.nv, d.nfem,
+                       d.mass, d.x, d.xt, d.diag, d.vals, d.gshape, d.gcontact);
     SG_LAUNCH_CHECK();
 }
 
@@ -663,7 +668,7 @@ void launch_ground(const SimDev& d, double kappa, bool hessian, double* gcontact
     if(!d.use_ground || d.ns == 0)
         return;
     hipLaunchKernelGGL(k_ground, dim3(grid_for(d.ns, 256)), dim3(256), 0, s, d.gcount, d.gp, d.x,
-                       d.gn, d.goff, d.dHat, kappa, d.btype, d.diag, d.vals, gcontact,
+                       d.gn, d.goff, d.dHat, kappa, d.btype, block_sink(d), gcontact,
                        hessian ? 1 : 0);
     SG_LAUNCH_CHECK();
 }
@@ -680,7 +685,9 @@ void launch_barrier(const SimDev& d, double kappa, bool hessian, double* gcontac
 
 void launch_rhs(const SimDev& d, double* rhs, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rhs, dim3(grid_for(3 * (long long)d.nv, 256)), dim3(256), 0, s, d.nv,
+    if(d.nfem == 0)
+        return;
+    hipLaunchKernelGGL(k_rhs, dim3(grid_for(3 * (long long)d.nfem, 256)), dim3(256), 0, s, d.nfem,
                        d.btype, d.gcontact, d.gshape, rhs);
     SG_LAUNCH_CHECK();
 }
@@ -696,6 +703,7 @@ EnergyLayout energy_layout(const SimDev& d)
     L.n[5]   = (d.fric.mu_gd != 0.0 && d.use_ground && d.ns) ? (int)grid_for(d.ns, 256) : 0;
     L.n[6]   = d.cloth.nf ? (int)grid_for(d.cloth.nf, 256) : 0;
     L.n[7]   = d.cloth.nh ? (int)grid_for(d.cloth.nh, 256) : 0;
+    L.n[8]   = d.abd.nb ? (int)grid_for(d.abd.nb, 256) : 0;
     L.off[0] = 0;
     for(int k = 1; k < kEnergyTerms; k++)
         L.off[k] = L.off[k - 1] + L.n[k - 1];
@@ -719,6 +727,7 @@ void launch_energy(const SimDev& d, double kappa, double* partial, double* terms
                            d.gn, d.goff, d.dHat, kappa, partial + L.off[3]);
     launch_energy_friction(d, L, partial, s);
     launch_energy_cloth(d, L, partial, s);
+    launch_energy_abd(d, L, partial, s);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial, L, terms);
     SG_LAUNCH_CHECK();
 }
@@ -738,8 +747,9 @@ void launch_max_abs(int n, const double* a, double* out, hipStream_t s)
 void launch_step_forward(const SimDev& d, const double* xs, const double* dx, double alpha,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_forward, dim3(grid_for(3 * (long long)d.nv, 256)), dim3(256), 0, s,
-                       d.nv, d.btype, xs, dx, alpha, d.x);
+    if(d.nfem)
+        hipLaunchKernelGGL(k_step_forward, dim3(grid_for(3 * (long long)d.nfem, 256)), dim3(256),
+                           0, s, d.nfem, d.btype, xs, dx, alpha, d.x);
     SG_LAUNCH_CHECK();
 }
 

@@ -38,6 +38,23 @@ struct ClothDev
     size_t          off_tri = 0, off_hinge = 0;  // record offsets in the element buffer
 };
 
+// affine bodies: vertices [nfem, nv) move with their body, x = p + A xbar;
+// the system nodes are [FEM vertices | 4 per body (p, a1, a2, a3)]
+struct AbdDev
+{
+    int           nb   = 0;        // bodies
+    int           nfem = 0;        // first ABD vertex = first body node
+    const int*    bstart = nullptr;  // nb + 1 vertex ranges
+    const int*    fixed  = nullptr;  // per body
+    const double* M      = nullptr;  // 12x12 dyadic mass per body (row-major)
+    const double* vol    = nullptr;  // per body
+    double*       q      = nullptr;  // 12 per body
+    double*       qs     = nullptr;  // q at the line-search start
+    double*       gbody  = nullptr;  // 12 per body: shape gradient
+    const int*    slots  = nullptr;  // 16 per body: BSR slot of node block (m, n)
+    double        kappa  = 0.0;
+};
+
 struct SimDev
 {
     int    nv = 0, nt = 0, ns = 0, nf = 0, ne = 0;
@@ -77,6 +94,12 @@ struct SimDev
     int*      overflow  = nullptr;  // [0] pairs, [1] coo, [2] ccd
     FricDev   fric;
     ClothDev  cloth;
+    // node map of every vertex: node_of[v] = v (FEM) or the body's first
+    // node; xbar = body-frame coordinates of ABD vertices (zero for FEM)
+    int           nfem    = 0;
+    const int*    node_of = nullptr;
+    const double* xbar    = nullptr;
+    AbdDev        abd;
 };
 
 constexpr int kFemStride = 102;  // 10 blocks x 9 + 12 gradient entries per tet
@@ -85,8 +108,9 @@ constexpr int kPairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
 constexpr int kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
 // partial-sum segments of the energy terms: kinetic, elastic, barrier,
-// ground, friction, ground friction, cloth membrane, cloth bending
-constexpr int kEnergyTerms = 8;
+// ground, friction, ground friction, cloth membrane, cloth bending, affine
+// shape
+constexpr int kEnergyTerms = 9;
 struct EnergyLayout
 {
     int n[kEnergyTerms];
@@ -105,6 +129,12 @@ void launch_friction_build(const SimDev& d, double kappa, hipStream_t s);
 void launch_friction(const SimDev& d, bool hessian, double* gcontact, hipStream_t s);
 void launch_energy_friction(const SimDev& d, const EnergyLayout& L, double* partial,
                             hipStream_t s);
+// abd.hip
+void launch_abd_body(const SimDev& d, bool hessian, hipStream_t s);
+void launch_abd_rhs(const SimDev& d, double* rhs, hipStream_t s);
+void launch_abd_expand(const SimDev& d, const double* dy, double* dx, hipStream_t s);
+void launch_abd_step(const SimDev& d, const double* dy, double alpha, hipStream_t s);
+void launch_energy_abd(const SimDev& d, const EnergyLayout& L, double* partial, hipStream_t s);
 // cloth.hip
 void launch_cloth(const SimDev& d, bool hessian, hipStream_t s);
 void launch_energy_cloth(const SimDev& d, const EnergyLayout& L, double* partial, hipStream_t s);

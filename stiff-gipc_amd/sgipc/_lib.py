@@ -266,11 +266,17 @@ class Simulation:
                                         P(ground, C.c_uint32), gcap, C.byref(ng)))
         return pairs[: counts[0]].copy(), counts, ground[: ng.value].copy()
 
+    @property
+    def nodes(self):
+        """system block rows: FEM vertices, then 4 per affine body"""
+        return int(self.sizes()["n_nodes"])
+
     def assemble(self):
-        g = np.zeros(3 * self.nv)
+        nn = self.nodes
+        g = np.zeros(3 * nn)
         nnz = C.c_int32()
         check(lib().sgipc_assemble(self.h, P(g), C.byref(nnz), None, None, None, 0))
-        rowptr = np.zeros(self.nv + 1, np.int32)
+        rowptr = np.zeros(nn + 1, np.int32)
         cols = np.zeros(nnz.value, np.int32)
         vals = np.zeros((nnz.value, 3, 3))
         check(lib().sgipc_assemble(self.h, P(g), C.byref(nnz), P(rowptr, C.c_int32),
@@ -321,6 +327,6 @@ class Simulation:
     def sizes(self):
         out = (C.c_int64 * 8)()
         check(lib().sgipc_problem_sizes(self.h, out))
-        keys = ["n_verts", "n_tets", "nnzb", "coo_blocks", "mas_banks", "mas_levels", "pairs",
+        keys = ["n_nodes", "n_tets", "nnzb", "coo_blocks", "mas_banks", "mas_levels", "pairs",
                 "ground"]
         return dict(zip(keys, list(out)))
