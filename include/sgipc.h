@@ -156,6 +156,9 @@ typedef struct sgipc_iter_stats {
     int32_t line_search_steps;
     int32_t converged;        /* Newton stop test met before solving   */
     int32_t solver_fallback;  /* 1: the mixed solve was redone in FP64 */
+    int32_t warm_start;       /* solve continued from the affine (1) or the
+                                 network-corrected (2) coarse guess        */
+    int32_t reserved;
     double  pcg_rel_residual; /* FP64 ||b-Ax||/||b||                   */
     double  alpha;            /* accepted step                         */
     double  alpha_ccd;        /* CCD / ground bound before line search */
@@ -201,6 +204,33 @@ int sgipc_advance(sgipc_t ctx, int32_t n, sgipc_advance_stats* stats);
 int sgipc_profile_enable(sgipc_t ctx, int32_t every);
 int sgipc_profile_query(sgipc_t ctx, const char* kernel, int32_t* launches,
                         double* total_ms);
+/* ---- learned affine coarse warm start (no counterpart in the reference) ----
+ * MAS's coarse spaces hold one translation per aggregate.  Each aggregate of
+ * the coarsest MAS level (its free FEM nodes) gets an affine space
+ * x_i = p + A (x_i - centroid) = J_i q (q in R^12, as for affine bodies):
+ * q_c = (P_c^T A P_c)^-1 P_c^T b per aggregate, then, in mode 2,
+ * q_c += |q_c| MLP(q_c / |q_c|, r_c / |r_c|) with r_c = P_c^T b (layers
+ * 24 -> 128 -> 128 -> 12 with ReLU; BF16 weights, FP32 accumulation).  The
+ * PCG solve continues from x0 = P q with the zero-guess stop threshold; plain
+ * MAS judges the guess, and the zero-guess solve is the fallback. */
+#define SGIPC_MLP_PARAMS 21260 /* W1[24][128] b1[128] W2[128][128] b2[128] W3[128][12] b3[12] */
+#define SGIPC_MLP_IN 24
+#define SGIPC_MLP_OUT 12
+/* mode 0 off, 1 affine Galerkin guess only, 2 guess corrected by the network */
+int sgipc_learned_mode(sgipc_t ctx, int32_t mode);
+/* training samples from the last solving Newton iteration: features
+ * [n][SGIPC_MLP_IN] and targets [n][SGIPC_MLP_OUT] ((q* - q_c) / |q_c| with q*
+ * the least-squares affine fit of the solution on the aggregate) */
+int sgipc_learned_samples(sgipc_t ctx, float* feat, float* target, int32_t cap, int32_t* n);
+/* full-batch Adam on the GPU (FP32 master weights, BF16 forward); final loss */
+int sgipc_mlp_train(sgipc_t ctx, const float* feat, const float* target, int32_t n,
+                    int32_t epochs, double lr, double* loss);
+/* mean squared error and its gradient (SGIPC_MLP_PARAMS floats) */
+int sgipc_mlp_loss_grad(sgipc_t ctx, const float* feat, const float* target, int32_t n,
+                        float* grad, double* loss);
+int sgipc_mlp_set(sgipc_t ctx, const float* weights);
+int sgipc_mlp_get(sgipc_t ctx, float* weights);
+
 /* problem sizes for algorithmic byte counts: out[8] = {n_nodes (system block
  * rows: FEM vertices + 4 per affine body), n_tets (elastic), nnzb static,
  * coo blocks, mas banks, mas levels, self pairs, ground pairs} */

@@ -218,6 +218,14 @@ void   orc_abd_x(const double xbar[3], const double q[12], double x[3]);
 void   orc_abd_jt(const double xbar[3], const double v[3], double out[12]);
 void   orc_abd_jmat(const double xbar[3], double J[36]);
 
+/* ---------------- learned warm start network (orc_mlp.c) ---------------- */
+float  orc_bf16_round(float f);
+int    orc_mlp_params(void);
+/* loss (mean squared error) and, when grad != NULL, its gradient (FP64) at
+ * FP32 master weights w (matrices used rounded to BF16); Y[n][12] optional */
+double orc_mlp_loss_grad(const float* w, const float* X, const float* T, int n,
+                         double* grad, float* Y);
+
 /* ---------------- world / Newton (orc_world.c) ---------------- */
 typedef struct orc_world orc_world;
 orc_world* orc_world_create(const sgipc_scene_desc* s);

@@ -626,7 +626,13 @@ static int seg_tri(v3 e0, v3 e1, v3 t0, v3 t1, v3 t2)
     ((A).x * (B).y * (C).z + (A).y * (B).z * (C).x + (A).z * (B).x * (C).y                   \
      - (A).z * (B).y * (C).x - (A).x * (B).z * (C).y - (B).x * (A).y * (C).z)
     double det = DETC(c0, c1, c2);
-    if(fabs(det) < 1e-20)
+    /* segTriIntersect's |det| < 1e-20, plus a scale-relative test so that an
+     * edge lying in the triangle's plane (coplanar surface faces) is rejected
+     * whatever the mesh size; the absolute constant alone lets round-off
+     * determinants through at centimetre scales */
+    double sc = sqrt((c0.x * c0.x + c0.y * c0.y + c0.z * c0.z) * (c1.x * c1.x + c1.y * c1.y + c1.z * c1.z)
+                     * (c2.x * c2.x + c2.y * c2.y + c2.z * c2.z));
+    if(fabs(det) < 1e-20 || fabs(det) <= 1e-12 * sc)
         return 0;
     v3     b = vsub(e0, t0);
     double u = DETC(b, c1, c2) / det, vv = DETC(c0, b, c2) / det, t = DETC(c0, c1, b) / det;

@@ -142,6 +142,12 @@ def lib():
         L.orc_abd_x.argtypes = [dp, dp, dp]
         L.orc_abd_jt.argtypes = [dp, dp, dp]
         L.orc_abd_jmat.argtypes = [dp, dp]
+        L.orc_bf16_round.restype = C.c_float
+        L.orc_bf16_round.argtypes = [C.c_float]
+        L.orc_mlp_params.restype = C.c_int
+        L.orc_mlp_loss_grad.restype = d
+        L.orc_mlp_loss_grad.argtypes = [C.POINTER(C.c_float)] * 3 + [C.c_int, dp,
+                                                                      C.POINTER(C.c_float)]
         L.orc_world_nodes.restype = C.c_int
         L.orc_world_nodes.argtypes = [C.c_void_p]
         L.orc_pt_ccd.restype = C.c_double
@@ -574,3 +580,16 @@ def abd_jmat(xbar):
     J = np.zeros(36)
     lib().orc_abd_jmat(P(_f64(xbar)), P(J))
     return J.reshape(3, 12)
+
+
+# ------------------------------------------------------------- warm-start MLP
+def mlp_loss_grad(w, X, T, want_y=False):
+    fp = C.POINTER(C.c_float)
+    w = np.ascontiguousarray(w, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    g = np.zeros(lib().orc_mlp_params())
+    Y = np.zeros((X.shape[0], 12), np.float32)
+    L = lib().orc_mlp_loss_grad(w.ctypes.data_as(fp), X.ctypes.data_as(fp), T.ctypes.data_as(fp),
+                                X.shape[0], P(g), Y.ctypes.data_as(fp))
+    return (L, g, Y) if want_y else (L, g)

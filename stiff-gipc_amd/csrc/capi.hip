@@ -395,6 +395,57 @@ int sgipc_profile_query(sgipc_t h, const char* kernel, int32_t* launches, double
     });
 }
 
+int sgipc_learned_mode(sgipc_t h, int32_t mode)
+{
+    return guard([&] { ctx_of(h).set_learned(mode); });
+}
+
+int sgipc_learned_samples(sgipc_t h, float* feat, float* target, int32_t cap, int32_t* n)
+{
+    return guard([&] {
+        SG_CHECK(feat && target && n && cap >= 0, SGIPC_EINVAL, "null output");
+        *n = ctx_of(h).learned_samples(feat, target, cap);
+    });
+}
+
+int sgipc_mlp_train(sgipc_t h, const float* feat, const float* target, int32_t n,
+                    int32_t epochs, double lr, double* loss)
+{
+    return guard([&] {
+        SG_CHECK(feat && target, SGIPC_EINVAL, "null batch");
+        double L = ctx_of(h).mlp_train(feat, target, n, epochs, lr);
+        if(loss)
+            *loss = L;
+    });
+}
+
+int sgipc_mlp_loss_grad(sgipc_t h, const float* feat, const float* target, int32_t n,
+                        float* grad, double* loss)
+{
+    return guard([&] {
+        SG_CHECK(feat && target, SGIPC_EINVAL, "null batch");
+        double L = ctx_of(h).mlp_loss_grad(feat, target, n, grad);
+        if(loss)
+            *loss = L;
+    });
+}
+
+int sgipc_mlp_set(sgipc_t h, const float* w)
+{
+    return guard([&] {
+        SG_CHECK(w, SGIPC_EINVAL, "null weights");
+        ctx_of(h).mlp_set(w);
+    });
+}
+
+int sgipc_mlp_get(sgipc_t h, float* w)
+{
+    return guard([&] {
+        SG_CHECK(w, SGIPC_EINVAL, "null weights");
+        ctx_of(h).mlp_get(w);
+    });
+}
+
 int sgipc_problem_sizes(sgipc_t h, int64_t* out)
 {
     return guard([&] {

@@ -347,7 +347,12 @@ __device__ bool seg_tri(V3 e0, V3 e1, V3 t0, V3 t1, V3 t2)
     if(dot(n, e0 - t0) * dot(n, e1 - t0) > 0)
         return false;
     double det = det_cols(c0, c1, c2);
-    if(fabs(det) < 1e-20)
+    // segTriIntersect's |det| < 1e-20, plus a scale-relative test so that an
+    // edge lying in the triangle's plane (coplanar surface faces) is rejected
+    // whatever the mesh size; the absolute constant alone lets round-off
+    // determinants through at centimetre scales
+    double sc = sqrt(sqn(c0) * sqn(c1) * sqn(c2));
+    if(fabs(det) < 1e-20 || fabs(det) <= 1e-12 * sc)
         return false;
     V3     b = e0 - t0;
     double u = det_cols(b, c1, c2) / det, v = det_cols(c0, b, c2) / det,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "../../include/sgipc.h"
+#include "learned.h"
 #include "linsys.h"
 #include "sim_kernels.h"
 
@@ -39,6 +40,19 @@ class Context
     void profile_enable(int every);
     void profile_query(const char* name, int32_t* launches, double* total_ms);
     void problem_sizes(int64_t* out);
+    // learned coarse warm start: mode 0 off, 1 Galerkin-Jacobi guess, 2 network
+    void set_learned(int mode);
+    int  learned_samples(float* feat, float* target, int cap);
+    double mlp_train(const float* X, const float* T, int n, int epochs, double lr)
+    {
+        return mlp_.train(X, T, n, epochs, lr, s_);
+    }
+    double mlp_loss_grad(const float* X, const float* T, int n, float* grad)
+    {
+        return mlp_.loss_grad(X, T, n, grad, s_);
+    }
+    void mlp_set(const float* w) { mlp_.set(w, s_); SG_HIP(hipStreamSynchronize(s_)); }
+    void mlp_get(float* w) const { mlp_.get(w, s_); }
 
     DBuf<double> x_, v_;
 
@@ -98,6 +112,13 @@ class Context
     DBuf<int>      overflow_;
     DBuf<double>   epart_, eterms_;
     int            pair_cap_ = 0, cand_cap_ = 0;
+
+    Mlp          mlp_;
+    Coarse       coarse_;
+    int          learned_ = 0;
+    DBuf<double> x0_, b2_, x1_, b3_, zt_;
+    PcgStats     solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
+                              int* warm);
 
     Bsr   A_;
     Coo   coo_;

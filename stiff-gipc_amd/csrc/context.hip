@@ -969,8 +969,11 @@ void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0,
                               hipMemcpyDeviceToDevice, s_));
     step_forward(alpha);
     build_bvh();
-    while(is_intersected())
+    // the halvings end at alpha = 0 (the intersection-free start) unless the
+    // direction is not finite; bounded so that case reports an error
+    for(int h = 0; is_intersected(); h++)
     {
+        SG_CHECK(h < 1100, SGIPC_ENUMERIC, "line search found no intersection-free step");
         alpha /= 2.0;
         alpha = std::min(cfl_alpha, alpha);
         step_forward(alpha);
@@ -994,8 +997,9 @@ void Context::line_search(double& alpha, double cfl_alpha, int* nls, double* e0,
     if(alpha < LF)
     {
         bool recompute = false;
-        while(is_intersected())
+        for(int h = 0; is_intersected(); h++)
         {
+            SG_CHECK(h < 1100, SGIPC_ENUMERIC, "line search found no intersection-free step");
             alpha *= desc_.armijo_beta;
             alpha = std::clamp(alpha, desc_.armijo_alpha_min, cfl_alpha);
             step_forward(alpha);
@@ -1103,7 +1107,9 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     cfg.use_rz       = desc_.pcg_use_preconditioned_norm != 0;
     cfg.mixed        = desc_.solver_precision == SGIPC_SOLVE_MIXED;
     double*  sol     = nabd_ ? dy_.data() : dx_.data();
-    PcgStats ps      = pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+    PcgStats ps      = solve_linear(S, pc, cfg, sol, &loc.warm_start);
+    SG_CHECK(ps.rel_residual == ps.rel_residual, SGIPC_ENUMERIC,
+             "the linear solve produced a non-finite direction");
     if(nabd_)  // vertex displacements J dq of the affine-body vertices
         launch_abd_expand(dev_, dy_.data(), dx_.data(), s_);
     loc.pcg_iterations   = ps.iterations;
@@ -1148,6 +1154,96 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     prof_.resolve();
     if(st)
         *st = loc;
+}
+
+// A sol = rhs.  With a warm-start mode on and a coarse MAS level, the solve
+// continues from a coarse guess x0 (A y = rhs - A x0, sol = x0 + y) with the
+// stop threshold of the zero-guess solve.  Plain MAS judges the guesses: the
+// affine guess and, in mode 2, the network-corrected one are measured by
+// their preconditioned residuals, the lower one is used when it beats the
+// zero guess, and the zero-guess solve runs whenever the warm solve does not
+// converge.
+PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
+                               int* warm)
+{
+    *warm = 0;
+    if(learned_ && pc == Precond::Mas && coarse_.setup(mas_, s_))
+    {
+        size_t n = 3 * (size_t)nn_;
+        x0_.resize(n);
+        b2_.resize(n);
+        zt_.resize(n);
+        double* q = scratch_.data() + 8;  // q[0] plain, q[1] affine, q[2] learned
+        SG_HIP(hipMemsetAsync(q, 0, 3 * sizeof(double), s_));
+        // r.z (or r.r) of a right-hand side into q[k]
+        auto measure = [&](const double* r, int k) {
+            if(cfg.use_rz)
+            {
+                mas_.apply(r, zt_.data(), s_);
+                launch_dot((int)n, r, zt_.data(), q + k, s_);
+            }
+            else
+                launch_dot((int)n, r, r, q + k, s_);
+        };
+        coarse_.build(nn_, nfem_, btype_.data(), x_.data(), A_, &coo_, rhs_.data(), s_);
+        coarse_.guess(nullptr, x0_.data(), s_);
+        residual(S, rhs_.data(), x0_.data(), b2_.data(), s_);
+        measure(rhs_.data(), 0);
+        measure(b2_.data(), 1);
+        bool net = learned_ == 2 && mlp_.ready();
+        if(net)
+        {
+            x1_.resize(n);
+            b3_.resize(n);
+            coarse_.guess(&mlp_, x1_.data(), s_);
+            residual(S, rhs_.data(), x1_.data(), b3_.data(), s_);
+            measure(b3_.data(), 2);
+        }
+        double h[3];
+        SG_HIP(hipMemcpyAsync(h, q, sizeof(h), hipMemcpyDeviceToHost, s_));
+        SG_HIP(hipStreamSynchronize(s_));
+        int     pick = 1;  // the guess with the lowest preconditioned residual
+        double  qb   = h[1];
+        double* xg   = x0_.data();
+        double* bg   = b2_.data();
+        if(net && h[2] < h[1])
+        {
+            pick = 2;
+            qb   = h[2];
+            xg   = x1_.data();
+            bg   = b3_.data();
+        }
+        if(h[0] > 0.0 && qb < h[0])
+        {
+            PcgConfig c = cfg;
+            c.q_ref     = h[0];
+            PcgStats ps = pcg_.solve(S, pc, &mas_, bg, sol, c, s_);
+            if(ps.converged)
+            {
+                launch_axpy_d((int)n, 1.0, xg, sol, s_);
+                ps.rel_residual = pcg_.true_residual(S, rhs_.data(), sol, s_);
+                *warm           = pick;
+                return ps;
+            }
+        }
+    }
+    return pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+}
+
+void Context::set_learned(int mode)
+{
+    SG_CHECK(mode >= 0 && mode <= 2, SGIPC_EINVAL, "learned mode must be 0, 1 or 2");
+    SG_CHECK(mode < 2 || mlp_.ready(), SGIPC_EINVAL, "no trained network (train or set it first)");
+    learned_ = mode;
+}
+
+int Context::learned_samples(float* feat, float* target, int cap)
+{
+    SG_CHECK(mas_.ready() && coarse_.setup(mas_, s_), SGIPC_EINVAL,
+             "no coarse MAS level (run a solving Newton iteration with MAS first)");
+    // positions of the solve: the line search started from xsave_
+    coarse_.build(nn_, nfem_, btype_.data(), xsave_.data(), A_, &coo_, rhs_.data(), s_);
+    return coarse_.samples(nabd_ ? dy_.data() : dx_.data(), feat, target, cap, s_);
 }
 
 void Context::begin_step()

@@ -87,6 +87,8 @@ double read_slot_set(const double* d, hipStream_t s);
 void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t s);
 // r = b - A x
 void residual(const SysView& S, const double* b, const double* x, double* r, hipStream_t s);
+// y += a x (FP64)
+void launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t s);
 // dot(a, b) accumulated into the slot set `out` (must be zeroed)
 void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t s);
 // refresh the FP32 copies of the matrix (static pattern and contact rows)
@@ -128,6 +130,8 @@ class Mas
     const std::vector<int>& level_sizes() const { return h_lvl_; }  // (count, off) pairs
     std::vector<int> chain_host(hipStream_t s) const;
     bool ready() const { return built_; }
+    const int*    chain_dev() const { return chain_.data(); }  // nv x (L - 1) global ids
+    const double* bank_mats() const { return B_.data(); }      // nbanks x 48 x 48 Galerkin
     int  banks() const { return total_clusters_ / 16; }
     Profiler* prof = nullptr;
 
@@ -170,6 +174,9 @@ struct PcgConfig
 {
     double rel_tol     = 1e-4;
     double abs_tol     = 0.0;
+    // > 0: the relative threshold is taken against this initial quantity
+    // (r.z or r.r of the zero-guess right-hand side) instead of the solve's own
+    double q_ref       = 0.0;
     int    max_iter    = 0;
     bool   use_rz      = true;
     int    check_every = 10;
@@ -223,6 +230,8 @@ class Pcg
     // block-Jacobi inverse of the diagonal blocks (DiagPreconditioner)
     void build_diag(const Bsr& A, hipStream_t s);
     void apply_diag(int nb, const double* r, double* z, hipStream_t s);
+    // FP64 ||b - A x|| / ||b||
+    double   true_residual(const SysView& S, const double* b, const double* x, hipStream_t s);
     Profiler* prof = nullptr;
 
   private:
@@ -232,7 +241,6 @@ class Pcg
                      const PcgConfig& cfg, hipStream_t s, const double* b64, double* x64);
     PcgStats solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
                          const PcgConfig& cfg, hipStream_t s);
-    double   true_residual(const SysView& S, const double* b, const double* x, hipStream_t s);
     template <typename V>
     PcgVecs<V>& vecs();
     template <typename V>

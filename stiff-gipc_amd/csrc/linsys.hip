@@ -755,7 +755,7 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         return st;
     }
     double rel2 = cfg.rel_tol * cfg.rel_tol, abs2 = cfg.abs_tol * cfg.abs_tol;
-    double q0   = cfg.use_rz ? rz0 : rr0;
+    double q0   = cfg.q_ref > 0.0 ? cfg.q_ref : (cfg.use_rz ? rz0 : rr0);
     double thr  = rel2 * q0 > abs2 ? rel2 * q0 : abs2;
 
     PcgCtl ctl_only;
@@ -917,6 +917,19 @@ PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* 
         return f;
     }
     return st;
+}
+
+__global__ void k_axpy_d(int n, double a, const double* __restrict__ x, double* __restrict__ y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+        y[i] += a * x[i];
+}
+
+void launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy_d, dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, x, y);
+    SG_LAUNCH_CHECK();
 }
 
 }  // namespace sgipc

@@ -43,7 +43,8 @@ class PcgStats(C.Structure):
 class IterStats(C.Structure):
     _fields_ = [("pcg_iterations", C.c_int32), ("n_pairs", C.c_int32), ("n_ground", C.c_int32),
                 ("line_search_steps", C.c_int32), ("converged", C.c_int32),
-                ("solver_fallback", C.c_int32),
+                ("solver_fallback", C.c_int32), ("warm_start", C.c_int32),
+                ("reserved", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("alpha", C.c_double),
                 ("alpha_ccd", C.c_double), ("xdelta", C.c_double), ("energy0", C.c_double),
                 ("energy1", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
@@ -51,7 +52,7 @@ class IterStats(C.Structure):
                 ("ms_total", C.c_double)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f not in ("pad", "reserved")}
 
 
 class StepStats(C.Structure):
@@ -81,8 +82,10 @@ EXPORTS = [
     "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
     "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step", "sgipc_advance",
     "sgipc_profile_enable", "sgipc_profile_query", "sgipc_problem_sizes",
-    "sgipc_set_state", "sgipc_set_pcg",
+    "sgipc_set_state", "sgipc_set_pcg", "sgipc_learned_mode", "sgipc_learned_samples",
+    "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
 ]
+MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
 
 
 def lib():
@@ -121,6 +124,16 @@ def lib():
             L.sgipc_profile_enable.argtypes = [C.c_void_p, C.c_int32]
             L.sgipc_profile_query.argtypes = [C.c_void_p, C.c_char_p, ip, dp]
             L.sgipc_problem_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+            fp = C.POINTER(C.c_float)
+            L.sgipc_learned_mode.argtypes = [C.c_void_p, C.c_int32]
+            L.sgipc_learned_samples.argtypes = [C.c_void_p, fp, fp, C.c_int32,
+                                                C.POINTER(C.c_int32)]
+            L.sgipc_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int32, C.c_int32, C.c_double,
+                                          C.POINTER(C.c_double)]
+            L.sgipc_mlp_loss_grad.argtypes = [C.c_void_p, fp, fp, C.c_int32, fp,
+                                              C.POINTER(C.c_double)]
+            L.sgipc_mlp_set.argtypes = [C.c_void_p, fp]
+            L.sgipc_mlp_get.argtypes = [C.c_void_p, fp]
         _lib = L
     return _lib
 
@@ -323,6 +336,45 @@ class Simulation:
         ms = C.c_double()
         check(lib().sgipc_profile_query(self.h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    # ---- learned coarse warm start
+    def learned_mode(self, mode):
+        check(lib().sgipc_learned_mode(self.h, int(mode)))
+
+    def learned_samples(self, cap=1 << 16):
+        f = np.zeros((cap, MLP_IN), np.float32)
+        t = np.zeros((cap, MLP_OUT), np.float32)
+        n = C.c_int32()
+        check(lib().sgipc_learned_samples(self.h, P(f, C.c_float), P(t, C.c_float), cap,
+                                          C.byref(n)))
+        return f[:n.value].copy(), t[:n.value].copy()
+
+    def mlp_train(self, feat, target, epochs=200, lr=1e-3):
+        f = np.ascontiguousarray(feat, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_double()
+        check(lib().sgipc_mlp_train(self.h, P(f, C.c_float), P(t, C.c_float), f.shape[0],
+                                    int(epochs), float(lr), C.byref(loss)))
+        return loss.value
+
+    def mlp_loss_grad(self, feat, target):
+        f = np.ascontiguousarray(feat, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        check(lib().sgipc_mlp_loss_grad(self.h, P(f, C.c_float), P(t, C.c_float), f.shape[0],
+                                        P(g, C.c_float), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_weights(self, w=None):
+        if w is not None:
+            w = np.ascontiguousarray(w, np.float32)
+            assert w.size == MLP_PARAMS
+            check(lib().sgipc_mlp_set(self.h, P(w, C.c_float)))
+            return w
+        out = np.zeros(MLP_PARAMS, np.float32)
+        check(lib().sgipc_mlp_get(self.h, P(out, C.c_float)))
+        return out
 
     def sizes(self):
         out = (C.c_int64 * 8)()

@@ -1,0 +1,158 @@
+"""The learned affine coarse warm start on the GPU (no reference counterpart):
+the network's BF16/FP32 loss and gradient against the oracle's FP64 pass,
+GPU training, the affine Galerkin guess against a numpy restatement on the
+GPU's own system, and what the warm start gains over plain MAS-PCG."""
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+NP, NI, NO = 21260, 24, 12
+
+
+def _cubes(seed=3, **kw):
+    # spacing 0.55: with +/-15% jitter no two rotated cubes (half diagonal
+    # 0.173) can overlap at the start
+    from sgipc import scene as S
+    return S.cubes_scene(n_side=2, cells=4, size=0.2, spacing=0.55, seed=seed, **kw)
+
+
+def test_mlp_loss_and_gradient_match_oracle(oracle):
+    import sgipc
+    sim = sgipc.Simulation(_cubes())
+    rng = np.random.default_rng(0)
+    w = (rng.normal(size=NP) * 0.08).astype(np.float32)
+    X = rng.normal(size=(96, NI)).astype(np.float32)
+    T = rng.normal(size=(96, NO)).astype(np.float32)
+    sim.mlp_weights(w)
+    np.testing.assert_array_equal(sim.mlp_weights(), w)
+    L, g = sim.mlp_loss_grad(X, T)
+    Lo, go = oracle.mlp_loss_grad(w, X, T)
+    print(f"[mlp-grad] loss gpu={L:.9g} oracle={Lo:.9g} grad rel={rel(g, go):.2e}", flush=True)
+    assert L == pytest.approx(Lo, rel=1e-5)
+    assert rel(g, go) < 1e-4
+
+
+def test_mlp_training_reduces_loss(oracle):
+    import sgipc
+    sim = sgipc.Simulation(_cubes())
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(512, NI)).astype(np.float32)
+    R = rng.normal(size=(NI, NO)) / np.sqrt(NI)
+    T = np.tanh(X @ R).astype(np.float32)
+    w0 = np.zeros(NP, np.float32)
+    w0[:NI * 128] = (rng.normal(size=NI * 128) * np.sqrt(2.0 / NI)).astype(np.float32)
+    o2 = NI * 128 + 128
+    w0[o2:o2 + 128 * 128] = (rng.normal(size=128 * 128) * np.sqrt(2.0 / 128)).astype(np.float32)
+    sim.mlp_weights(w0)
+    L0, _ = sim.mlp_loss_grad(X, T)
+    L1 = sim.mlp_train(X, T, epochs=400, lr=2e-3)
+    w1 = sim.mlp_weights()
+    Lo, _ = oracle.mlp_loss_grad(w1, X, T)
+    print(f"[mlp-train] loss {L0:.4g} -> {L1:.4g} (oracle at trained weights {Lo:.4g})", flush=True)
+    assert np.all(np.isfinite(w1))
+    assert L1 < 0.1 * L0
+    assert L1 == pytest.approx(Lo, rel=1e-4)
+
+
+def test_affine_guess_matches_numpy(oracle):
+    """features (q_c / |q_c|, r_c / |r_c|) and targets of the coarse systems
+    of one Newton solve against a numpy restatement: aggregates from the
+    oracle's MAS on the GPU-assembled matrix, P_c = [J_i] over free nodes,
+    q_c = (P^T A P)^-1 P^T b, target = (lsq fit of dx - q_c) / |q_c|"""
+    import sgipc
+    sc = _cubes()
+    sc.mas_precision = sgipc.SGIPC_MAS_FP64
+    sc.pcg_rel_tol = 1e-10
+    # vertex velocity noise: the Newton direction has a non-affine part
+    sc.v = 0.05 * np.random.default_rng(4).normal(size=sc.x.shape)
+    sim = sgipc.Simulation(sc)
+    sim.time_step()                   # falling: no contact at the next step's start
+    x = sim.positions().copy()
+    g, rp, cl, vl = sim.assemble()
+    sim.advance(1)                    # begins the step and solves this system
+    f, t = sim.learned_samples()
+    nb = rp.size - 1
+    rows = np.repeat(np.arange(nb), np.diff(rp))
+    up = rows <= cl
+    A = oracle.bsr_from_upper_triplets(nb, rows[up], cl[up], vl[up].reshape(-1, 9))
+    mas = oracle.Mas(A, sc.part_id, fp32=False)
+    L = mas.levels
+    top = mas.chain()[:, L - 2]
+    H = A.dense()
+    dx = np.linalg.solve(H, g).reshape(-1, 3)
+    feats, targs = [], []
+    for c in np.unique(top):
+        nodes = np.where((top == c) & (sc.btype == 0))[0]
+        xb = x[nodes] - x[nodes].mean(0)
+        P = np.concatenate([oracle.abd_jmat(v) for v in xb], 0)
+        idx = (3 * nodes[:, None] + np.arange(3)).ravel()
+        r = P.T @ g[idx]
+        q = np.linalg.solve(P.T @ H[np.ix_(idx, idx)] @ P, r)
+        qs = np.linalg.lstsq(P, dx[nodes].ravel(), rcond=None)[0]
+        feats.append(np.concatenate([q / np.linalg.norm(q), r / np.linalg.norm(r)]))
+        targs.append((qs - q) / np.linalg.norm(q))
+    feats, targs = np.array(feats), np.array(targs)
+    print(f"[affine-guess] aggregates gpu={f.shape[0]} numpy={feats.shape[0]} "
+          f"feat={rel(f, feats):.2e} target={rel(t, targs):.2e}", flush=True)
+    assert f.shape == feats.shape
+    assert rel(f, feats) < 1e-6     # features are stored in FP32
+    assert rel(t, targs) < 1e-5
+
+
+def _run(mode, seed, steps, weights=None, mixed=False):
+    import sgipc
+    sc = _cubes(seed=seed, fall=0.02)
+    if mixed:
+        sc.solver_precision = sgipc.SGIPC_SOLVE_MIXED
+    sim = sgipc.Simulation(sc)
+    if weights is not None:
+        sim.mlp_weights(weights)
+    sim.learned_mode(mode)
+    pcg = 0
+    for k in range(steps):
+        st = sim.time_step()
+        pcg += st["pcg_iterations"]
+        print(f"[warm-run] mode={mode} step={k} {st}", flush=True)
+    return sim, pcg
+
+
+def test_warm_start_gain(oracle):
+    """train on one scene's own Newton solves, evaluate on a held-out scene:
+    the affine guess and the learned correction against plain MAS-PCG, with
+    trajectories equal to the solver's tolerance"""
+    import sgipc
+    # collect (features, targets) from a training scene's solves, solved
+    # tightly so that the targets carry the coupling, not PCG truncation
+    sc = _cubes(seed=11, fall=0.02)
+    sc.pcg_rel_tol = 1e-8
+    sim = sgipc.Simulation(sc)
+    sim.learned_mode(1)
+    F, T = [], []
+    for k in range(30):
+        st = sim.advance(1)
+        f, t = sim.learned_samples()
+        F.append(f)
+        T.append(t)
+        print(f"[warm-collect] it={k} {st} samples={f.shape[0]}", flush=True)
+    F, T = np.concatenate(F), np.concatenate(T)
+    trainer = sgipc.Simulation(_cubes())
+    loss = trainer.mlp_train(F, T, epochs=600, lr=1e-3)
+    w = trainer.mlp_weights()
+    L0 = oracle.mlp_loss_grad(np.zeros(NP, np.float32), F, T)[0]
+    print(f"[warm-train] samples={F.shape[0]} loss untrained={L0:.4g} trained={loss:.4g}",
+          flush=True)
+    steps = 16                        # free fall, landing on the ground, resting
+    s0, p0 = _run(0, 3, steps)
+    s1, p1 = _run(1, 3, steps)
+    s2, p2 = _run(2, 3, steps, weights=w)
+    x0 = s0.positions()
+    e1 = np.abs(s1.positions() - x0).max() / np.abs(x0).max()
+    e2 = np.abs(s2.positions() - x0).max() / np.abs(x0).max()
+    print(f"[warm-gain] pcg plain={p0} affine={p1} learned={p2}  "
+          f"dev affine={e1:.2e} learned={e2:.2e}", flush=True)
+    assert p1 < 0.8 * p0
+    assert p2 <= 1.1 * p1
+    assert e1 < 1e-3 and e2 < 1e-3
