@@ -9,8 +9,11 @@ dhat = 1e-3 m, MAS-PCG to 1e-4 relative (preconditioned) residual.
 A bench "step" is one Newton iteration that solves (assembly, MAS build,
 PCG, CCD bounds, line search) of the running time integration.  The scene is
 first integrated for --settle time steps (untimed) so the timed iterations
-include ground and cube-cube contacts.  Mode: FP64 assembly/SpMV/PCG with the
-reference's FP32 MAS apply (the reference's "plain FP64" path).
+include ground and cube-cube contacts.  Value: FP64 assembly/SpMV/PCG with the
+reference's FP32 MAS apply (the reference's "plain FP64" path); the "modes"
+object adds the FP64 true-residual and mixed-precision solves and the solves
+continued from the affine coarse guess with and without the learned
+correction (trained, untimed, on the settled scene's own Newton systems).
 
 Prints ONE JSON line on rank 0.
 """
@@ -62,6 +65,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="all", choices=["all", "ref"],
                     help="time all solver modes or only the reference FP64 one")
+    ap.add_argument("--train-iters", type=int, default=12,
+                    help="Newton solves whose coarse systems train the network")
+    ap.add_argument("--train-epochs", type=int, default=400)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -96,23 +102,60 @@ def main():
                 f"({time.time() - t0:.1f} s)")
     state = (sim.positions(), sim.velocities(), sim.xtilde(), sim.kappa)
 
+    # ---- training data for the learned coarse correction (untimed): solves
+    # of the settled scene with the affine warm start, tight tolerance so the
+    # targets carry the coupling between aggregates; full-batch Adam on GPU
+    train = None
+    if args.modes == "all" and args.train_iters > 0:
+        t0 = time.time()
+        sim.set_pcg(rel_tol=1e-8, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
+        sim.learned_mode(1)
+        F, T = [], []
+        for _ in range(args.train_iters):
+            sim.advance(1)
+            f, t = sim.learned_samples()
+            F.append(f)
+            T.append(t)
+        F, T = np.concatenate(F), np.concatenate(T)
+        t1 = time.time()
+        loss = sim.mlp_train(F, T, epochs=args.train_epochs, lr=1e-3)
+        t2 = time.time()
+        sim.learned_mode(0)
+        train = {"samples": int(F.shape[0]), "collect_s": round(t1 - t0, 2),
+                 "train_s": round(t2 - t1, 2), "epochs": args.train_epochs,
+                 "loss_untrained": float((T.astype(np.float64) ** 2).mean()),
+                 "loss_trained": loss}
+        log(f"learned correction: {train}")
+
     # ---- timed Newton iterations, each solver mode from the same settled state.
     # fp64_ref (the value): the reference's plain FP64 solve, stop test on the
     # preconditioned norm r.z every 10 iterations.  fp64_true / mixed_true: stop
     # on the FP64 true residual ||b - A x|| / ||b|| <= 1e-4, FP64 PCG versus FP32
     # inner PCG with FP64 refinement.
+    # fp64_affine / fp64_learned / mixed_learned: the same solves continued
+    # from the affine coarse guess, or from the guess the trained network
+    # corrects (MAS picks the better guess; zero-guess fallback), with the
+    # zero-guess stop threshold.
+    ref = dict(rel_tol=1e-4, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
+    true64 = dict(rel_tol=1e-4, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_FP64)
+    true32 = dict(rel_tol=1e-4, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_MIXED)
     modes = [
-        ("fp64_ref", dict(rel_tol=1e-4, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)),
-        ("fp64_true", dict(rel_tol=1e-4, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_FP64)),
-        ("mixed_true", dict(rel_tol=1e-4, use_rz=False,
-                            solver_precision=sgipc.SGIPC_SOLVE_MIXED)),
+        ("fp64_ref", ref, 0),
+        ("fp64_true", true64, 0),
+        ("mixed_true", true32, 0),
+        ("fp64_affine", ref, 1),
+        ("fp64_learned", ref, 2),
+        ("mixed_learned", true32, 2),
     ]
     results = {}
-    for name, cfg in modes:
+    for name, cfg, lmode in modes:
         if args.modes != "all" and name != "fp64_ref":
+            continue
+        if lmode == 2 and train is None:
             continue
         sim.set_state(*state)
         sim.set_pcg(**cfg)
+        sim.learned_mode(lmode)
         if args.warmup:
             sim.advance(args.warmup)
         # HIP-event timing of every 10th launch of the profiled kernels during
@@ -140,6 +183,7 @@ def main():
                         f"{tot / n * 1e3:.2f} us avg")
             ref_adv, ref_ms = adv, ms
             sim.profile(0)
+    sim.learned_mode(0)
     adv, ms = ref_adv, ref_ms
     sz = sim.sizes()
     log(f"sizes={sz}")
@@ -193,6 +237,7 @@ def main():
         "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
         "max_pcg_rel_residual": adv["max_pcg_rel_residual"],
         "modes": results,
+        "learned_training": train,
         "config": {
             "workload": f"{args.n_side ** 3} tet cubes of {args.cells}^3 hex cells "
                         f"({sc.n_tets} tets, {sc.n_verts} verts), E log-uniform [1e5,1e9] Pa, "
