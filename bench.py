@@ -38,9 +38,11 @@ def algorithmic_bytes(name, sz, nparts):
     """Bytes one launch must move at minimum (each operand read or written once)."""
     nv, nt, nnzb, ncoo = sz["n_nodes"], sz["n_tets"], sz["nnzb"], sz["coo_blocks"]
     if name == "spmv":
-        # static BSR: 72 B block + 4 B column; rowptr; x read, y write (24 B each);
-        # COO tail: block + 2 ids + x gather + y read-modify-write
-        return nnzb * 76 + (nv + 1) * 4 + nv * 48 + ncoo * (72 + 8 + 24 + 48)
+        # the PCG SpMV forms p = z + beta p_prev on the fly and computes
+        # y = A p: static BSR 72 B block + 4 B column and its row pointers;
+        # contact rows (row-sorted) 72 B block + 4 B column and row pointers;
+        # per node z and p_prev read, p and y written (24 B each)
+        return nnzb * 76 + ncoo * 76 + 2 * (nv + 1) * 4 + nv * 96
     if name == "mas_apply_l0":
         # FP32 48x48 inverse per level-0 bank; per dof: read x, p, Ap, r, write
         # x, r (FP64) and Z (FP32); slot map

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -173,6 +174,21 @@ class Profiler
         ent.pending.push_back(p);
         SG_HIP(hipEventRecord(p.first, s));
     }
+    // events for hipExtLaunchKernel, which records them with the dispatch
+    // itself: they bracket the kernel's own execution (what a kernel-trace
+    // profiler reports), without a separate marker packet before and after.
+    // Nulls when profiling is off or this launch is not sampled.
+    std::pair<hipEvent_t, hipEvent_t> kernel_events(const char* name)
+    {
+        if(!enabled)
+            return {nullptr, nullptr};
+        auto& ent = e_[name];
+        if((ent.calls++ % (sample > 0 ? sample : 1)) != 0)
+            return {nullptr, nullptr};
+        auto p = get();
+        ent.pending.push_back(p);
+        return p;
+    }
     void end(const char* name, hipStream_t s)
     {
         if(!enabled)
@@ -250,6 +266,22 @@ inline unsigned grid_for(long long n, int block)
 {
     long long g = (n + block - 1) / block;
     return (unsigned)(g < 1 ? 1 : g);
+}
+
+// The command processor hands workgroups of a 1-D grid to the 8 XCDs round
+// robin (workgroup b to XCD b % 8), and each XCD has its own L2.  This maps
+// the hardware workgroup id to a logical one so that each XCD works on one
+// contiguous range of logical workgroups: XCD k takes logical ids
+// [k q + min(k, r), (k + 1) q + min(k + 1, r)) with q = G / 8, r = G % 8,
+// a bijection of [0, G).  Kernels whose neighbouring rows share operands
+// (SpMV's x gathers) then read each operand into one L2 instead of eight.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
+{
+    unsigned q = G / kXcds, r = G % kXcds;
+    unsigned k = b % kXcds, i = b / kXcds;
+    return k * q + (k < r ? k : r) + i;
 }
 
 // ---------------------------------------------------------------- wave ops

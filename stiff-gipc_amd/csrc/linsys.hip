@@ -247,14 +247,54 @@ __global__ void __launch_bounds__(256)
             return;
         }
     }
+    // contiguous row ranges per XCD: a row's neighbours (x gathers) are
+    // mostly rows of the same range, so they hit that XCD's L2
+    int  gid  = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int  row  = gid / kRowLanes;
+    int  g    = gid % kRowLanes;
+    bool live = row < nb;
+    // independent loads first: row extents (static and contact rows) and the
+    // two slot sets beta is formed from
+    int k0 = 0, k1 = 0, q0 = 0, q1 = 0;
+    if(live)
+    {
+        k0 = rowptr[row];
+        k1 = rowptr[row + 1];
+        if(rp2)
+        {
+            q0 = rp2[row];
+            q1 = rp2[row + 1];
+        }
+    }
+    const V* pp = (MODE == MODE_PCG && pprev) ? pprev : x;
+    // the row's own p_i (stored by the row owner, and its p.Ap term): raw
+    // loads issued before the passes, combined after them
+    const bool own = live && g == 0 && (MODE == MODE_PCG || (MODE == MODE_PLAIN && dot));
+    V          ox[3] = {0, 0, 0}, op[3] = {0, 0, 0};
+    if(own)
+    {
+#pragma unroll
+        for(int c = 0; c < 3; c++)
+        {
+            ox[c] = x[3 * (size_t)row + c];
+            if(MODE == MODE_PCG)
+                op[c] = pp[3 * (size_t)row + c];
+        }
+    }
+    // MODE_PCG: p_j = z_j + beta p_prev_j; on the first iteration (no p_prev)
+    // beta = 0 and p_prev aliases z, so fma(0, z_j, z_j) = z_j and every
+    // gather takes the same branch-free path
     V beta = 0;
     if(MODE == MODE_PCG && pprev)
-        beta = (V)(slot_set_sum(rz_a) / slot_set_sum(rz_b));
+    {
+        double sa = rz_a[__lane_id() * kSlotStride], sb = rz_b[__lane_id() * kSlotStride];
+        beta      = (V)(wave_sum(sa) / wave_sum(sb));  // slot_set_sum of both sets
+    }
     auto fetch = [&](int j, V& x0, V& x1, V& x2) {
         const V* xj = x + 3 * (size_t)j;
-        if(MODE == MODE_PCG && pprev)
+        if(MODE == MODE_PCG)
         {
-            const V* pj = pprev + 3 * (size_t)j;
+            const V* pj = pp + 3 * (size_t)j;
             x0          = fma(beta, pj[0], xj[0]);
             x1          = fma(beta, pj[1], xj[1]);
             x2          = fma(beta, pj[2], xj[2]);
@@ -266,36 +306,50 @@ __global__ void __launch_bounds__(256)
             x2 = xj[2];
         }
     };
-    int  gid  = blockIdx.x * blockDim.x + threadIdx.x;
-    int  row  = gid / kRowLanes;
-    int  g    = gid % kRowLanes;
-    bool live = row < nb;
-    V    a0 = 0, a1 = 0, a2 = 0;
+    V a0 = 0, a1 = 0, a2 = 0;
+    // A lane takes blocks k, k + 8, k + 16, ... of its row, two per pass: both
+    // column ids and both blocks are loaded together, then both gathers, so
+    // a pass costs two memory round trips.  Accumulation order is the
+    // block order k, k + 8, ... as with one block per pass.
+    auto row_pass = [&](const int* __restrict__ cl, const V* __restrict__ vl, int lo, int hi) {
+        for(int k = lo + g; k < hi; k += 2 * kRowLanes)
+        {
+            const int  kb  = k + kRowLanes;
+            const bool two = kb < hi;
+            const int  kk  = two ? kb : k;  // in range either way
+            const int  ca  = cl[k];
+            const int  cb  = cl[kk];
+            V          A[9], B[9];
+            const V*   pa = vl + 9 * (size_t)k;
+            const V*   pb = vl + 9 * (size_t)kk;
+#pragma unroll
+            for(int q = 0; q < 9; q++)
+            {
+                A[q] = pa[q];
+                B[q] = pb[q];
+            }
+            // keep the block loads ahead of the gathers (the scheduler would
+            // otherwise issue them after the column ids have returned)
+            __builtin_amdgcn_sched_barrier(0);
+            V x0, x1, x2, y0, y1, y2;
+            fetch(ca, x0, x1, x2);
+            fetch(cb, y0, y1, y2);
+#pragma unroll
+            for(int q = 0; q < 9; q++)
+                B[q] = two ? B[q] : (V)0;  // a zero block adds exactly +0
+            a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+            a1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+            a2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+            a0 += B[0] * y0 + B[1] * y1 + B[2] * y2;
+            a1 += B[3] * y0 + B[4] * y1 + B[5] * y2;
+            a2 += B[6] * y0 + B[7] * y1 + B[8] * y2;
+        }
+    };
     if(live)
     {
-        int k1 = rowptr[row + 1];
-        for(int k = rowptr[row] + g; k < k1; k += kRowLanes)
-        {
-            const V* B = vals + 9 * (size_t)k;
-            V        x0, x1, x2;
-            fetch(cols[k], x0, x1, x2);
-            a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
-            a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
-            a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
-        }
+        row_pass(cols, vals, k0, k1);
         if(rp2)
-        {
-            int q1 = rp2[row + 1];
-            for(int k = rp2[row] + g; k < q1; k += kRowLanes)
-            {
-                const V* B = vals2 + 9 * (size_t)k;
-                V        x0, x1, x2;
-                fetch(cols2[k], x0, x1, x2);
-                a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
-                a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
-                a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
-            }
-        }
+            row_pass(cols2, vals2, q0, q1);
     }
 #pragma unroll
     for(int o = kRowLanes / 2; o > 0; o >>= 1)
@@ -319,10 +373,15 @@ __global__ void __launch_bounds__(256)
             y[i + 0] = a0;
             y[i + 1] = a1;
             y[i + 2] = a2;
-            if(MODE == MODE_PCG || dot)
+            if(own)
             {
-                V x0, x1, x2;
-                fetch(row, x0, x1, x2);
+                V x0 = ox[0], x1 = ox[1], x2 = ox[2];
+                if(MODE == MODE_PCG)
+                {
+                    x0 = fma(beta, op[0], ox[0]);
+                    x1 = fma(beta, op[1], ox[1]);
+                    x2 = fma(beta, op[2], ox[2]);
+                }
                 if(MODE == MODE_PCG)
                 {
                     pcur[i + 0] = x0;
@@ -371,16 +430,17 @@ static inline void contact_rows(const SysView& S, const int*& rp2, const int*& c
 template <int MODE, typename V>
 static void launch_spmv(const SysView& S, const V* x, const V* pprev, V* pcur, const V* b, V* y,
                         const double* rz_a, const double* rz_b, double* dot, const PcgCtl& pc,
-                        hipStream_t s)
+                        hipStream_t s, std::pair<hipEvent_t, hipEvent_t> ev = {nullptr, nullptr})
 {
     const Bsr& A = *S.A;
     const int* rp2;
     const int* c2;
     const V*   v2;
     contact_rows<V>(S, rp2, c2, v2);
-    hipLaunchKernelGGL((k_spmv<MODE, V>), dim3(grid_for((long long)A.nb * kRowLanes, 256)),
-                       dim3(256), 0, s, A.nb, A.rowptr.data(), A.cols.data(),
-                       MatVals<V>::a(S), rp2, c2, v2, x, pprev, pcur, b, y, rz_a, rz_b, dot, pc);
+    hipExtLaunchKernelGGL((k_spmv<MODE, V>), dim3(grid_for((long long)A.nb * kRowLanes, 256)),
+                          dim3(256), 0, s, ev.first, ev.second, 0, A.nb, A.rowptr.data(),
+                          A.cols.data(), MatVals<V>::a(S), rp2, c2, v2, x, pprev, pcur, b, y, rz_a,
+                          rz_b, dot, pc);
     SG_LAUNCH_CHECK();
 }
 
@@ -777,12 +837,10 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         }
         V*       pcur  = W.p[k & 1].data();
         const V* pprev = (k == 1) ? nullptr : W.p[(k - 1) & 1].data();
-        if(prof)
-            prof->begin("spmv", s);
         launch_spmv<MODE_PCG, V>(S, W.z.data(), pprev, pcur, nullptr, W.Ap.data(), rz_set(k - 1),
-                                 k >= 2 ? rz_set(k - 2) : rz_set(k - 1), pap_set(k), pcx, s);
-        if(prof)
-            prof->end("spmv", s);
+                                 k >= 2 ? rz_set(k - 2) : rz_set(k - 1), pap_set(k), pcx, s,
+                                 prof ? prof->kernel_events("spmv")
+                                      : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr});
         if(test)
         {
             SG_HIP(hipMemcpyAsync(h_ctl_ + 4 * slot, ctl_.data(), 2 * sizeof(int),

@@ -389,20 +389,32 @@ __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B,
 // overlap; every lane stores unconditionally so the compiler keeps the loads
 // together instead of sinking them into a guarded block.
 template <typename T>
-__device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T rv)
+__device__ __forceinline__ void bank_load(const T* __restrict__ inv, int gb, T (&pv)[BD])
 {
     const int t  = __lane_id();
     const int tc = t < BD ? t : 0;
     const T*  P  = inv + (size_t)gb * BD * BD + tc;
-    T         pv[BD];
 #pragma unroll
     for(int u = 0; u < BD; u++)
         pv[u] = P[u * BD];
+}
+
+template <typename T>
+__device__ __forceinline__ T bank_apply(const T (&pv)[BD], T rv)
+{
     T acc = 0;
 #pragma unroll
     for(int u = 0; u < BD; u++)
         acc += pv[u] * __shfl(rv, u, 64);
     return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T rv)
+{
+    T pv[BD];
+    bank_load(inv, gb, pv);
+    return bank_apply(pv, rv);
 }
 
 // Same product with R_u read from LDS (rs[0..48), a broadcast read per u).
@@ -428,25 +440,45 @@ __device__ __forceinline__ T bank_product_lds(const T* __restrict__ inv, int gb,
 // kernel two levels up reads them after this launch).  rv: lane t < 48 holds
 // component t % 3 of bank node t / 3; node_up: global id of that node's
 // parent (or -1).
+// Ordering of LDS between the lanes of a single-wave workgroup: the LDS
+// instructions of one wave execute in program order, so a compiler-only
+// wave barrier suffices; __syncthreads would also make the wave wait for its
+// outstanding global stores (workgroup-scope release).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+}
+
+// up2_node: going_next[first + t / 3] for t < 3 ncl when up2 (the caller may
+// have loaded it early), else unused.  Single-wave workgroups only.
 template <typename T>
 __device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, int first,
-                                              int ncl, const int* __restrict__ going_next,
-                                              T* __restrict__ R, bool up2)
+                                              int ncl, int up2_node, T* __restrict__ R, bool up2)
 {
     const int t = __lane_id();
     if(t < BD)
         Rc[t] = (T)0;
-    __syncthreads();
+    wave_lds_sync();
     if(node_up >= 0)
         atomicAdd(&Rc[3 * (node_up - first) + t % 3], rv);
-    __syncthreads();
+    wave_lds_sync();
     if(t < 3 * ncl)
     {
         T v = Rc[t];
         R[3 * (size_t)first + t] = v;
         if(up2)
-            atomicAdd(&R[3 * (size_t)going_next[first + t / 3] + t % 3], v);
+            atomicAdd(&R[3 * (size_t)up2_node + t % 3], v);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, int first,
+                                              int ncl, const int* __restrict__ going_next,
+                                              T* __restrict__ R, bool up2)
+{
+    const int t   = __lane_id();
+    int       up2n = (up2 && t < 3 * ncl) ? going_next[first + t / 3] : 0;
+    restrict_bank(Rc, rv, node_up, first, ncl, up2n, R, up2);
 }
 
 // Level-0 banks (one wave each): optional PCG update of x and r
@@ -470,33 +502,71 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
     if(done && *done)
         return;
     __shared__ T Rc[BD];
-    double a = 0.0;
-    if(x)
-        a = slot_set_sum(rz_prev) / slot_set_sum(pap);
     const int b  = blockIdx.x;
     const int t  = threadIdx.x;
     const int s  = b * BANK + t / 3;
     const int cc = t % 3;
-    int       v  = (t < BD) ? vert_of_slot[s] : -1;
-    V         rv = 0;
+    // issue order: the small loads the vector update depends on, then the
+    // bank's inverse, so its fetch overlaps the dependent chain
+    // vert_of_slot -> (x, r, p, Ap) -> stores
+    int    v  = (t < BD) ? vert_of_slot[s] : -1;
+    double ra = 0.0, pa = 0.0;
+    if(x)
+    {
+        ra = rz_prev[t * kSlotStride];
+        pa = pap[t * kSlotStride];
+    }
+    // restriction targets: this node's parent and (up2) the parents of the
+    // bank's clusters, independent of everything before the restriction
+    // (loads are unconditional from a valid fallback address, bank_ncl + b,
+    // so no branch joins force an early wait on them)
+    int first = 0, ncl = 0;
+    if(has_coarse)
+    {
+        first = bank_first[b];
+        ncl   = bank_ncl[b];
+    }
+    const int gn = *((has_coarse && t < BD) ? going_next + s : bank_ncl + b);
+    T         pv[BD];
+    bank_load(inv, b, pv);
+    // after the inverse loads: the address waits for `first`
+    const int up2n =
+        *((has_coarse && up2 && t < 3 * ncl) ? going_next + first + t / 3 : bank_ncl + b);
+    double a = 0.0;
+    if(x)
+        a = wave_sum(ra) / wave_sum(pa);  // slot_set_sum of both sets
+    // the updated x and r are stored after the product, so no wait for a
+    // store acknowledgement sits between the loads and the product
+    V      rv = 0, xn = 0;
+    size_t d  = 3 * (size_t)(v >= 0 ? v : 0) + cc;
     if(v >= 0)
     {
-        size_t d  = 3 * (size_t)v + cc;
-        V      av = (V)a;
-        rv        = r[d];
+        V av = (V)a;
+        rv   = r[d];
         if(x)
         {
-            x[d] += av * p[d];
+            xn = x[d] + av * p[d];
             rv -= av * Ap[d];
-            r[d] = rv;
         }
     }
-    T rt = (T)rv;
-    T z  = bank_product(inv, b, rt);
+    // z = Binv r with r broadcast from LDS (12 ds_read_b128 instead of 48
+    // cross-lane permutes; same products, same order)
+    __shared__ T Rs[64];
+    T rt  = (T)rv;
+    Rs[t] = rt;
+    wave_lds_sync();
+    T z = 0;
+#pragma unroll
+    for(int u = 0; u < BD; u++)
+        z += pv[u] * Rs[u];
     Z[(size_t)b * ZB + t] = z;
+    if(x && v >= 0)
+    {
+        x[d] = xn;
+        r[d] = rv;
+    }
     if(has_coarse)
-        restrict_bank(Rc, rt, v >= 0 ? going_next[s] : -1, bank_first[b], bank_ncl[b],
-                      going_next, R, up2 != 0);
+        restrict_bank(Rc, rt, v >= 0 ? gn : -1, first, ncl, up2n, R, up2 != 0);
 }
 
 // Banks of one coarse level, one wave each: z = Binv R and the next level's
@@ -604,9 +674,14 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
     }
 }
 
+constexpr int kChainFast = 8;  // hierarchies up to 9 levels take an unrolled gather
+
 // z_v = Z[slot(v)] + sum_l Z[chain_l(v)];  r.z into a slot set; workgroup 0
-// clears two slot sets for later iterations
-template <typename T, typename V>
+// clears two slot sets for later iterations.  NL = L - 1 chain entries per
+// node, known at compile time for NL <= kChainFast (every chain id is
+// loaded, then every gather issued: two memory round trips instead of two
+// per level; same summation order), or -1 for a runtime count.
+template <typename T, typename V, int NL>
 __global__ void __launch_bounds__(256)
     k_mas_collect(int nv, int L, const int* __restrict__ slot_of, const int* __restrict__ chain,
                   const T* __restrict__ Z, const V* __restrict__ r, V* __restrict__ z,
@@ -621,23 +696,61 @@ __global__ void __launch_bounds__(256)
     double d = 0.0;
     if(v < nv)
     {
-        int      s  = slot_of[v];
-        const T* zs = Z + (size_t)(s >> 4) * ZB + 3 * (s & 15);
-        T        z0 = zs[0], z1 = zs[1], z2 = zs[2];
-        for(int l = 1; l < L; l++)
+        int        s  = slot_of[v];
+        const T*   zs = Z + (size_t)(s >> 4) * ZB + 3 * (s & 15);
+        const int* ch = chain + (size_t)v * (L - 1);
+        T          z0, z1, z2;
+        double     r0 = 0.0, r1 = 0.0, r2 = 0.0;  // loaded with the ids, not after the stores
+        if(rz_out)
         {
-            int      g  = chain[(size_t)v * (L - 1) + l - 1];
-            const T* zg = Z + (size_t)(g >> 4) * ZB + 3 * (g & 15);
-            z0 += zg[0];
-            z1 += zg[1];
-            z2 += zg[2];
+            r0 = (double)r[3 * (size_t)v];
+            r1 = (double)r[3 * (size_t)v + 1];
+            r2 = (double)r[3 * (size_t)v + 2];
+        }
+        if constexpr(NL >= 0)
+        {
+            int g[NL > 0 ? NL : 1];
+#pragma unroll
+            for(int l = 0; l < NL; l++)
+                g[l] = ch[l];
+            z0 = zs[0];
+            z1 = zs[1];
+            z2 = zs[2];
+            T zz[NL > 0 ? NL : 1][3];
+#pragma unroll
+            for(int l = 0; l < NL; l++)
+            {
+                const T* zg = Z + (size_t)(g[l] >> 4) * ZB + 3 * (g[l] & 15);
+                zz[l][0]    = zg[0];
+                zz[l][1]    = zg[1];
+                zz[l][2]    = zg[2];
+            }
+#pragma unroll
+            for(int l = 0; l < NL; l++)
+            {
+                z0 += zz[l][0];
+                z1 += zz[l][1];
+                z2 += zz[l][2];
+            }
+        }
+        else
+        {
+            z0 = zs[0];
+            z1 = zs[1];
+            z2 = zs[2];
+            for(int l = 0; l < L - 1; l++)
+            {
+                const T* zg = Z + (size_t)(ch[l] >> 4) * ZB + 3 * (ch[l] & 15);
+                z0 += zg[0];
+                z1 += zg[1];
+                z2 += zg[2];
+            }
         }
         z[3 * (size_t)v]     = (V)z0;
         z[3 * (size_t)v + 1] = (V)z1;
         z[3 * (size_t)v + 2] = (V)z2;
         if(rz_out)
-            d = (double)r[3 * (size_t)v] * (double)z0 + (double)r[3 * (size_t)v + 1] * (double)z1
-                + (double)r[3 * (size_t)v + 2] * (double)z2;
+            d = r0 * (double)z0 + r1 * (double)z1 + r2 * (double)z2;
     }
     if(rz_out)
         slot_set_add(rz_out, d);
@@ -866,13 +979,11 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
     const int up2_level = (lb >= 1 && lt < L) ? lb - 1 : -1;  // level whose kernel feeds R_lt
     T*        clr       = nullptr;                            // R_lt, cleared by collect
     int       nclr      = 0;
-    if(prof)
-        prof->begin("mas_apply_l0", s);
-    hipLaunchKernelGGL((k_mas_l0<T, V>), dim3(nparts), dim3(64), 0, s, vert_of_slot, x, r, p, Ap,
-                       rz_prev, pap, inv, going_next, bank_first, bank_ncl, R, Z, L > 1 ? 1 : 0,
-                       up2_level == 0 ? 1 : 0, done);
-    if(prof)
-        prof->end("mas_apply_l0", s);
+    auto ev = prof ? prof->kernel_events("mas_apply_l0")
+                   : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    hipExtLaunchKernelGGL((k_mas_l0<T, V>), dim3(nparts), dim3(64), 0, s, ev.first, ev.second, 0,
+                          vert_of_slot, x, r, p, Ap, rz_prev, pap, inv, going_next, bank_first,
+                          bank_ncl, R, Z, L > 1 ? 1 : 0, up2_level == 0 ? 1 : 0, done);
     SG_LAUNCH_CHECK();
     if(L > 1)
     {
@@ -916,8 +1027,24 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         }
     }
     SG_CHECK(nclr <= (long long)grid_for(nv, 256) * 256, -1, "MAS: tail level larger than the vertex count");
-    hipLaunchKernelGGL((k_mas_collect<T, V>), dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L,
-                       slot_of, chain, Z, r, z, rz_out, zero0, zero1, clr, nclr, done);
+    auto collect = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L, slot_of, chain,
+                           Z, r, z, rz_out, zero0, zero1, clr, nclr, done);
+    };
+    static_assert(kChainFast == 8, "collect dispatch below covers NL = 0 .. 8");
+    switch(L - 1)
+    {
+    case 0: collect(k_mas_collect<T, V, 0>); break;
+    case 1: collect(k_mas_collect<T, V, 1>); break;
+    case 2: collect(k_mas_collect<T, V, 2>); break;
+    case 3: collect(k_mas_collect<T, V, 3>); break;
+    case 4: collect(k_mas_collect<T, V, 4>); break;
+    case 5: collect(k_mas_collect<T, V, 5>); break;
+    case 6: collect(k_mas_collect<T, V, 6>); break;
+    case 7: collect(k_mas_collect<T, V, 7>); break;
+    case 8: collect(k_mas_collect<T, V, 8>); break;
+    default: collect(k_mas_collect<T, V, -1>); break;
+    }
     SG_LAUNCH_CHECK();
 }
 
