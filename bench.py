@@ -54,6 +54,23 @@ def algorithmic_bytes(name, sz, nparts):
     return 0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a
+    plain copy); None when absent."""
+    path = os.path.join(ROOT, "profiles", "spmv_64cubes_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    parts = line.split(":", 1)[1].split()
+                    if len(parts) == 2 and parts[0] == kernel:
+                        return int(parts[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,8 +216,11 @@ def main():
         avg_s = tot / n * 1e-3
         byt = algorithmic_bytes(dom, sz, nparts)
         ach = byt / avg_s / 1e9
+        default_workload = (args.n_side, args.cells, args.settle) == (4, 8, 40)
+        traffic = measured_traffic(dom) if default_workload else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/spmv_64cubes_hbm_traffic.md" if traffic else None,
                 "algorithmic_bytes_per_launch": int(byt), "avg_launch_us": round(avg_s * 1e6, 3),
                 "launches": n}
 

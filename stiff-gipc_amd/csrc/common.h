@@ -389,4 +389,29 @@ __device__ __forceinline__ unsigned wave_append(unsigned* counter, bool pred)
     return base + (unsigned)__popcll(lt);
 }
 
+// wave-aggregated append into per-key counters: one atomic per distinct key
+// in the wave (lanes with key < 0 take no slot); returns this lane's slot
+// under counters[key]
+__device__ __forceinline__ unsigned wave_append_keyed(unsigned* counters, int key)
+{
+    const unsigned           lane = __lane_id();
+    const unsigned long long lt   = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    unsigned                 mine = 0;
+    unsigned long long       todo = __ballot(key >= 0);
+    while(todo)
+    {
+        int                leader = __ffsll((long long)todo) - 1;
+        int                k      = __shfl(key, leader, 64);
+        unsigned long long m      = __ballot(key == k) & todo;
+        unsigned           base   = 0;
+        if((int)lane == leader)
+            base = atomicAdd(counters + k, (unsigned)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if(key == k)
+            mine = base + (unsigned)__popcll(m & lt);
+        todo &= ~m;
+    }
+    return mine;
+}
+
 }  // namespace sgipc

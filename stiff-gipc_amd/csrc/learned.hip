@@ -20,6 +20,8 @@
 // affine fit of the converged Newton direction on the aggregate.
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/sgipc.h"
 #include "learned.h"
 
@@ -41,8 +43,6 @@ __global__ void k_agg_of(int nn, int nfem, const int* __restrict__ btype, int L,
                          unsigned* __restrict__ cnt)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if(v >= nn)
-        return;
     int c = -1;
     if(v < nfem && btype[v] == 0)
     {
@@ -50,20 +50,20 @@ __global__ void k_agg_of(int nn, int nfem, const int* __restrict__ btype, int L,
         if(c < 0 || c >= nc)
             c = -1;
     }
-    agg[v] = c;
-    if(c >= 0)
-        atomicAdd(cnt + c, 1u);
+    if(v < nn)
+        agg[v] = c;
+    // few coarsest aggregates: per-node atomics on them would serialise
+    wave_append_keyed(cnt, c);
 }
 
 __global__ void k_agg_fill(int nn, const int* __restrict__ agg, const int* __restrict__ ptr,
                            unsigned* __restrict__ cur, int* __restrict__ nodes)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if(v >= nn)
-        return;
-    int c = agg[v];
+    int      v    = blockIdx.x * blockDim.x + threadIdx.x;
+    int      c    = v < nn ? agg[v] : -1;
+    unsigned slot = wave_append_keyed(cur, c);
     if(c >= 0)
-        nodes[ptr[c] + (int)atomicAdd(cur + c, 1u)] = v;
+        nodes[ptr[c] + (int)slot] = v;
 }
 
 // one 64-lane workgroup per aggregate: centroid of its nodes
@@ -120,18 +120,21 @@ __device__ __forceinline__ void acc_jaj(double (&U)[78], const double* A, const 
                 }
 }
 
-// one 64-lane workgroup per aggregate: Galerkin matrix P^T A P (upper 78) and
-// P^T b over the blocks of the static pattern and the contact rows whose
-// column node is in the same aggregate
+// Galerkin matrix P^T A P (upper 78) and P^T b over the blocks of the static
+// pattern and the contact rows whose column node is in the same aggregate.
+// G single-wave workgroups per aggregate (grid nc * G), one node per lane per
+// pass; each workgroup's 90 partial sums are reduced through LDS (one row
+// per lane, fixed order) and added into Ac / rc, which start at zero.
 __global__ void __launch_bounds__(64) k_agg_galerkin(
-    const int* __restrict__ ptr, const int* __restrict__ nodes, const int* __restrict__ agg,
+    int G, const int* __restrict__ ptr, const int* __restrict__ nodes, const int* __restrict__ agg,
     const double* __restrict__ x, const double* __restrict__ cen,
     const int* __restrict__ rowptr, const int* __restrict__ cols, const double* __restrict__ vals,
     const int* __restrict__ crp, const int* __restrict__ crc, const double* __restrict__ crv,
     const double* __restrict__ b, double* __restrict__ Ac, double* __restrict__ rc)
 {
-    int    c = blockIdx.x;
-    double U[78], r[12];
+    __shared__ double red[90][65];
+    const int c = blockIdx.x / G, gpart = blockIdx.x % G;
+    double    U[78], r[12];
 #pragma unroll
     for(int k = 0; k < 78; k++)
         U[k] = 0.0;
@@ -139,7 +142,7 @@ __global__ void __launch_bounds__(64) k_agg_galerkin(
     for(int k = 0; k < 12; k++)
         r[k] = 0.0;
     double cx = cen[3 * c], cy = cen[3 * c + 1], cz = cen[3 * c + 2];
-    for(int k = ptr[c] + (int)threadIdx.x; k < ptr[c + 1]; k += 64)
+    for(int k = ptr[c] + gpart * 64 + (int)threadIdx.x; k < ptr[c + 1]; k += 64 * G)
     {
         int    i     = nodes[k];
         double xi[3] = {x[3 * (size_t)i] - cx, x[3 * (size_t)i + 1] - cy, x[3 * (size_t)i + 2] - cz};
@@ -169,19 +172,21 @@ __global__ void __launch_bounds__(64) k_agg_galerkin(
             }
         }
     }
+    const int t = threadIdx.x;
 #pragma unroll
     for(int k = 0; k < 78; k++)
-    {
-        double t = wave_sum(U[k]);
-        if(threadIdx.x == 0)
-            Ac[78 * (size_t)c + k] = t;
-    }
+        red[k][t] = U[k];
 #pragma unroll
     for(int k = 0; k < 12; k++)
+        red[78 + k][t] = r[k];
+    __builtin_amdgcn_wave_barrier();  // single wave: LDS accesses stay in order
+    for(int row = t; row < 90; row += 64)
     {
-        double t = wave_sum(r[k]);
-        if(threadIdx.x == 0)
-            rc[12 * (size_t)c + k] = t;
+        double acc = 0.0;
+        for(int l = 0; l < 64; l++)
+            acc += red[row][l];
+        if(acc != 0.0)
+            atomicAdd(row < 78 ? Ac + 78 * (size_t)c + row : rc + 12 * (size_t)c + (row - 78), acc);
     }
 }
 
@@ -692,10 +697,18 @@ void Coarse::build(int nn, int nfem, const int* btype, const double* x, const Bs
     hipLaunchKernelGGL(k_agg_centroid, dim3(nc_), dim3(64), 0, s, ptr_.data(), nodes_.data(), x,
                        cen_.data());
     bool rows = coo && coo->rows_valid;
-    hipLaunchKernelGGL(k_agg_galerkin, dim3(nc_), dim3(64), 0, s, ptr_.data(), nodes_.data(),
-                       agg_.data(), x, cen_.data(), A.rowptr.data(), A.cols.data(), A.vals.data(),
-                       rows ? coo->rp.data() : nullptr, rows ? coo->rc.data() : nullptr,
-                       rows ? coo->rv.data() : nullptr, b, Ac_.data(), rc_.data());
+    // enough workgroups per aggregate for about one node per lane
+    unsigned hmax = 1;
+    for(int c = 0; c < nc_; c++)
+        hmax = std::max(hmax, h[c]);
+    int G = (int)std::min<unsigned>(64u, (hmax + 63u) / 64u);
+    SG_HIP(hipMemsetAsync(Ac_.data(), 0, sizeof(double) * 78 * (size_t)nc_, s));
+    SG_HIP(hipMemsetAsync(rc_.data(), 0, sizeof(double) * 12 * (size_t)nc_, s));
+    hipLaunchKernelGGL(k_agg_galerkin, dim3(nc_ * G), dim3(64), 0, s, G, ptr_.data(),
+                       nodes_.data(), agg_.data(), x, cen_.data(), A.rowptr.data(), A.cols.data(),
+                       A.vals.data(), rows ? coo->rp.data() : nullptr,
+                       rows ? coo->rc.data() : nullptr, rows ? coo->rv.data() : nullptr, b,
+                       Ac_.data(), rc_.data());
     hipLaunchKernelGGL(k_agg_solve, dim3(grid_for(nc_, 64)), dim3(64), 0, s, nc_, Ac_.data(),
                        rc_.data(), qc_.data(), feat_.data(), scale_.data());
     SG_LAUNCH_CHECK();
