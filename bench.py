@@ -76,7 +76,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=3, help="untimed Newton iterations")
-    ap.add_argument("--settle", type=int, default=40, help="untimed time steps before warmup")
+    ap.add_argument("--workload", default="cubes64", choices=["cubes64", "hybrid", "cloth_abd"],
+                    help="cubes64: 64 tet cubes (the default bench line); hybrid: 512 affine "
+                         "boxes fired into a 983k-tet bed; cloth_abd: 256x256 cloth over 16 "
+                         "affine boxes")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed time steps before warmup (default 40 / 4 / 60 by workload)")
     ap.add_argument("--n-side", type=int, default=4)
     ap.add_argument("--cells", type=int, default=8)
     ap.add_argument("--mas-fp64", action="store_true", help="all-FP64 MAS instead of FP32 apply")
@@ -99,8 +104,27 @@ def main():
     from sgipc import scene as S
 
     log("device:", sgipc.device_info())
-    sc = S.cubes_scene(n_side=args.n_side, cells=args.cells, size=0.2, spacing=0.6, fall=0.5,
-                       seed=0)
+    if args.workload == "hybrid":
+        sc = S.hybrid_scene()
+        settle_default = 4
+        workload = (f"512 affine boxes (6^3 cells, 1,296 tets each, kappa 1e11) at 10 m/s into "
+                    f"a 64x64x40-cell bed (983,040 tets, E 5e4 Pa, bottom layer fixed); "
+                    f"{sc.n_tets} tets, {sc.n_verts} verts; dt 0.005, dhat 5e-4, mu 0.3")
+    elif args.workload == "cloth_abd":
+        sc = S.cloth_scene(n=256, size=1.0, height=0.3, n_boxes=16, box_size=0.1, box_cells=3,
+                           E=1e6, bend_stiffness=1e-4, thickness=1e-3, abd_boxes=True,
+                           abd_kappa=1e11)
+        settle_default = 60
+        workload = (f"cloth sheet 256x256 verts ({sc.tris.shape[0]} triangles, 1 m), membrane "
+                    f"E 1e6 Pa, bending 1e-4, thickness 1e-3, dropped 0.3 m over 16 affine boxes "
+                    f"(kappa 1e11); dt 0.01, dhat 1e-3")
+    else:
+        sc = S.cubes_scene(n_side=args.n_side, cells=args.cells, size=0.2, spacing=0.6, fall=0.5,
+                           seed=0)
+        settle_default = 40
+        workload = None
+    if args.settle is None:
+        args.settle = settle_default
     sc.mas_precision = sgipc.SGIPC_MAS_FP64 if args.mas_fp64 else sgipc.SGIPC_MAS_FP32
     sc.pcg_rel_tol = 1e-4
     nparts = int(sc.part_id.max()) + 1
@@ -114,7 +138,7 @@ def main():
     t0 = time.time()
     for k in range(args.settle):
         st = sim.time_step()
-        if k % 10 == 9 or k == args.settle - 1:
+        if k % 10 == 9 or k == args.settle - 1 or args.settle <= 10 or args.workload != "cubes64":
             x = sim.positions()
             log(f"settle step {k + 1}/{args.settle}: newton={st['newton_iterations']} "
                 f"pcg={st['pcg_iterations']} ymin={x[:, 1].min():.4f} "
@@ -216,7 +240,8 @@ def main():
         avg_s = tot / n * 1e-3
         byt = algorithmic_bytes(dom, sz, nparts)
         ach = byt / avg_s / 1e9
-        default_workload = (args.n_side, args.cells, args.settle) == (4, 8, 40)
+        default_workload = (args.workload, args.n_side, args.cells, args.settle) == (
+            "cubes64", 4, 8, 40)
         traffic = measured_traffic(dom) if default_workload else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -226,7 +251,7 @@ def main():
 
     # ---- CPU baseline: the oracle (single thread) from the same settled state
     cpu = None
-    if not args.no_cpu and args.cpu_iters > 0:
+    if not args.no_cpu and args.cpu_iters > 0 and args.workload == "cubes64":
         try:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle as O
@@ -261,10 +286,11 @@ def main():
         "modes": results,
         "learned_training": train,
         "config": {
-            "workload": f"{args.n_side ** 3} tet cubes of {args.cells}^3 hex cells "
-                        f"({sc.n_tets} tets, {sc.n_verts} verts), E log-uniform [1e5,1e9] Pa, "
-                        f"nu 0.4, 0.5 m drop onto ground, dt 0.01, dhat 1e-3, "
-                        f"{args.settle} settle steps",
+            "workload": (workload or
+                         f"{args.n_side ** 3} tet cubes of {args.cells}^3 hex cells "
+                         f"({sc.n_tets} tets, {sc.n_verts} verts), E log-uniform [1e5,1e9] Pa, "
+                         f"nu 0.4, 0.5 m drop onto ground, dt 0.01, dhat 1e-3")
+                        + f", {args.settle} settle steps",
             "mode": "FP64 PCG/SpMV, " + ("FP64" if args.mas_fp64 else "FP32") + " MAS apply",
             "pcg_rel_tol": 1e-4,
             "parallelism": "single GPU",

@@ -880,7 +880,10 @@ void Context::assemble(bool hessian)
     launch_rhs(dev_, rhs_.data(), s_);
     launch_abd_rhs(dev_, rhs_.data(), s_);
     if(hessian)
+    {
+        coo_.merge(s_);
         coo_.build_rows(s_);
+    }
 }
 
 double Context::energy()

@@ -68,6 +68,16 @@ struct Coo
     }
     // counting sort of the live entries by row (after every assembly)
     void build_rows(hipStream_t s);
+    // merge entries with equal (row, col) in place (sum of their blocks),
+    // count = number of distinct blocks; returns the count before merging
+    unsigned merge(hipStream_t s);
+    // open-addressing table of the merge (grown to twice the live count)
+    DBuf<unsigned long long> hkeys;  // P, empty = ~0
+    DBuf<double>             hvals;  // 9 * P, zero between merges
+    DBuf<int>                hlist;  // occupied slots, in insertion order
+    DBuf<unsigned>           hcount; // 1
+    DBuf<int>                hfail;  // 1: a probe sequence ran through the table
+    size_t                   hcap = 0;
 };
 
 // view of A = static BSR + COO tail

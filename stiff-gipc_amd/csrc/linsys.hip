@@ -157,6 +157,123 @@ __global__ void k_coo_scatter(const unsigned* __restrict__ count, unsigned cap,
         rv[9 * (size_t)pos + q] = v[9 * (size_t)t + q];
 }
 
+// ------------------------------------------------------------- merge
+// Contact blocks mapped to affine-body nodes repeat the same (row, col) many
+// times (every contact between two bodies lands on the same 16 node pairs).
+// The entries are merged by hashing (row, col) into an open-addressing table
+// with linear probing: the first thread to claim a slot records it in the
+// slot list, every thread adds its block with FP64 atomics; the slots are
+// then written back over the raw entries and cleared for the next merge.
+// This is the merge of the reference's triplet converter (converter.cu:23-189,
+// radix sort + segmented reduction) done in one pass without sorting.
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+__global__ void k_coo_hash(unsigned n, const int* __restrict__ r, const int* __restrict__ c,
+                           const double* __restrict__ v, unsigned long long* __restrict__ keys,
+                           double* __restrict__ vals, unsigned long long mask,
+                           int* __restrict__ list, unsigned* __restrict__ nlist,
+                           int* __restrict__ fail)
+{
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= n)
+        return;
+    unsigned long long key = ((unsigned long long)(unsigned)r[t] << 32) | (unsigned)c[t];
+    unsigned long long h   = mix64(key) & mask;
+    for(unsigned long long probe = 0; probe <= mask; probe++)
+    {
+        unsigned long long prev = atomicCAS(keys + h, kEmptyKey, key);
+        if(prev == kEmptyKey)
+            list[atomicAdd(nlist, 1u)] = (int)h;
+        if(prev == kEmptyKey || prev == key)
+        {
+#pragma unroll
+            for(int q = 0; q < 9; q++)
+            {
+                double x = v[9 * (size_t)t + q];
+                if(x != 0.0)
+                    atomicAdd(vals + 9 * h + q, x);
+            }
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    *fail = 1;  // unreachable while the table is at most half full
+}
+
+// write the distinct blocks back over the raw entries and clear their slots
+__global__ void k_coo_unhash(const unsigned* __restrict__ nlist, const int* __restrict__ list,
+                             unsigned long long* __restrict__ keys, double* __restrict__ vals,
+                             int* __restrict__ r, int* __restrict__ c, double* __restrict__ v,
+                             unsigned* __restrict__ count)
+{
+    unsigned n = *nlist;
+    unsigned u = blockIdx.x * blockDim.x + threadIdx.x;
+    if(u == 0)
+        *count = n;
+    if(u >= n)
+        return;
+    int                h   = list[u];
+    unsigned long long key = keys[h];
+    r[u]                   = (int)(key >> 32);
+    c[u]                   = (int)(key & 0xffffffffu);
+#pragma unroll
+    for(int q = 0; q < 9; q++)
+    {
+        v[9 * (size_t)u + q]      = vals[9 * (size_t)h + q];
+        vals[9 * (size_t)h + q] = 0.0;
+    }
+    keys[h] = kEmptyKey;
+}
+
+unsigned Coo::merge(hipStream_t s)
+{
+    unsigned n = 0;
+    SG_HIP(hipMemcpyAsync(&n, count.data(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    n = n < (unsigned)cap ? n : (unsigned)cap;
+    if(n < 2)
+        return n;
+    size_t want = 1024;
+    while(want < 2 * (size_t)n)
+        want <<= 1;
+    if(want > hcap)
+    {
+        hkeys.resize(want);
+        hvals.resize(9 * want);
+        hlist.resize(want);
+        hcount.resize(1);
+        hfail.resize(1);
+        SG_HIP(hipMemsetAsync(hkeys.data(), 0xff, want * sizeof(unsigned long long), s));
+        SG_HIP(hipMemsetAsync(hvals.data(), 0, 9 * want * sizeof(double), s));
+        hcap = want;
+    }
+    SG_HIP(hipMemsetAsync(hcount.data(), 0, sizeof(unsigned), s));
+    SG_HIP(hipMemsetAsync(hfail.data(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_coo_hash, dim3(grid_for(n, 256)), dim3(256), 0, s, n, r.data(),
+                       c.data(), v.data(), hkeys.data(), hvals.data(),
+                       (unsigned long long)(hcap - 1), hlist.data(), hcount.data(), hfail.data());
+    SG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_coo_unhash, dim3(grid_for(n, 256)), dim3(256), 0, s, hcount.data(),
+                       hlist.data(), hkeys.data(), hvals.data(), r.data(), c.data(), v.data(),
+                       count.data());
+    SG_LAUNCH_CHECK();
+    int fail = 0;
+    SG_HIP(hipMemcpyAsync(&fail, hfail.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    SG_CHECK(!fail, -5, "contact block merge: hash table full");
+    return n;
+}
+
 void Coo::build_rows(hipStream_t s)
 {
     SG_CHECK(nb > 0, -5, "contact CSR: reserve() not called");

@@ -451,25 +451,41 @@ def stack_scene(n=2, cells=3, size=0.1, gap=0.5e-3, rot_deg=7.0, seed=0, E=1e6, 
 
 def cloth_scene(n=16, size=0.3, height=0.15, n_boxes=1, box_size=0.1, box_cells=2, seed=0,
                 E=1e6, nu=0.3, bend_stiffness=1e-4, thickness=1e-3, density=200.0,
-                strain_rate=0.1, box_E=1e9, box_rho=1000.0, fixed_boxes=True, **kw):
+                strain_rate=0.1, box_E=1e9, box_rho=1000.0, fixed_boxes=True, abd_boxes=False,
+                **kw):
     """Cloth sheet of n x n vertices (size metres square, 2 (n-1)^2 triangles)
     held horizontally at 'height' above n_boxes tet boxes of box_size metres
     resting on the ground at seeded random positions under the sheet (one
     box: centred).  Membrane E, nu; the bending stiffness k = E_b h^3 /
     (24 (1 - nu^2)) is set directly through E_b = 24 (1 - nu^2) k / h^3.
-    Boxes are fixed obstacles (btype 1) unless fixed_boxes is False."""
+    Boxes are fixed obstacles (btype 1) unless fixed_boxes is False; with
+    abd_boxes they are free affine bodies (kappa from abd_kappa)."""
     rng = np.random.default_rng(seed)
     xc, tc, pc = cloth_grid(n, size, origin=(-size / 2, height, -size / 2))
     xb0, tb0, pb0 = hex_grid((box_cells,) * 3, box_size / box_cells)
     xb0 = xb0 - xb0.mean(0)
     bodies = []
+    placed = []
     for k in range(n_boxes):  # n_boxes = 0: the sheet and the ground only
         if n_boxes == 1:
             ctr = np.array([0.0, 0.0, 0.0])
         else:
-            ctr = np.array([rng.uniform(-0.4, 0.4) * size, 0.0, rng.uniform(-0.4, 0.4) * size])
+            # seeded positions, redrawn until the (axis-aligned) boxes are at
+            # least 2 cm apart
+            for _ in range(10000):
+                ctr = np.array([rng.uniform(-0.4, 0.4) * size, 0.0,
+                                rng.uniform(-0.4, 0.4) * size])
+                if all(max(abs(ctr[0] - p[0]), abs(ctr[2] - p[2])) >= box_size + 0.02
+                       for p in placed):
+                    break
+            else:
+                raise ValueError("cloth_scene: no room for the boxes under the sheet")
+            placed.append(ctr)
         ctr[1] = box_size / 2 + 1e-3 * (1 + k % 3)
-        bodies.append((xb0 + ctr, tb0, pb0, box_E, 0.3, box_rho, 1 if fixed_boxes else 0))
+        if abd_boxes:
+            bodies.append((xb0 + ctr, tb0, pb0, box_E, 0.3, box_rho, 0, True))
+        else:
+            bodies.append((xb0 + ctr, tb0, pb0, box_E, 0.3, box_rho, 1 if fixed_boxes else 0))
     bend_E = 24.0 * (1.0 - nu * nu) * bend_stiffness / thickness ** 3
     defaults = dict(dt=0.01, dhat=1e-3, name=f"cloth{n}x{n}_{n_boxes}boxes", cloth_E=E,
                     cloth_nu=nu, cloth_bend_E=bend_E, cloth_thickness=thickness,
@@ -501,5 +517,52 @@ def abd_scene(n_abd=2, cells=2, size=0.1, gap=0.5e-3, fem_base=True, base_E=1e6,
     defaults = dict(dt=0.01, dhat=1e-3, name=f"abd{n_abd}", abd_kappa=abd_kappa)
     defaults.update(kw)
     sc = assemble_bodies(bodies, **defaults)
+    sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
+    return sc
+
+
+def hybrid_scene(n_side=8, box_cells=6, bed_cells=(64, 40, 64), bed_size=1.6, box_size=0.12,
+                 speed=10.0, gap=5e-3, bed_E=5e4, nu=0.4, rho=1000.0, abd_kappa=1e11, mu=0.3,
+                 max_tilt_deg=15.0, seed=0, **kw):
+    """Stiff/soft impact: n_side^3 affine boxes of box_cells^3 hex cells
+    (box_size metres) on an n_side^3 lattice over a soft tet bed of
+    bed_cells = (nx, ny, nz) hex cells (ny vertical, bed_size metres along x
+    and z, bottom layer fixed), all boxes moving down at 'speed' m/s, the
+    lowest layer 'gap' above the bed.  Boxes are tilted by seeded rotations
+    of at most max_tilt_deg; friction mu for self contact.  Defaults: 512
+    boxes of 1,296 tets into a 983,040-tet bed, dt = 0.005, dhat = 5e-4."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = bed_cells
+    h = bed_size / nx
+    xb, tb, pb = hex_grid(bed_cells, h, origin=(-bed_size / 2, 0.0, -bed_size / 2 * nz / nx))
+    bt_bed = (xb[:, 1] <= 1e-12).astype(np.int32)      # bottom layer fixed
+    top = ny * h
+    x0, t0, p0 = hex_grid((box_cells,) * 3, box_size / box_cells)
+    x0 = x0 - x0.mean(0)
+    pitch = bed_size / n_side
+    bodies = [(xb, tb, pb, bed_E, nu, rho, bt_bed)]
+    vel = [np.zeros_like(xb)]
+    for a in range(n_side):
+        for b in range(n_side):
+            for c in range(n_side):
+                ax = rng.normal(size=3)
+                ax /= np.linalg.norm(ax)
+                ang = np.deg2rad(rng.uniform(-max_tilt_deg, max_tilt_deg))
+                K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+                R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+                xr = x0 @ R.T
+                ctr = np.array([(a + 0.5) * pitch - bed_size / 2, 0.0,
+                                (c + 0.5) * pitch - bed_size / 2 * nz / nx])
+                ctr[1] = top + gap - xr[:, 1].min() + b * pitch
+                bodies.append((xr + ctr, t0, p0, 1e9, 0.3, rho, 0, True))
+                vel.append(np.tile([0.0, -speed, 0.0], (xr.shape[0], 1)))
+    defaults = dict(dt=0.005, dhat=5e-4, name=f"hybrid_{n_side ** 3}abd_bed{nx}x{ny}x{nz}",
+                    abd_kappa=abd_kappa, friction_mu=mu, ground_friction_mu=mu,
+                    use_ground=False)
+    defaults.update(kw)
+    sc = assemble_bodies(bodies, **defaults)
+    # assemble_bodies places FEM vertices first and affine bodies after, in
+    # body order, which is the order of `vel`
+    sc.v = np.concatenate(vel)
     sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)
     return sc

@@ -133,3 +133,63 @@ def test_abd_only_scene_free_fall():
         sim.time_step()
         drop = -9.8 * 0.01 ** 2 * n * (n + 1) / 2
         np.testing.assert_allclose(sim.positions(), xs + np.array([0.0, drop, 0.0]), atol=1e-10)
+
+
+def _hybrid():
+    import sgipc
+    from sgipc import scene as S
+    # 8 affine boxes (2^3 cells, kappa 1e11) fired at 2 m/s into a soft bed of
+    # 8x4x8 cells (E = 5e4, bottom layer fixed), friction mu = 0.3
+    sc = S.hybrid_scene(n_side=2, box_cells=2, bed_cells=(8, 4, 8), bed_size=0.4,
+                        box_size=0.08, speed=2.0, gap=2e-3)
+    sc.mas_precision = sgipc.SGIPC_MAS_FP64
+    return sc
+
+
+def test_hybrid_impact_newton_iterations_match(oracle):
+    """the first Newton iterations of the impact (ABD boxes, soft bed with a
+    fixed layer, friction) from the same state: iterates within 1e-8"""
+    import sgipc
+    sc = _hybrid()
+    sim = sgipc.Simulation(sc)
+    w = oracle.World(sc)
+    for it in range(3):
+        st = sim.newton_iteration()
+        so = w.newton_iteration()
+        xg, xo = sim.positions(), w.positions()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[hybrid-newton] it={it} pcg={st['pcg_iterations']}/{so['pcg_iterations']} "
+              f"alpha={st['alpha']:.6g}/{so['alpha']:.6g} err={err:.2e}", flush=True)
+        assert st["pcg_iterations"] == so["pcg_iterations"]
+        assert err < 1e-8
+
+
+def test_hybrid_impact_steps(oracle):
+    """time steps of the impact (30-40 Newton iterations each): each step
+    starts the oracle from the GPU's state and the two results agree to the
+    Newton-tolerance scale (GPU atomics can change the Newton path); the
+    boxes stay rigid and the fixed bed layer does not move"""
+    import sgipc
+    sc = _hybrid()
+    sim = sgipc.Simulation(sc)
+    w = oracle.World(sc)
+    abd = sc.abd_body
+    fixed = sc.btype == 1
+    x0 = sc.x.copy()
+    for step in range(4):
+        if step > 0:
+            w.set_state(sim.positions(), sim.velocities(), sim.xtilde(), sim.kappa)
+        sg = sim.time_step()
+        so = w.time_step()
+        xg, xo = sim.positions(), w.positions()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[hybrid-step] step={step} newton={sg['newton_iterations']}/"
+              f"{so['newton_iterations']} pcg={sg['pcg_iterations']}/{so['pcg_iterations']} "
+              f"err={err:.2e} box ymin={xg[abd >= 0, 1].min():.5f}", flush=True)
+        assert err < 1e-3
+        assert np.all(np.isfinite(sim.energy()))
+    np.testing.assert_array_equal(xg[fixed], x0[fixed])
+    for b in range(int(abd.max()) + 1):
+        d0 = np.linalg.norm(x0[abd == b][:, None] - x0[abd == b][None], axis=2)
+        d = np.linalg.norm(xg[abd == b][:, None] - xg[abd == b][None], axis=2)
+        assert np.abs(d - d0).max() < 1e-6
