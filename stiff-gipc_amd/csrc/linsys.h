@@ -40,6 +40,9 @@ struct Bsr
 struct Coo
 {
     int            cap = 0;
+    // host-side bound on the live entries (set by merge(), which reads the
+    // count): launches over the entries are sized by it instead of cap
+    unsigned       live = 0;
     DBuf<int>      r, c;     // cap
     DBuf<double>   v;        // 9 * cap
     DBuf<unsigned> count;    // 1 (device)

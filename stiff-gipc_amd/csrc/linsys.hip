@@ -240,7 +240,8 @@ unsigned Coo::merge(hipStream_t s)
     unsigned n = 0;
     SG_HIP(hipMemcpyAsync(&n, count.data(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
-    n = n < (unsigned)cap ? n : (unsigned)cap;
+    n    = n < (unsigned)cap ? n : (unsigned)cap;
+    live = n;
     if(n < 2)
         return n;
     size_t want = 1024;
@@ -278,7 +279,8 @@ void Coo::build_rows(hipStream_t s)
 {
     SG_CHECK(nb > 0, -5, "contact CSR: reserve() not called");
     SG_HIP(hipMemsetAsync(fill.data(), 0, (nb + 1) * sizeof(int), s));
-    hipLaunchKernelGGL(k_coo_hist, dim3(grid_for(cap, 256)), dim3(256), 0, s, count.data(),
+    const long long ent = live ? live : (unsigned)cap;  // entries to visit
+    hipLaunchKernelGGL(k_coo_hist, dim3(grid_for(ent, 256)), dim3(256), 0, s, count.data(),
                        (unsigned)cap, r.data(), fill.data());
     SG_LAUNCH_CHECK();
     int nt = (nb + kScanTile - 1) / kScanTile;
@@ -289,7 +291,7 @@ void Coo::build_rows(hipStream_t s)
                        tiles.data(), rp.data(), nt);
     SG_LAUNCH_CHECK();
     SG_HIP(hipMemsetAsync(fill.data(), 0, (nb + 1) * sizeof(int), s));
-    hipLaunchKernelGGL(k_coo_scatter, dim3(grid_for(cap, 256)), dim3(256), 0, s, count.data(),
+    hipLaunchKernelGGL(k_coo_scatter, dim3(grid_for(ent, 256)), dim3(256), 0, s, count.data(),
                        (unsigned)cap, r.data(), c.data(), v.data(), rp.data(), fill.data(),
                        rc.data(), rv.data());
     SG_LAUNCH_CHECK();

@@ -60,6 +60,28 @@ __global__ void k_l0_mask_coo(const unsigned* __restrict__ count, unsigned cap,
         atomicOr(mask + a, 1u << (b & 15));
 }
 
+// mask[key] |= bits for the lanes with key >= 0, merging equal keys of a
+// wave (up to 8 distinct keys; further lanes use their own atomic)
+__device__ __forceinline__ void wave_or_keyed(unsigned* mask, int key, unsigned bits)
+{
+    unsigned long long todo = __ballot(key >= 0);
+    for(int round = 0; round < 8 && todo; round++)
+    {
+        int      leader = __ffsll((long long)todo) - 1;
+        int      k      = __shfl(key, leader, 64);
+        bool     mine   = key == k;
+        unsigned v      = mine ? bits : 0u;
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1)
+            v |= __shfl_xor(v, o, 64);
+        if((int)__lane_id() == leader)
+            atomicOr(mask + k, v);
+        todo &= ~__ballot(mine);
+    }
+    if(todo & (1ull << __lane_id()))
+        atomicOr(mask + key, bits);
+}
+
 // level l >= 1: mask bits from static cross-bank edges and COO entries
 __global__ void k_lx_mask(int nx, const int* __restrict__ xa, const int* __restrict__ xb,
                           const unsigned* __restrict__ coo_count, int coo_cap,
@@ -69,25 +91,21 @@ __global__ void k_lx_mask(int nx, const int* __restrict__ xa, const int* __restr
     int t     = blockIdx.x * blockDim.x + threadIdx.x;
     int ncoo  = coo_count ? (int)*coo_count : 0;
     ncoo      = ncoo < coo_cap ? ncoo : coo_cap;
-    int u, v;
-    if(t < nx)
+    int a = -1, b = -1;  // every lane stays active for the wave merge below
+    if(t < nx + ncoo)
     {
-        u = xa[t];
-        v = xb[t];
+        int u = t < nx ? xa[t] : cr[t - nx];
+        int v = t < nx ? xb[t] : cc[t - nx];
+        a     = node_l[u];
+        b     = node_l[v];
+        if(!(a != b && (a >> 4) == (b >> 4)))
+            a = b = -1;
     }
-    else if(t < nx + ncoo)
-    {
-        u = cr[t - nx];
-        v = cc[t - nx];
-    }
-    else
-        return;
-    int a = node_l[u], b = node_l[v];
-    if(a != b && (a >> 4) == (b >> 4))
-    {
-        atomicOr(mask + a, 1u << (b & 15));
-        atomicOr(mask + b, 1u << (a & 15));
-    }
+    // At coarse levels most edges of a wave land on the same few nodes: the
+    // bits of equal targets are OR-ed across the wave first, then one atomic
+    // per target
+    wave_or_keyed(mask, a, 1u << (b & 15));
+    wave_or_keyed(mask, b, 1u << (a & 15));
 }
 
 __device__ __forceinline__ int level_count(const int* lvl, int level, int total_map)
@@ -290,12 +308,15 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
         int si = slot_of[i], sj = slot_of[j];
         if((si >> 4) == (sj >> 4))
         {
+            // level 0: every (row, col) block appears once (static pattern
+            // and merged contact blocks are disjoint), so each bank entry
+            // has a single writer
             double* M = B + (size_t)(si >> 4) * BD * BD + (3 * (si & 15)) * BD + 3 * (sj & 15);
 #pragma unroll
             for(int a = 0; a < 3; a++)
 #pragma unroll
                 for(int c = 0; c < 3; c++)
-                    atomicAdd(M + a * BD + c, b[3 * a + c]);
+                    M[a * BD + c] = b[3 * a + c];
         }
     }
     for(int l = 1; l < L; l++)
@@ -309,15 +330,26 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
             if((gi >> 4) == (gj >> 4))
                 key = ((long long)gi << 32) | (unsigned)gj;
         }
-        unsigned long long pending = __ballot(key != -1);
-        while(pending)
+        // Lanes sharing a target block are summed across the wave first (one
+        // atomic per entry and group) when the group is large; the rounds
+        // are capped, and lanes of small groups add their own block, so a
+        // wave with many distinct targets (fine levels) does not pay a wave
+        // reduction per target.
+        unsigned long long pending = __ballot(key != -1), direct = 0;
+        for(int round = 0; round < 6 && pending; round++)
         {
-            int       leader = __ffsll((long long)pending) - 1;
-            long long lk     = __shfl(key, leader, 64);
-            bool      mine   = (key == lk);
-            unsigned long long mm = __ballot(mine);
-            int       lgi = (int)(lk >> 32), lgj = (int)(lk & 0xffffffffll);
-            double*   M   = B + (size_t)(lgi >> 4) * BD * BD + (3 * (lgi & 15)) * BD + 3 * (lgj & 15);
+            int                leader = __ffsll((long long)pending) - 1;
+            long long          lk     = __shfl(key, leader, 64);
+            bool               mine   = (key == lk);
+            unsigned long long mm     = __ballot(mine);
+            pending &= ~mm;
+            if(__popcll(mm) < 4)
+            {
+                direct |= mm;  // small group: its lanes add their own blocks
+                continue;
+            }
+            int     lgi = (int)(lk >> 32), lgj = (int)(lk & 0xffffffffll);
+            double* M   = B + (size_t)(lgi >> 4) * BD * BD + (3 * (lgi & 15)) * BD + 3 * (lgj & 15);
 #pragma unroll
             for(int e = 0; e < 9; e++)
             {
@@ -325,7 +357,13 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
                 if((int)__lane_id() == leader)
                     atomicAdd(M + (e / 3) * BD + (e % 3), s);
             }
-            pending &= ~mm;
+        }
+        if((pending | direct) & (1ull << __lane_id()))
+        {
+            double* M = B + (size_t)(gi >> 4) * BD * BD + (3 * (gi & 15)) * BD + 3 * (gj & 15);
+#pragma unroll
+            for(int e = 0; e < 9; e++)
+                atomicAdd(M + (e / 3) * BD + (e % 3), b[e]);
         }
     }
 }
@@ -859,12 +897,14 @@ void Mas::build(const SysView& S, hipStream_t s)
 {
     const Coo* coo     = (S.coo && S.coo->cap > 0) ? S.coo : nullptr;
     int        coo_cap = coo ? coo->cap : 0;
+    // launches over the contact entries: the live bound when known
+    long long  coo_ent = coo ? (coo->live ? (long long)coo->live : (long long)coo_cap) : 0;
     // ---- hierarchy (ReorderRealtime)
     SG_HIP(hipMemcpyAsync(mask_.data(), mask0_static_.data(), total_map_ * sizeof(unsigned),
                           hipMemcpyDeviceToDevice, s));
     if(coo)
     {
-        hipLaunchKernelGGL(k_l0_mask_coo, dim3(grid_for(coo_cap, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_l0_mask_coo, dim3(grid_for(coo_ent, 256)), dim3(256), 0, s,
                            coo->count.data(), (unsigned)coo_cap, coo->r.data(), coo->c.data(),
                            slot_of_.data(), mask_.data());
         SG_LAUNCH_CHECK();
@@ -873,7 +913,7 @@ void Mas::build(const SysView& S, hipStream_t s)
     for(int l = 1; l < L_; l++)
     {
         SG_HIP(hipMemsetAsync(mask_.data(), 0, total_map_ * sizeof(unsigned), s));
-        long long ne = (long long)nx_ + coo_cap;
+        long long ne = (long long)nx_ + coo_ent;
         if(ne > 0)
         {
             hipLaunchKernelGGL(k_lx_mask, dim3(grid_for(ne, 256)), dim3(256), 0, s, nx_,
@@ -923,7 +963,7 @@ void Mas::assemble(const SysView& S, hipStream_t s)
     const Coo* coo     = (S.coo && S.coo->cap > 0) ? S.coo : nullptr;
     size_t     nbanks  = total_clusters_ / BANK;
     SG_HIP(hipMemsetAsync(B_.data(), 0, nbanks * BD * BD * sizeof(double), s));
-    long long n = (long long)A.nnzb + (coo ? coo->cap : 0);
+    long long n = (long long)A.nnzb + (coo ? (coo->live ? (long long)coo->live : coo->cap) : 0);
     hipLaunchKernelGGL(k_mas_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, A.nnzb,
                        A.rows.data(), A.cols.data(), A.vals.data(),
                        coo ? coo->count.data() : nullptr, coo ? coo->cap : 0,
