@@ -582,13 +582,24 @@ __global__ void k_dot3(int n, const double* __restrict__ a, const double* __rest
         atomicAdd(out, d);
 }
 
-__global__ void k_max_abs(int n, const double* __restrict__ a, double* out)
+// grid-stride max |a_i|: one atomic per workgroup (a single address would
+// serialise one atomic per wave)
+__global__ void __launch_bounds__(256) k_max_abs(int n, const double* __restrict__ a, double* out)
 {
-    int    i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = (i < n) ? fabs(a[i]) : 0.0;
-    d        = wave_max(d);
-    if(__lane_id() == 0 && d > 0.0)
-        atomic_max_pos(out, d);
+    __shared__ double wm[4];
+    double            d = 0.0;
+    for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        d = fmax(d, fabs(a[i]));
+    d = wave_max(d);
+    if(__lane_id() == 0)
+        wm[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        d = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        if(d > 0.0)
+            atomic_max_pos(out, d);
+    }
 }
 
 __global__ void k_step_forward(int nv, const int* __restrict__ btype, const double* __restrict__ xs,
@@ -740,7 +751,8 @@ void launch_dot(int n, const double* a, const double* b, double* out, hipStream_
 
 void launch_max_abs(int n, const double* a, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_max_abs, dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, out);
+    unsigned g = grid_for(n, 256);
+    hipLaunchKernelGGL(k_max_abs, dim3(g < 1024 ? g : 1024), dim3(256), 0, s, n, a, out);
     SG_LAUNCH_CHECK();
 }
 

@@ -196,6 +196,9 @@ struct PcgConfig
     // mixed precision: FP32 PCG whose periodic residual recomputation is done
     // in FP64 (with use_rz = false, rel_tol bounds the FP64 true residual)
     bool   mixed = false;
+    // mixed precision: stop the FP32 iteration when the tested residual has
+    // not halved over this many stop tests (0: the solver's default, 20)
+    int    stall_checks = 0;
 };
 
 struct PcgStats
@@ -204,6 +207,7 @@ struct PcgStats
     int    outer_iterations = 0;  // 1, or 2 when the mixed mode fell back to FP64
     int    breakdown        = 0;  // 1 non-finite r.z, 2 diverged (mixed mode)
     bool   converged        = false;
+    bool   stalled          = false;  // mixed mode: FP32 iteration stopped making progress
     double rz0 = 0, rz = 0;
     double rel_residual = 0;
 };

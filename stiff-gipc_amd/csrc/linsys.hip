@@ -10,6 +10,8 @@
 // host reads back only at the convergence test.
 #include "linsys.h"
 
+#include <cstring>
+
 namespace sgipc
 {
 
@@ -355,7 +357,10 @@ __global__ void __launch_bounds__(256)
         bool   bad = !(rzc == rzc) || !(q == q);
         bool   div = pc.div_thr > 0.0 && q > pc.div_thr;
         if(blockIdx.x == 0 && threadIdx.x == 0)
+        {
             pc.ctl_d[0] = rzc;
+            pc.ctl_d[1] = q;  // the tested quantity, read back by the host
+        }
         if(q <= pc.thr || bad || div)
         {
             if(blockIdx.x == 0 && threadIdx.x == 0)
@@ -942,6 +947,11 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
     ctl_only.ctl_d = ctl_d_.data();
     int  slot = 0, pend_slot = 0;
     bool have_pending = false;
+    // mixed mode: the FP32 iteration is abandoned when the tested FP64
+    // residual has not halved over cfg.stall_checks consecutive tests (the
+    // caller then continues in FP64 from the FP64 iterate)
+    double best_q = 0.0;
+    int    stall = 0, stalled_at = 0;
     for(int k = 1; k <= maxit; k++)
     {
         PcgCtl     pcx  = ctl_only;
@@ -964,12 +974,30 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         {
             SG_HIP(hipMemcpyAsync(h_ctl_ + 4 * slot, ctl_.data(), 2 * sizeof(int),
                                   hipMemcpyDeviceToHost, s));
+            if(reliable && cfg.stall_checks > 0)
+                SG_HIP(hipMemcpyAsync(h_ctl_ + 4 * slot + 2, ctl_d_.data() + 1, sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
             SG_HIP(hipEventRecord(ev_[slot], s));
             if(have_pending)
             {
                 SG_HIP(hipEventSynchronize(ev_[pend_slot]));
                 if(h_ctl_[4 * pend_slot] != 0)
                     break;
+                if(reliable && cfg.stall_checks > 0)
+                {
+                    double q;
+                    memcpy(&q, h_ctl_ + 4 * pend_slot + 2, sizeof(double));
+                    if(best_q == 0.0 || q < 0.5 * best_q)
+                    {
+                        best_q = q;
+                        stall  = 0;
+                    }
+                    else if(++stall >= cfg.stall_checks)
+                    {
+                        stalled_at = k;  // launches up to k are in the stream
+                        break;
+                    }
+                }
             }
             have_pending = true;
             pend_slot    = slot;
@@ -1029,7 +1057,8 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
     st.breakdown  = hc[1];
     st.rz         = hd[0];
     st.converged  = hc[0] != 0 && hc[1] == 0;
-    st.iterations = hc[0] != 0 ? hc[0] : maxit;
+    st.iterations = hc[0] != 0 ? hc[0] : (stalled_at ? stalled_at : maxit);
+    st.stalled    = stalled_at != 0 && hc[0] == 0;
     return st;
 }
 
@@ -1078,6 +1107,8 @@ PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* 
     PcgConfig c32 = cfg;
     if(c32.max_iter <= 0)
         c32.max_iter = n < 20000 ? n : 20000;
+    if(c32.stall_checks <= 0)
+        c32.stall_checks = 20;
     PcgStats st = solve_t<float>(S, pc, mas, v32_.rhs.data(), v32_.sol.data(), c32, s, b, x);
     st.outer_iterations = 1;
     st.rel_residual     = true_residual(S, b, x, s);
@@ -1085,12 +1116,33 @@ PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* 
               && (st.converged || cfg.max_iter > 0);
     if(!ok)
     {
-        // the FP32 iteration broke down or diverged (FP32 rounding of a badly
-        // conditioned operator): solve again in FP64
-        PcgStats f           = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
-        f.iterations        += st.iterations;
-        f.outer_iterations   = 2;
-        f.rel_residual       = true_residual(S, b, x, s);
+        // The FP32 iteration broke down, diverged or stalled (FP32 rounding
+        // of a badly conditioned operator).  With a true-residual stop test
+        // and an FP64 iterate better than zero, FP64 PCG continues from it
+        // on A y = b - A x with the stop threshold of the original system;
+        // otherwise FP64 solves again from zero.
+        PcgStats f;
+        bool     cont = !cfg.use_rz && st.breakdown != 1 && st.rel_residual == st.rel_residual
+                    && st.rel_residual < 1.0;
+        if(cont)
+        {
+            residual(S, b, x, v64_.res.data(), s);
+            PcgConfig c64 = cfg;
+            if(c64.q_ref <= 0.0)  // threshold of this system: |b|^2
+            {
+                SG_HIP(hipMemsetAsync(aux_set(3), 0, kSlotSet * sizeof(double), s));
+                dot_slots(n, b, b, aux_set(3), s);
+                c64.q_ref = read_slot_set(aux_set(3), s);
+            }
+            f = solve_t<double>(S, pc, mas, v64_.res.data(), v64_.sol.data(), c64, s, nullptr,
+                                nullptr);
+            launch_axpy_d(n, 1.0, v64_.sol.data(), x, s);
+        }
+        else
+            f = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
+        f.iterations      += st.iterations;
+        f.outer_iterations = 2;
+        f.rel_residual     = true_residual(S, b, x, s);
         return f;
     }
     return st;

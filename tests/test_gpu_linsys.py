@@ -152,3 +152,29 @@ def test_mixed_precision_deep_hierarchy(deep, oracle):
           f"relres={st.rel_residual:.3e}", flush=True)
     assert st.converged and st.rel_residual <= 1e-8
     assert rel(A.spmv(x), b) <= 1.01e-8
+
+
+def test_mixed_precision_stiff_contrast(systems, oracle):
+    """A system with a 1e8 stiffness contrast (one cube's rows and columns
+    scaled by 1e4, i.e. D A D): the FP32 iteration may stall or diverge; the
+    mixed solve must still reach the FP64 true residual asked for (FP64
+    continuation from its iterate), within the FP32 plus FP64 iteration
+    budget, and agree with the dense solve"""
+    import sgipc
+    sc, g, A = systems["cubes8"]
+    nb = A.nb
+    scale = np.where(sc.body_id == 0, 1e4, 1.0)
+    rows = np.repeat(np.arange(nb), np.diff(A.rowptr))
+    vals = A.vals.reshape(-1, 3, 3) * (scale[rows] * scale[A.cols])[:, None, None]
+    ls = sgipc.LinearSystem(A.rowptr, A.cols, vals.reshape(A.vals.shape), sc.part_id,
+                            sgipc.SGIPC_PRECOND_MAS, sgipc.SGIPC_MAS_FP32)
+    b = g * np.repeat(scale, 3)
+    x, st = ls.pcg(b, rel_tol=1e-10, use_rz=False, solver_precision=sgipc.SGIPC_SOLVE_MIXED)
+    x64, st64 = ls.pcg(b, rel_tol=1e-10, use_rz=False)
+    print(f"[mixed-stiff] it={st.iterations} outer={st.outer_iterations} "
+          f"relres={st.rel_residual:.3e}; fp64 it={st64.iterations} "
+          f"relres={st64.rel_residual:.3e}", flush=True)
+    assert st.rel_residual <= 1e-10
+    assert st64.rel_residual <= 1e-10
+    assert st.iterations <= 4 * st64.iterations + 400
+    assert rel(x, x64) < 1e-5
