@@ -329,6 +329,14 @@ orc_mas* orc_mas_create(const orc_bsr* A, const int32_t* part_id, int fp32)
         if(fp32)
             for(int e = 0; e < 48 * 48; e++)
                 M[e] = (double)(float)M[e];
+        /* only the upper 3x3 blocks are kept (MasMatrixSymf, eigen_data.h:125-128,
+         * stored by __inverse6_P96x96, MASPreconditioner.cu:628-724); the apply
+         * uses their transposes below the diagonal */
+        for(int I = 1; I < BANK; I++)
+            for(int J = 0; J < I; J++)
+                for(int a = 0; a < 3; a++)
+                    for(int b = 0; b < 3; b++)
+                        M[(3 * I + a) * 48 + 3 * J + b] = M[(3 * J + b) * 48 + 3 * I + a];
     }
     m->inv = B;
     free(ea);

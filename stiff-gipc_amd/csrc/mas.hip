@@ -413,10 +413,15 @@ __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B,
         }
         __syncthreads();
     }
+    // Stored transposed (dst[u][t] = Binv[t][u], so that the lanes of a wave
+    // read a row with unit stride), keeping only the upper 3x3 blocks of the
+    // inverse, as the reference's MasMatrixSymf does (eigen_data.h:125-128,
+    // written by __inverse6_P96x96, MASPreconditioner.cu:628-724): below the
+    // block diagonal the operator is the transpose of the upper blocks
     T* dst = inv + (size_t)bank * BD * BD;
     if(c < BD)
         for(int r = 0; r < BD; r++)
-            dst[r * BD + c] = (T)M[c][r];  // transposed
+            dst[r * BD + c] = (T)((r / 3 >= c / 3) ? M[c][r] : M[r][c]);
 }
 
 // ---------------------------------------------------------------- apply
