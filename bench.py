@@ -212,6 +212,7 @@ def main():
                          "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
                          "max_rel_residual": adv["max_pcg_rel_residual"],
                          "fp64_fallbacks": adv["solver_fallbacks"]}
+
         log(f"timed [{name}]: {args.steps} Newton iterations in {(t1 - t0):.3f} s -> "
             f"{ms:.3f} ms/iter; pcg/iter={adv['pcg_iterations'] / args.steps:.1f} "
             f"max_rel_res={adv['max_pcg_rel_residual']:.2e} "
@@ -226,6 +227,24 @@ def main():
                         f"{tot / n * 1e3:.2f} us avg")
             ref_adv, ref_ms = adv, ms
             sim.profile(0)
+        # phase split (untimed, after the timed region): device time of
+        # assembly + stop test, MAS build + solve, CCD bounds, line search,
+        # averaged over a few further Newton iterations that solve
+        ph = {"assembly": 0.0, "solve": 0.0, "ccd": 0.0, "line_search": 0.0}
+        nph = 0
+        for _ in range(6):
+            it = sim.newton_iteration()
+            if it["converged"]:
+                sim.advance(1)  # begins the next time step
+                continue
+            for key in ph:
+                ph[key] += it["ms_" + key]
+            nph += 1
+            if nph == 3:
+                break
+        if nph:
+            results[name]["phases_ms"] = {k: round(v / nph, 3) for k, v in ph.items()}
+            log(f"phases [{name}]: {results[name]['phases_ms']}")
     sim.learned_mode(0)
     adv, ms = ref_adv, ref_ms
     sz = sim.sizes()

@@ -284,6 +284,20 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
     return k * q + (k < r ? k : r) + i;
 }
 
+// ---------------------------------------------------------------- hashing
+// open-addressing tables keyed by 64-bit (row << 32 | col)
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
 // ---------------------------------------------------------------- wave ops
 __device__ __forceinline__ double wave_sum(double v)
 {

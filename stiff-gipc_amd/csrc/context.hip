@@ -623,8 +623,7 @@ Context::Context(const sgipc_scene_desc& d)
     cand_.resize(cand_cap_);
     overflow_.resize(4);
     overflow_.zero(s_);
-    // affine-body contact blocks expand into up to 4x4 node blocks
-    coo_.reserve((nabd_ ? 48 : 6) * (size_t)pair_cap_, nn_);
+    coo_.reserve(nn_, s_);
 
     dev_.nv = nv_, dev_.nt = nt_, dev_.ns = ns_, dev_.nf = nf_, dev_.ne = ne_;
     dev_.dt = d.dt, dev_.dHat = dHat_, dev_.goff = d.ground_offset;
@@ -644,10 +643,7 @@ Context::Context(const sgipc_scene_desc& d)
     dev_.gp = gp_.data(), dev_.gcount = gcount_.data();
     dev_.pairs = pairs_.data(), dev_.pairs_raw = pairs_raw_.data(), dev_.pcount = pcount_.data();
     dev_.pair_cap = pair_cap_;
-    dev_.coo_r = coo_.r.data(), dev_.coo_c = coo_.c.data(), dev_.coo_v = coo_.v.data();
-    dev_.coo_count = coo_.count.data(), dev_.coo_cap = coo_.cap;
     dev_.overflow = overflow_.data();
-    coo_.count.zero(s_);
 
     // lagged friction sets (built at every step start)
     fr_code_.resize(pair_cap_);
@@ -864,26 +860,37 @@ void Context::build_cp()
 
 void Context::assemble(bool hessian)
 {
-    if(hessian)
+    for(int attempt = 0;; attempt++)
     {
-        A_.vals.zero(s_);
-        coo_.count.zero(s_);
+        if(hessian)
+        {
+            A_.vals.zero(s_);
+            coo_.begin(s_);
+        }
+        dev_.hkeys  = coo_.hkeys.data();
+        dev_.hvals  = coo_.hvals.data();
+        dev_.hlist  = coo_.hlist.data();
+        dev_.hcount = coo_.hcount.data();
+        dev_.hfail  = coo_.hfail.data();
+        dev_.hmask  = (unsigned long long)(coo_.hcap - 1);
+        launch_init_diag(dev_, s_);
+        launch_abd_body(dev_, hessian, s_);
+        launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
+        launch_friction(dev_, hessian, gc_.data(), s_);
+        launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
+        prof_.begin("fem", s_);
+        launch_fem(dev_, hessian, s_);
+        prof_.end("fem", s_);
+        launch_rhs(dev_, rhs_.data(), s_);
+        launch_abd_rhs(dev_, rhs_.data(), s_);
+        if(!hessian)
+            return;
+        if(coo_.finish(s_))
+            break;
+        // the contact table ran full: it has been enlarged, assemble again
+        SG_CHECK(attempt < 6, SGIPC_ECAPACITY, "contact block table cannot grow further");
     }
-    launch_init_diag(dev_, s_);
-    launch_abd_body(dev_, hessian, s_);
-    launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
-    launch_friction(dev_, hessian, gc_.data(), s_);
-    launch_ground(dev_, kappa_, hessian, gc_.data(), s_);
-    prof_.begin("fem", s_);
-    launch_fem(dev_, hessian, s_);
-    prof_.end("fem", s_);
-    launch_rhs(dev_, rhs_.data(), s_);
-    launch_abd_rhs(dev_, rhs_.data(), s_);
-    if(hessian)
-    {
-        coo_.merge(s_);
-        coo_.build_rows(s_);
-    }
+    coo_.build_rows(s_);
 }
 
 double Context::energy()

@@ -344,28 +344,28 @@ __device__ __forceinline__ double pair_d2(int type, const double* x, const int i
     return type == 2 ? d_pt(a, b, c, d) : d_ee(a, b, c, d);
 }
 
-// gradient of the unsigned distance (closest points; envelope theorem)
-__device__ inline void dist_grad(int type, const double* x, const int ids[4], double g[12])
+// gradient of the unsigned distance (closest points; envelope theorem):
+// g_k = w_k u for the element's vertices k < nv, u the unit direction
+// between the closest points; returns nv
+__device__ inline int dist_wu(int type, const double* x, const int ids[4], double w[4], V3& u)
 {
     V3 x0 = ld3(x + 3 * ids[0]), x1 = ld3(x + 3 * ids[1]);
     if(type == 0)
     {
-        V3 u = x0 - x1;
+        u    = x0 - x1;
         u    = u * (1.0 / sqrt(sqn(u)));
-        g[0] = u.x, g[1] = u.y, g[2] = u.z, g[3] = -u.x, g[4] = -u.y, g[5] = -u.z;
-        return;
+        w[0] = 1.0, w[1] = -1.0;
+        return 2;
     }
     V3 x2 = ld3(x + 3 * ids[2]);
     if(type == 1)
     {
         V3     e = x2 - x1;
         double t = dot(x0 - x1, e) / sqn(e);
-        V3     u = x0 - (x1 + e * t);
+        u        = x0 - (x1 + e * t);
         u        = u * (1.0 / sqrt(sqn(u)));
-        double w[3] = {1.0, -(1.0 - t), -t};
-        for(int k = 0; k < 3; k++)
-            g[3 * k] = w[k] * u.x, g[3 * k + 1] = w[k] * u.y, g[3 * k + 2] = w[k] * u.z;
-        return;
+        w[0] = 1.0, w[1] = -(1.0 - t), w[2] = -t;
+        return 3;
     }
     V3 x3 = ld3(x + 3 * ids[3]);
     if(type == 2)
@@ -377,22 +377,19 @@ __device__ inline void dist_grad(int type, const double* x, const int ids[4], do
         double a = sqn(e1), b = dot(e1, e2), c = sqn(e2);
         double r1 = dot(pq, e1), r2 = dot(pq, e2), D = a * c - b * b;
         double be = (c * r1 - b * r2) / D, ga = (a * r2 - b * r1) / D;
-        V3     u  = nv * h;
+        u         = nv * h;
         u         = u * (1.0 / sqrt(sqn(u)));
-        double w[4] = {1.0, -(1.0 - be - ga), -be, -ga};
-        for(int k = 0; k < 4; k++)
-            g[3 * k] = w[k] * u.x, g[3 * k + 1] = w[k] * u.y, g[3 * k + 2] = w[k] * u.z;
-        return;
+        w[0] = 1.0, w[1] = -(1.0 - be - ga), w[2] = -be, w[3] = -ga;
+        return 4;
     }
-    V3     u = x1 - x0, v = x3 - x2, w = x0 - x2;
-    double a = sqn(u), b = dot(u, v), c = sqn(v), d = dot(u, w), e = dot(v, w);
+    V3     eu = x1 - x0, v = x3 - x2, ww = x0 - x2;
+    double a = sqn(eu), b = dot(eu, v), c = sqn(v), d = dot(eu, ww), e = dot(v, ww);
     double D = a * c - b * b;
     double s = (b * e - c * d) / D, t = (a * e - b * d) / D;
-    V3     pq = w + u * s - v * t;
-    pq        = pq * (1.0 / sqrt(sqn(pq)));
-    double wt[4] = {1.0 - s, s, -(1.0 - t), -t};
-    for(int k = 0; k < 4; k++)
-        g[3 * k] = wt[k] * pq.x, g[3 * k + 1] = wt[k] * pq.y, g[3 * k + 2] = wt[k] * pq.z;
+    u        = ww + eu * s - v * t;
+    u        = u * (1.0 / sqrt(sqn(u)));
+    w[0] = 1.0 - s, w[1] = s, w[2] = -(1.0 - t), w[3] = -t;
+    return 4;
 }
 
 __device__ __forceinline__ double barrier_lambda0(double I5, double dHat, double kappa)
@@ -424,27 +421,28 @@ __global__ void __launch_bounds__(128) k_barrier(const unsigned* __restrict__ co
     double I5   = sq * sq;
     double L    = log(I5);
     double pk1  = sq * 2 * (2 * kappa * dHat * dHat * L * (I5 - 1) * (I5 + I5 * L - 1)) / I5;
-    double gd[12];
-    dist_grad(type, x, ids, gd);
-    for(int i = 0; i < 3 * nv; i++)
-        gd[i] *= (1.0 / dhs);
+    // d(sqrt(I5))/dx = g / sqrt(dhat), g_k = w_k u
+    double w[4];
+    V3     u;
+    dist_wu(type, x, ids, w, u);
+    double cw[4];
+    for(int a = 0; a < nv; a++)
+        cw[a] = w[a] * (1.0 / dhs);
+    const double uu[3] = {u.x, u.y, u.z};
     for(int a = 0; a < nv; a++)
         for(int c = 0; c < 3; c++)
-            atomicAdd(gcontact + 3 * ids[a] + c, pk1 * gd[3 * a + c]);
+            atomicAdd(gcontact + 3 * ids[a] + c, pk1 * (cw[a] * uu[c]));
     if(!hessian)
         return;
     double lam0 = barrier_lambda0(I5, dHat, kappa);
     if(d * d < 1e-6 * dHat)
         lam0 = barrier_lambda0(1e-6, dHat, kappa);
-    for(int a = 0; a < nv; a++)
-        for(int b = a; b < nv; b++)
-        {
-            double blk[9];
-            for(int r = 0; r < 3; r++)
-                for(int c = 0; c < 3; c++)
-                    blk[r * 3 + c] = lam0 * gd[3 * a + r] * gd[3 * b + c];
-            sink.put(ids[a], ids[b], blk, a == b);
-        }
+    // H_ab = lambda0 (dsq/dx_a)(dsq/dx_b)^T = cw_a cw_b (lambda0 u u^T)
+    double K[9];
+    for(int r = 0; r < 3; r++)
+        for(int c = 0; c < 3; c++)
+            K[r * 3 + c] = lam0 * uu[r] * uu[c];
+    sink.put_rank(ids, nv, cw, K);
 }
 
 __global__ void k_rhs(int nv, const int* __restrict__ btype, const double* __restrict__ gc,

@@ -304,14 +304,11 @@ __global__ void __launch_bounds__(128) k_friction(const double* __restrict__ x, 
         for(int cc = 0; cc < 3; cc++)
             P[3 * r + cc] = a0 * B[2 * cc] + a1 * B[2 * cc + 1];
     }
-    for(int a = 0; a < m; a++)
-        for(int b = a; b < m; b++)
-        {
-            double blk[9], k = sc * w[a] * w[b];
-            for(int q = 0; q < 9; q++)
-                blk[q] = k * P[q];
-            sink.put(ids[a], ids[b], blk, a == b);
-        }
+    // H_ab = w_a w_b (sc P)
+    double K[9];
+    for(int q = 0; q < 9; q++)
+        K[q] = sc * P[q];
+    sink.put_rank(ids, m, w, K);
 }
 
 __global__ void __launch_bounds__(256) k_friction_ground(const double* __restrict__ x, V3 n,

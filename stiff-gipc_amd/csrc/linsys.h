@@ -39,9 +39,11 @@ struct Bsr
 
 struct Coo
 {
-    int            cap = 0;
-    // host-side bound on the live entries (set by merge(), which reads the
-    // count): launches over the entries are sized by it instead of cap
+    // distinct contact blocks (row, col, 3x3), written by finish() from the
+    // table the assembly inserts into
+    int            cap = 0;  // capacity of r, c, v, rc, rv (entries)
+    // host-side bound on the live entries (set by finish()): launches over
+    // the entries are sized by it instead of cap
     unsigned       live = 0;
     DBuf<int>      r, c;     // cap
     DBuf<double>   v;        // 9 * cap
@@ -55,32 +57,28 @@ struct Coo
     DBuf<double>   rv;       // 9 * cap
     DBuf<float>    rv32;     // FP32 copy (mixed precision)
     bool           rows_valid = false;
-    void reserve(int n, int nrows)
-    {
-        cap = n;
-        nb  = nrows;
-        r.resize(n);
-        c.resize(n);
-        v.resize(9 * (size_t)n);
-        count.resize(1);
-        rp.resize(nrows + 1);
-        fill.resize(nrows + 1);
-        tiles.resize((nrows + 1023) / 1024 + 2);
-        rc.resize(n);
-        rv.resize(9 * (size_t)n);
-    }
+    // open-addressing table the contact kernels insert into (Coo::table_*):
+    // keys (row << 32 | col, empty = ~0), 9 values per slot (zero between
+    // assemblies), the list of claimed slots, its length and a full flag
+    DBuf<unsigned long long> hkeys;
+    DBuf<double>             hvals;
+    DBuf<int>                hlist;
+    DBuf<unsigned>           hcount;
+    DBuf<int>                hfail;
+    size_t                   hcap = 0;  // slots, a power of two
+    void reserve(int nrows, hipStream_t s);
+    // before an assembly: empty list, no entries
+    void begin(hipStream_t s);
+    // after an assembly: the claimed slots become the entries (count) and
+    // are cleared; false when the table ran full (it has been enlarged and
+    // the assembly must be redone)
+    bool finish(hipStream_t s);
     // counting sort of the live entries by row (after every assembly)
     void build_rows(hipStream_t s);
-    // merge entries with equal (row, col) in place (sum of their blocks),
-    // count = number of distinct blocks; returns the count before merging
-    unsigned merge(hipStream_t s);
-    // open-addressing table of the merge (grown to twice the live count)
-    DBuf<unsigned long long> hkeys;  // P, empty = ~0
-    DBuf<double>             hvals;  // 9 * P, zero between merges
-    DBuf<int>                hlist;  // occupied slots, in insertion order
-    DBuf<unsigned>           hcount; // 1
-    DBuf<int>                hfail;  // 1: a probe sequence ran through the table
-    size_t                   hcap = 0;
+
+  private:
+    void grow_table(size_t slots, hipStream_t s);
+    void ensure_entries(int n);
 };
 
 // view of A = static BSR + COO tail

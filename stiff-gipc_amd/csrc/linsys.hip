@@ -162,57 +162,14 @@ __global__ void k_coo_scatter(const unsigned* __restrict__ count, unsigned cap,
 // ------------------------------------------------------------- merge
 // Contact blocks mapped to affine-body nodes repeat the same (row, col) many
 // times (every contact between two bodies lands on the same 16 node pairs).
-// The entries are merged by hashing (row, col) into an open-addressing table
-// with linear probing: the first thread to claim a slot records it in the
-// slot list, every thread adds its block with FP64 atomics; the slots are
-// then written back over the raw entries and cleared for the next merge.
-// This is the merge of the reference's triplet converter (converter.cu:23-189,
-// radix sort + segmented reduction) done in one pass without sorting.
-constexpr unsigned long long kEmptyKey = ~0ull;
+// The contact kernels therefore insert their blocks into an open-addressing
+// table keyed by (row, col) (BlockSink::table_add): equal keys are summed on
+// insertion with FP64 atomics.  finish() turns the claimed slots into the
+// entry arrays and clears them for the next assembly.  This is the merge of
+// the reference's triplet converter (converter.cu:23-189, radix sort +
+// segmented reduction) done without sorting.
 
-__device__ __forceinline__ unsigned long long mix64(unsigned long long k)
-{
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ull;
-    k ^= k >> 33;
-    return k;
-}
-
-__global__ void k_coo_hash(unsigned n, const int* __restrict__ r, const int* __restrict__ c,
-                           const double* __restrict__ v, unsigned long long* __restrict__ keys,
-                           double* __restrict__ vals, unsigned long long mask,
-                           int* __restrict__ list, unsigned* __restrict__ nlist,
-                           int* __restrict__ fail)
-{
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= n)
-        return;
-    unsigned long long key = ((unsigned long long)(unsigned)r[t] << 32) | (unsigned)c[t];
-    unsigned long long h   = mix64(key) & mask;
-    for(unsigned long long probe = 0; probe <= mask; probe++)
-    {
-        unsigned long long prev = atomicCAS(keys + h, kEmptyKey, key);
-        if(prev == kEmptyKey)
-            list[atomicAdd(nlist, 1u)] = (int)h;
-        if(prev == kEmptyKey || prev == key)
-        {
-#pragma unroll
-            for(int q = 0; q < 9; q++)
-            {
-                double x = v[9 * (size_t)t + q];
-                if(x != 0.0)
-                    atomicAdd(vals + 9 * h + q, x);
-            }
-            return;
-        }
-        h = (h + 1) & mask;
-    }
-    *fail = 1;  // unreachable while the table is at most half full
-}
-
-// write the distinct blocks back over the raw entries and clear their slots
+// claimed slots -> entries (r, c, v); slots cleared
 __global__ void k_coo_unhash(const unsigned* __restrict__ nlist, const int* __restrict__ list,
                              unsigned long long* __restrict__ keys, double* __restrict__ vals,
                              int* __restrict__ r, int* __restrict__ c, double* __restrict__ v,
@@ -237,44 +194,90 @@ __global__ void k_coo_unhash(const unsigned* __restrict__ nlist, const int* __re
     keys[h] = kEmptyKey;
 }
 
-unsigned Coo::merge(hipStream_t s)
+void Coo::reserve(int nrows, hipStream_t s)
 {
-    unsigned n = 0;
-    SG_HIP(hipMemcpyAsync(&n, count.data(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));
-    n    = n < (unsigned)cap ? n : (unsigned)cap;
-    live = n;
-    if(n < 2)
-        return n;
-    size_t want = 1024;
-    while(want < 2 * (size_t)n)
-        want <<= 1;
-    if(want > hcap)
-    {
-        hkeys.resize(want);
-        hvals.resize(9 * want);
-        hlist.resize(want);
-        hcount.resize(1);
-        hfail.resize(1);
-        SG_HIP(hipMemsetAsync(hkeys.data(), 0xff, want * sizeof(unsigned long long), s));
-        SG_HIP(hipMemsetAsync(hvals.data(), 0, 9 * want * sizeof(double), s));
-        hcap = want;
-    }
+    nb = nrows;
+    count.resize(1);
+    rp.resize(nrows + 1);
+    fill.resize(nrows + 1);
+    tiles.resize((nrows + 1023) / 1024 + 2);
+    hcount.resize(1);
+    hfail.resize(1);
+    grow_table(size_t(1) << 16, s);
+    ensure_entries(1 << 15);
+    count.zero(s);
+    hcount.zero(s);
+    hfail.zero(s);
+}
+
+void Coo::grow_table(size_t slots, hipStream_t s)
+{
+    if(slots <= hcap)
+        return;
+    hkeys.resize(slots);
+    hvals.resize(9 * slots);
+    hlist.resize(slots);
+    SG_HIP(hipMemsetAsync(hkeys.data(), 0xff, slots * sizeof(unsigned long long), s));
+    SG_HIP(hipMemsetAsync(hvals.data(), 0, 9 * slots * sizeof(double), s));
+    hcap = slots;
+}
+
+void Coo::ensure_entries(int n)
+{
+    if(n <= cap)
+        return;
+    int c2 = cap > 0 ? cap : 1;
+    while(c2 < n)
+        c2 *= 2;
+    r.resize(c2);
+    c.resize(c2);
+    v.resize(9 * (size_t)c2);
+    rc.resize(c2);
+    rv.resize(9 * (size_t)c2);
+    cap = c2;
+}
+
+void Coo::begin(hipStream_t s)
+{
     SG_HIP(hipMemsetAsync(hcount.data(), 0, sizeof(unsigned), s));
     SG_HIP(hipMemsetAsync(hfail.data(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_coo_hash, dim3(grid_for(n, 256)), dim3(256), 0, s, n, r.data(),
-                       c.data(), v.data(), hkeys.data(), hvals.data(),
-                       (unsigned long long)(hcap - 1), hlist.data(), hcount.data(), hfail.data());
-    SG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_coo_unhash, dim3(grid_for(n, 256)), dim3(256), 0, s, hcount.data(),
-                       hlist.data(), hkeys.data(), hvals.data(), r.data(), c.data(), v.data(),
-                       count.data());
-    SG_LAUNCH_CHECK();
-    int fail = 0;
+    SG_HIP(hipMemsetAsync(count.data(), 0, sizeof(unsigned), s));
+    rows_valid = false;
+}
+
+bool Coo::finish(hipStream_t s)
+{
+    unsigned n = 0;
+    int      fail = 0;
+    SG_HIP(hipMemcpyAsync(&n, hcount.data(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
     SG_HIP(hipMemcpyAsync(&fail, hfail.data(), sizeof(int), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
-    SG_CHECK(!fail, -5, "contact block merge: hash table full");
-    return n;
+    if(fail)
+    {
+        // the keys of this attempt stay in the old table: allocate afresh
+        size_t want = 4 * hcap;
+        hcap        = 0;
+        grow_table(want, s);
+        return false;
+    }
+    ensure_entries((int)n);
+    if(n > 0)
+        hipLaunchKernelGGL(k_coo_unhash, dim3(grid_for(n, 256)), dim3(256), 0, s, hcount.data(),
+                           hlist.data(), hkeys.data(), hvals.data(), r.data(), c.data(),
+                           v.data(), count.data());
+    SG_LAUNCH_CHECK();
+    live = n;
+    // keep the table at most half full for the next assembly (the slots
+    // of this one were cleared above, so a fresh larger table is equivalent)
+    if(2 * (size_t)n > hcap)
+    {
+        size_t want = hcap;
+        while(want < 4 * (size_t)n)
+            want <<= 1;
+        hcap = 0;
+        grow_table(want, s);
+    }
+    return true;
 }
 
 void Coo::build_rows(hipStream_t s)

@@ -87,10 +87,14 @@ struct SimDev
     unsigned* pcount   = nullptr;
     int       pair_cap = 0;
     // COO tail
-    int *     coo_r = nullptr, *coo_c = nullptr;
-    double*   coo_v     = nullptr;
-    unsigned* coo_count = nullptr;
-    int       coo_cap   = 0;
+    // contact blocks outside the static pattern: open-addressing table keyed
+    // by (row, col), merged as they are inserted (owned by Coo)
+    unsigned long long* hkeys  = nullptr;
+    double*             hvals  = nullptr;
+    int*                hlist  = nullptr;
+    unsigned*           hcount = nullptr;
+    int*                hfail  = nullptr;
+    unsigned long long  hmask  = 0;
     int*      overflow  = nullptr;  // [0] pairs, [1] coo, [2] ccd
     FricDev   fric;
     ClothDev  cloth;
