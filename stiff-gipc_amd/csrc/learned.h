@@ -68,7 +68,8 @@ class Coarse
 
   private:
     void ensure(int nc, int nn);
-    int           L_ = 0, nc_ = 0, off_ = 0, cap_ = 0, ncap_ = 0, nn_ = 0;
+    int           L_ = 0, lev_ = 0, nc_ = 0, off_ = 0, cap_ = 0, ncap_ = 0, nn_ = 0;
+    static constexpr int kMaxAggregateNodes = 4096;
     const int*    chain_ = nullptr;
     const double* x_     = nullptr;
     DBuf<int>      agg_, nodes_, ptr_, okf_;

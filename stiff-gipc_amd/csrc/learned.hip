@@ -37,8 +37,8 @@ __device__ __forceinline__ uint16_t f_to_bf16(float f)
 }
 
 // ------------------------------------------------------------- aggregates
-// aggregate of every node: free FEM nodes of a coarsest-level aggregate
-__global__ void k_agg_of(int nn, int nfem, const int* __restrict__ btype, int L,
+// aggregate of every node: free FEM nodes of an aggregate of level lev
+__global__ void k_agg_of(int nn, int nfem, const int* __restrict__ btype, int L, int lev,
                          const int* __restrict__ chain, int off, int nc, int* __restrict__ agg,
                          unsigned* __restrict__ cnt)
 {
@@ -46,7 +46,7 @@ __global__ void k_agg_of(int nn, int nfem, const int* __restrict__ btype, int L,
     int c = -1;
     if(v < nfem && btype[v] == 0)
     {
-        c = chain[(size_t)v * (L - 1) + L - 2] - off;
+        c = chain[(size_t)v * (L - 1) + lev - 1] - off;
         if(c < 0 || c >= nc)
             c = -1;
     }
@@ -670,8 +670,14 @@ bool Coarse::setup(const Mas& mas, hipStream_t s)
     if(L_ < 2 || !mas.ready())
         return false;
     const auto& lv = mas.level_sizes();
-    nc_            = lv[2 * (L_ - 1)];
-    off_           = lv[2 * (L_ - 1) + 1];
+    // the coarsest level whose aggregates hold at most kMaxAggregateNodes
+    // nodes on average (a connected body is a single node at the coarsest
+    // level, which would leave one affine block for the whole body)
+    lev_ = L_ - 1;
+    while(lev_ > 1 && (long long)lv[2 * lev_] * kMaxAggregateNodes < (long long)lv[0])
+        lev_--;
+    nc_  = lv[2 * lev_];
+    off_ = lv[2 * lev_ + 1];
     chain_         = mas.chain_dev();
     (void)s;
     return nc_ > 0;
@@ -684,7 +690,7 @@ void Coarse::build(int nn, int nfem, const int* btype, const double* x, const Bs
     SG_HIP(hipMemsetAsync(cnt_.data(), 0, (size_t)nc_ * sizeof(unsigned), s));
     SG_HIP(hipMemsetAsync(cur_.data(), 0, (size_t)nc_ * sizeof(unsigned), s));
     hipLaunchKernelGGL(k_agg_of, dim3(grid_for(nn, 256)), dim3(256), 0, s, nn, nfem, btype, L_,
-                       chain_, off_, nc_, agg_.data(), cnt_.data());
+                       lev_, chain_, off_, nc_, agg_.data(), cnt_.data());
     std::vector<unsigned> h(nc_);
     cnt_.download(h.data(), nc_, s);
     SG_HIP(hipStreamSynchronize(s));

@@ -60,7 +60,8 @@ def test_mlp_training_reduces_loss(oracle):
 def test_affine_guess_matches_numpy(oracle):
     """features (q_c / |q_c|, r_c / |r_c|) and targets of the coarse systems
     of one Newton solve against a numpy restatement: aggregates from the
-    oracle's MAS on the GPU-assembled matrix, P_c = [J_i] over free nodes,
+    oracle's MAS on the GPU-assembled matrix (the coarsest level with at most
+    4096 nodes per aggregate), P_c = [J_i] over free nodes,
     q_c = (P^T A P)^-1 P^T b, target = (lsq fit of dx - q_c) / |q_c|"""
     import sgipc
     sc = _cubes()
@@ -80,7 +81,12 @@ def test_affine_guess_matches_numpy(oracle):
     A = oracle.bsr_from_upper_triplets(nb, rows[up], cl[up], vl[up].reshape(-1, 9))
     mas = oracle.Mas(A, sc.part_id, fp32=False)
     L = mas.levels
-    top = mas.chain()[:, L - 2]
+    # aggregates: the coarsest level with at most 4096 nodes per aggregate
+    sizes = np.asarray(mas.level_sizes()).reshape(-1, 2)
+    lev = L - 1
+    while lev > 1 and sizes[lev, 0] * 4096 < sizes[0, 0]:
+        lev -= 1
+    top = mas.chain()[:, lev - 1]
     H = A.dense()
     dx = np.linalg.solve(H, g).reshape(-1, 3)
     feats, targs = [], []
