@@ -173,48 +173,49 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(int n, int shift,
     hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of hist (digit-major, 256 * nblk entries), one workgroup
+// exclusive scan of hist (digit-major, 256 * nblk entries), one workgroup:
+// thread t owns the contiguous chunk [t C, (t + 1) C), C = ceil(total / 1024);
+// chunk sums are scanned across the workgroup, then every chunk is rescanned
+// from its offset (two passes over the data instead of one block scan with
+// three barriers per 1024 entries)
 __global__ void __launch_bounds__(1024) k_sort_scan(int total, unsigned* __restrict__ hist)
 {
     __shared__ unsigned wsum[16];
-    __shared__ unsigned carry_s;
-    if(threadIdx.x == 0)
-        carry_s = 0;
-    __syncthreads();
-    for(int base = 0; base < total; base += 1024)
+    const int C  = (total + 1023) / 1024;
+    const int lo = threadIdx.x * C;
+    const int hi = lo + C < total ? lo + C : total;
+    unsigned  v  = 0;
+    for(int i = lo; i < hi; i++)
+        v += hist[i];
+    unsigned s = v;  // inclusive scan of the chunk sums
+    for(int o = 1; o < 64; o <<= 1)
     {
-        int      i = base + threadIdx.x;
-        unsigned v = (i < total) ? hist[i] : 0;
-        unsigned s = v;
-        for(int o = 1; o < 64; o <<= 1)
+        unsigned u = __shfl_up(s, o, 64);
+        if((int)__lane_id() >= o)
+            s += u;
+    }
+    const int w = threadIdx.x >> 6;
+    if(__lane_id() == 63)
+        wsum[w] = s;
+    __syncthreads();
+    if(threadIdx.x < 16)
+    {
+        unsigned ws = wsum[threadIdx.x];
+        for(int o = 1; o < 16; o <<= 1)
         {
-            unsigned u = __shfl_up(s, o, 64);
-            if((int)__lane_id() >= o)
-                s += u;
+            unsigned u = __shfl_up(ws, o, 16);
+            if((int)threadIdx.x >= o)
+                ws += u;
         }
-        int w = threadIdx.x >> 6;
-        if(__lane_id() == 63)
-            wsum[w] = s;
-        __syncthreads();
-        if(threadIdx.x < 16)
-        {
-            unsigned ws = wsum[threadIdx.x];
-            for(int o = 1; o < 16; o <<= 1)
-            {
-                unsigned u = __shfl_up(ws, o, 16);
-                if((int)threadIdx.x >= o)
-                    ws += u;
-            }
-            wsum[threadIdx.x] = ws;
-        }
-        __syncthreads();
-        unsigned excl = s - v + (w > 0 ? wsum[w - 1] : 0) + carry_s;
-        if(i < total)
-            hist[i] = excl;
-        __syncthreads();
-        if(threadIdx.x == 1023)
-            carry_s = excl + v;
-        __syncthreads();
+        wsum[threadIdx.x] = ws;
+    }
+    __syncthreads();
+    unsigned run = s - v + (w > 0 ? wsum[w - 1] : 0);
+    for(int i = lo; i < hi; i++)
+    {
+        unsigned h = hist[i];
+        hist[i]    = run;
+        run += h;
     }
 }
 

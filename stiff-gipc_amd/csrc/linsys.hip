@@ -10,6 +10,7 @@
 // host reads back only at the convergence test.
 #include "linsys.h"
 
+#include <cstdlib>
 #include <cstring>
 
 namespace sgipc
@@ -203,7 +204,19 @@ void Coo::reserve(int nrows, hipStream_t s)
     tiles.resize((nrows + 1023) / 1024 + 2);
     hcount.resize(1);
     hfail.resize(1);
-    grow_table(size_t(1) << 16, s);
+    // initial table size: 64 K slots, or SGIPC_CONTACT_TABLE_SLOTS (a power
+    // of two >= 16; small values exercise the grow-and-reassemble path)
+    size_t      slots = size_t(1) << 16;
+    const char* env   = getenv("SGIPC_CONTACT_TABLE_SLOTS");
+    if(env)
+    {
+        long long v = atoll(env);
+        SG_CHECK(v >= 16 && (v & (v - 1)) == 0, -1,
+                 "SGIPC_CONTACT_TABLE_SLOTS must be a power of two >= 16");
+        slots = (size_t)v;
+    }
+    hcap = 0;
+    grow_table(slots, s);
     ensure_entries(1 << 15);
     count.zero(s);
     hcount.zero(s);

@@ -198,3 +198,44 @@ def test_hybrid_impact_steps(oracle):
         d0 = np.linalg.norm(x0[abd == b][:, None] - x0[abd == b][None], axis=2)
         d = np.linalg.norm(xg[abd == b][:, None] - xg[abd == b][None], axis=2)
         assert np.abs(d - d0).max() < 1e-6
+
+
+def test_contact_table_growth(oracle, monkeypatch):
+    """a contact table of 16 slots (SGIPC_CONTACT_TABLE_SLOTS) runs full on
+    the first assemblies: the table grows and the assembly is redone until it
+    fits; the node system still equals the oracle's (affine boxes on a soft
+    box, rotated and sheared, contacts between all of them)"""
+    import sgipc
+    monkeypatch.setenv("SGIPC_CONTACT_TABLE_SLOTS", "16")
+    sc = _scene(n_abd=2, gap=4e-4)
+    sc.kappa = 1e4
+    x = sc.x.copy()
+    abd = sc.abd_body
+    rng = np.random.default_rng(5)
+    for b in (0, 1):
+        c = x[abd == b].mean(0)
+        A = np.eye(3) + 0.01 * rng.normal(size=(3, 3))
+        x[abd == b] = c + (x[abd == b] - c) @ A.T
+    sim = sgipc.Simulation(sc)
+    sim.set_positions(x)
+    w = oracle.World(sc)
+    w.set_positions(x)
+    g, rp, cl, vl = sim.assemble()
+    go, Ao = w.assemble()
+    Ag = _dense(sim.nodes, rp, cl, vl)
+    ncoo = sim.sizes()["coo_blocks"]
+    print(f"[table-growth] contact blocks={ncoo} g={rel(g, go):.2e} H={rel(Ag, Ao.dense()):.2e}",
+          flush=True)
+    assert ncoo > 16
+    assert rel(g, go) < 1e-10
+    assert rel(Ag, Ao.dense()) < 1e-10
+    # a landing step with the small initial table, against the oracle
+    sc2 = _scene(n_abd=2)
+    sim2 = sgipc.Simulation(sc2)
+    w2 = oracle.World(sc2)
+    for step in range(3):
+        sim2.time_step()
+        w2.time_step()
+    err = np.abs(sim2.positions() - w2.positions()).max() / np.abs(w2.positions()).max()
+    print(f"[table-growth] 3 steps err={err:.2e}", flush=True)
+    assert err < 1e-8
