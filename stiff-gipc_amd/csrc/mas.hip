@@ -369,50 +369,62 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
 }
 
 // One wave per bank: load 48x48, zero diagonal -> 1, in-place Gauss-Jordan
-// (no pivoting: bank matrices are SPD), store the transpose so the apply's
-// per-lane dot products read contiguous rows.
+// (no pivoting: bank matrices are SPD).  Lane c keeps column c in registers
+// (the pivot loop is unrolled so every row index is a constant); at step k
+// lane k publishes the pivot column to LDS and every lane reads it back as
+// broadcasts, so the only LDS traffic is 48 words per step.  The result is
+// staged through LDS once to store it transposed.
 template <typename T>
 __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B, T* __restrict__ inv)
 {
-    __shared__ double M[BD][BD + 1];
-    __shared__ double colk[BD];
+    __shared__ double colk[2][BD];
+    __shared__ double Mt[BD][BD + 1];
     const int     bank = blockIdx.x;
     const int     c    = threadIdx.x;
+    const int     cc   = c < BD ? c : 0;  // lanes 48..63 shadow column 0 (never stored)
     const double* src  = B + (size_t)bank * BD * BD;
-    if(c < BD)
-        for(int r = 0; r < BD; r++)
-        {
-            double v = src[r * BD + c];
-            if(r == c && v == 0.0)
-                v = 1.0;
-            M[r][c] = v;
-        }
-    __syncthreads();
+    double        m[BD];
+#pragma unroll
+    for(int r = 0; r < BD; r++)
+    {
+        double v = src[r * BD + cc];
+        if(r == cc && v == 0.0)
+            v = 1.0;
+        m[r] = v;
+    }
+#pragma clang loop unroll(full)
     for(int k = 0; k < BD; k++)
     {
-        if(c < BD)
-            colk[c] = M[c][k];
-        __syncthreads();
-        double piv = colk[k];
+        double* ck = colk[k & 1];
+        if(c == k)
+#pragma unroll
+            for(int r = 0; r < BD; r++)
+                ck[r] = m[r];
+        __builtin_amdgcn_wave_barrier();  // one wave: LDS accesses stay in order
+        double piv = ck[k];
         double ip  = 1.0 / piv;
-        if(c < BD)
+        if(c == k)
         {
-            if(c == k)
-            {
-                for(int r = 0; r < BD; r++)
-                    M[r][k] = (r == k) ? ip : -colk[r] * ip;
-            }
-            else
-            {
-                double a = M[k][c] * ip;
-                for(int r = 0; r < BD; r++)
-                    if(r != k)
-                        M[r][c] -= colk[r] * a;
-                M[k][c] = a;
-            }
+#pragma unroll
+            for(int r = 0; r < BD; r++)
+                m[r] = (r == k) ? ip : -ck[r] * ip;
         }
-        __syncthreads();
+        else
+        {
+            double a = m[k] * ip;
+#pragma unroll
+            for(int r = 0; r < BD; r++)
+                if(r != k)
+                    m[r] -= ck[r] * a;
+            m[k] = a;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+    if(c < BD)
+#pragma unroll
+        for(int r = 0; r < BD; r++)
+            Mt[r][c] = m[r];
+    __builtin_amdgcn_wave_barrier();
     // Stored transposed (dst[u][t] = Binv[t][u], so that the lanes of a wave
     // read a row with unit stride), keeping only the upper 3x3 blocks of the
     // inverse, as the reference's MasMatrixSymf does (eigen_data.h:125-128,
@@ -421,7 +433,7 @@ __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B,
     T* dst = inv + (size_t)bank * BD * BD;
     if(c < BD)
         for(int r = 0; r < BD; r++)
-            dst[r * BD + c] = (T)((r / 3 >= c / 3) ? M[c][r] : M[r][c]);
+            dst[r * BD + c] = (T)((r / 3 >= c / 3) ? Mt[c][r] : Mt[r][c]);
 }
 
 // ---------------------------------------------------------------- apply
