@@ -472,24 +472,6 @@ __device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T r
     return bank_apply(pv, rv);
 }
 
-// Same product with R_u read from LDS (rs[0..48), a broadcast read per u).
-template <typename T>
-__device__ __forceinline__ T bank_product_lds(const T* __restrict__ inv, int gb, const T* rs)
-{
-    const int t  = __lane_id();
-    const int tc = t < BD ? t : 0;
-    const T*  P  = inv + (size_t)gb * BD * BD + tc;
-    T         pv[BD];
-#pragma unroll
-    for(int u = 0; u < BD; u++)
-        pv[u] = P[u * BD];
-    T acc = 0;
-#pragma unroll
-    for(int u = 0; u < BD; u++)
-        acc += pv[u] * rs[u];
-    return acc;
-}
-
 // Cluster sums of one bank's R into the next level (plain stores: the bank
 // owns those nodes), and optionally into the level after it (atomics: the
 // kernel two levels up reads them after this launch).  rv: lane t < 48 holds
@@ -694,37 +676,45 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         return;
     }
     // ---- tail workgroup
-    __shared__ T Rs[3 * kTailNodes];
+    // Everything the tail reads from global memory is requested up front:
+    // R_lt, goingNext over the tail's node span (into LDS) and this wave's
+    // bank inverse (each wave owns at most one tail bank: the launch gives
+    // ceil(banks / kCoarseWaves) tail workgroups), so the chain is one
+    // memory round trip, the LDS restriction, and the product.
+    __shared__ T   Rs[3 * kTailNodes];
+    __shared__ int up[kTailNodes];
     const int base = lv.off[lt];
     const int span = lv.off[L - 1] + lv.count[L - 1] - base;
     const int nb   = (span + BANK - 1) / BANK;  // banks of the tail levels
+    const int b0   = base / BANK;
+    const int bi   = (blockIdx.x - nwg_b) * kCoarseWaves + w;
+    T         pv[BD];
+    bank_load(inv, b0 + (bi < nb ? bi : 0), pv);
+    const int nc = lv.count[lt];
     for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
-        Rs[i] = (T)0;
-    __syncthreads();
-    {
-        const int nc = lv.count[lt];
-        for(int i = threadIdx.x; i < 3 * nc; i += blockDim.x)
-            Rs[i] = R[3 * (size_t)base + i];
-    }
+        Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
+    for(int i = threadIdx.x; i < span; i += blockDim.x)
+        up[i] = going_next[base + i];
     __syncthreads();
     for(int l = lt; l + 1 < L; l++)
     {
-        const int n0 = lv.off[l], nc = lv.count[l];
-        for(int i = threadIdx.x; i < 3 * nc; i += blockDim.x)
+        const int n0 = lv.off[l] - base, nl = lv.count[l];
+        for(int i = threadIdx.x; i < 3 * nl; i += blockDim.x)
         {
             int g = n0 + i / 3;
-            atomicAdd(&Rs[3 * (going_next[g] - base) + i % 3], Rs[3 * (g - base) + i % 3]);
+            atomicAdd(&Rs[3 * (up[g] - base) + i % 3], Rs[3 * g + i % 3]);
         }
         __syncthreads();
     }
     // the tail workgroups split the tail banks (each restricted R redundantly)
-    const int b0  = base / BANK;
-    const int tw  = blockIdx.x - nwg_b;
-    const int ntw = gridDim.x - nwg_b;
-    for(int bi = tw * kCoarseWaves + w; bi < nb; bi += ntw * kCoarseWaves)
+    if(bi < nb)
     {
         const int gb = b0 + bi;
-        T         z  = bank_product_lds(inv, gb, Rs + 3 * (gb * BANK - base));
+        const T*  rs = Rs + 3 * (gb * BANK - base);
+        T         z  = 0;
+#pragma unroll
+        for(int u = 0; u < BD; u++)
+            z += pv[u] * rs[u];
         Z[(size_t)gb * ZB + t] = z;
     }
 }
