@@ -368,72 +368,166 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
     }
 }
 
-// One wave per bank: load 48x48, zero diagonal -> 1, in-place Gauss-Jordan
-// (no pivoting: bank matrices are SPD).  Lane c keeps column c in registers
-// (the pivot loop is unrolled so every row index is a constant); at step k
-// lane k publishes the pivot column to LDS and every lane reads it back as
-// broadcasts, so the only LDS traffic is 48 words per step.  The result is
-// staged through LDS once to store it transposed.
+// Ordering of LDS between the lanes of a single-wave workgroup: the LDS
+// instructions of one wave execute in program order, so a compiler-only
+// wave barrier suffices; __syncthreads would also make the wave wait for its
+// outstanding global stores (workgroup-scope release).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+}
+
+
+// One wave per bank: load 48x48 into LDS, zero diagonal -> 1, in-place
+// Gauss-Jordan without pivoting (bank matrices are SPD), blocked by 16x16
+// tiles so that three quarters of the work runs on FP64 MFMA.  Eliminating
+// the 16 pivots of tile k is, in exact arithmetic, the block step
+//   P = A_kk,  A_kj <- P^-1 A_kj,  A_ij <- A_ij - A_ik P^-1 A_kj,
+//   A_ik <- -A_ik P^-1,  A_kk <- P^-1          (i, j != k),
+// the same operator as the 48 scalar pivot steps.  P^-1 is formed by those
+// scalar steps on the tile alone (same per-element operations as an
+// unblocked elimination); the products are 16x16x16 MFMA tiles
+// (v_mfma_f64_16x16x4_f64, four per tile product).
+//
+// MFMA f64 16x16x4 operand layout (lane l, g = l / 16, j = l % 16):
+// A operand A[j][g], B operand B[g][j], accumulator element v of lane l is
+// C[g + 4 v][j] (rows interleaved across the lane groups).
+constexpr int kInvLd = BD + 1;  // LDS row stride (doubles) of the bank matrix
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// acc += sgn * X Y over one 16-deep tile product, X at (xr, xc), Y at
+// (yr, yc) of the LDS matrix S
+__device__ __forceinline__ dbl4 tile_mma(const double* S, int xr, int xc, int yr, int yc,
+                                         double sgn, dbl4 acc)
+{
+    const int l = __lane_id();
+    const int g = l >> 4, j = l & 15;
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+    {
+        double a = sgn * S[(xr + j) * kInvLd + xc + 4 * q + g];
+        double b = S[(yr + 4 * q + g) * kInvLd + yc + j];
+        acc      = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ dbl4 tile_load(const double* S, int r0, int c0)
+{
+    const int l = __lane_id();
+    const int g = l >> 4, j = l & 15;
+    dbl4      t;
+#pragma unroll
+    for(int v = 0; v < 4; v++)
+        t[v] = S[(r0 + g + 4 * v) * kInvLd + c0 + j];
+    return t;
+}
+
+__device__ __forceinline__ void tile_store(double* S, int r0, int c0, dbl4 t)
+{
+    const int l = __lane_id();
+    const int g = l >> 4, j = l & 15;
+#pragma unroll
+    for(int v = 0; v < 4; v++)
+        S[(r0 + g + 4 * v) * kInvLd + c0 + j] = t[v];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64) k_mas_invert(const double* __restrict__ B, T* __restrict__ inv)
 {
-    __shared__ double colk[2][BD];
-    __shared__ double Mt[BD][BD + 1];
+    __shared__ double S[BD * kInvLd];
     const int     bank = blockIdx.x;
-    const int     c    = threadIdx.x;
-    const int     cc   = c < BD ? c : 0;  // lanes 48..63 shadow column 0 (never stored)
+    const int     l    = threadIdx.x;
+    const int     g    = l >> 4, j = l & 15;
     const double* src  = B + (size_t)bank * BD * BD;
-    double        m[BD];
-#pragma unroll
-    for(int r = 0; r < BD; r++)
+    for(int e = l; e < BD * BD; e += 64)
     {
-        double v = src[r * BD + cc];
-        if(r == cc && v == 0.0)
-            v = 1.0;
-        m[r] = v;
+        int    r = e / BD, c = e % BD;
+        double v = src[e];
+        S[r * kInvLd + c] = (r == c && v == 0.0) ? 1.0 : v;
     }
-#pragma clang loop unroll(full)
-    for(int k = 0; k < BD; k++)
+    wave_lds_sync();
+#pragma unroll
+    for(int k = 0; k < 3; k++)
     {
-        double* ck = colk[k & 1];
-        if(c == k)
+        const int o = 16 * k;
+        // P^-1 in registers: lane l holds P[g + 4 v][j]
+        dbl4 t = tile_load(S, o, o);
 #pragma unroll
-            for(int r = 0; r < BD; r++)
-                ck[r] = m[r];
-        __builtin_amdgcn_wave_barrier();  // one wave: LDS accesses stay in order
-        double piv = ck[k];
-        double ip  = 1.0 / piv;
-        if(c == k)
+        for(int p = 0; p < 16; p++)
         {
+            const int pg = p & 3, pv = p >> 2;  // row p = pg + 4 pv
+            double    prow = __shfl(t[pv], 16 * pg + j, 64);  // P[p][j]
+            double    piv  = __shfl(t[pv], 16 * pg + p, 64);  // P[p][p]
+            double    pcol[4];
 #pragma unroll
-            for(int r = 0; r < BD; r++)
-                m[r] = (r == k) ? ip : -ck[r] * ip;
+            for(int v = 0; v < 4; v++)
+                pcol[v] = __shfl(t[v], 16 * g + p, 64);  // P[g + 4 v][p]
+            double ip = 1.0 / piv;
+            double a  = prow * ip;
+#pragma unroll
+            for(int v = 0; v < 4; v++)
+            {
+                const bool prow_here = (g + 4 * v == p);
+                if(j == p)
+                    t[v] = prow_here ? ip : -pcol[v] * ip;
+                else
+                    t[v] = prow_here ? a : t[v] - pcol[v] * a;
+            }
         }
-        else
+        tile_store(S, o, o, t);
+        wave_lds_sync();
+        // row panel: A_kj <- P^-1 A_kj (each product reads A_kj before its
+        // own store; LDS operations of one wave complete in program order)
+#pragma unroll
+        for(int jt = 0; jt < 3; jt++)
         {
-            double a = m[k] * ip;
-#pragma unroll
-            for(int r = 0; r < BD; r++)
-                if(r != k)
-                    m[r] -= ck[r] * a;
-            m[k] = a;
+            if(jt == k)
+                continue;
+            dbl4 u = {0.0, 0.0, 0.0, 0.0};
+            u      = tile_mma(S, o, o, o, 16 * jt, 1.0, u);
+            wave_lds_sync();
+            tile_store(S, o, 16 * jt, u);
+            wave_lds_sync();
         }
-        __builtin_amdgcn_wave_barrier();
+        // trailing tiles: A_ij -= A_ik (P^-1 A_kj)
+#pragma unroll
+        for(int it = 0; it < 3; it++)
+#pragma unroll
+            for(int jt = 0; jt < 3; jt++)
+            {
+                if(it == k || jt == k)
+                    continue;
+                dbl4 c = tile_load(S, 16 * it, 16 * jt);
+                c      = tile_mma(S, 16 * it, o, o, 16 * jt, -1.0, c);
+                wave_lds_sync();
+                tile_store(S, 16 * it, 16 * jt, c);
+                wave_lds_sync();
+            }
+        // column panel: A_ik <- -A_ik P^-1
+#pragma unroll
+        for(int it = 0; it < 3; it++)
+        {
+            if(it == k)
+                continue;
+            dbl4 c = {0.0, 0.0, 0.0, 0.0};
+            c      = tile_mma(S, 16 * it, o, o, o, -1.0, c);
+            wave_lds_sync();
+            tile_store(S, 16 * it, o, c);
+            wave_lds_sync();
+        }
     }
-    if(c < BD)
-#pragma unroll
-        for(int r = 0; r < BD; r++)
-            Mt[r][c] = m[r];
-    __builtin_amdgcn_wave_barrier();
     // Stored transposed (dst[u][t] = Binv[t][u], so that the lanes of a wave
     // read a row with unit stride), keeping only the upper 3x3 blocks of the
     // inverse, as the reference's MasMatrixSymf does (eigen_data.h:125-128,
     // written by __inverse6_P96x96, MASPreconditioner.cu:628-724): below the
     // block diagonal the operator is the transpose of the upper blocks
     T* dst = inv + (size_t)bank * BD * BD;
-    if(c < BD)
-        for(int r = 0; r < BD; r++)
-            dst[r * BD + c] = (T)((r / 3 >= c / 3) ? Mt[c][r] : Mt[r][c]);
+    for(int e = l; e < BD * BD; e += 64)
+    {
+        int r = e / BD, c = e % BD;
+        dst[e] = (T)((r / 3 >= c / 3) ? S[c * kInvLd + r] : S[r * kInvLd + c]);
+    }
 }
 
 // ---------------------------------------------------------------- apply
@@ -477,15 +571,6 @@ __device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T r
 // kernel two levels up reads them after this launch).  rv: lane t < 48 holds
 // component t % 3 of bank node t / 3; node_up: global id of that node's
 // parent (or -1).
-// Ordering of LDS between the lanes of a single-wave workgroup: the LDS
-// instructions of one wave execute in program order, so a compiler-only
-// wave barrier suffices; __syncthreads would also make the wave wait for its
-// outstanding global stores (workgroup-scope release).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_wave_barrier();
-}
-
 // up2_node: going_next[first + t / 3] for t < 3 ncl when up2 (the caller may
 // have loaded it early), else unused.  Single-wave workgroups only.
 template <typename T>
