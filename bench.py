@@ -212,6 +212,8 @@ def main():
                          "pcg_iterations_per_newton": round(adv["pcg_iterations"] / args.steps, 2),
                          "max_rel_residual": adv["max_pcg_rel_residual"],
                          "fp64_fallbacks": adv["solver_fallbacks"]}
+        if lmode:
+            results[name]["warm_starts"] = adv["warm_starts"]
 
         log(f"timed [{name}]: {args.steps} Newton iterations in {(t1 - t0):.3f} s -> "
             f"{ms:.3f} ms/iter; pcg/iter={adv['pcg_iterations'] / args.steps:.1f} "

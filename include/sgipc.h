@@ -188,6 +188,8 @@ typedef struct sgipc_advance_stats {
     int32_t solver_fallbacks;  /* mixed-mode solves redone in FP64          */
     double  max_pcg_rel_residual;
     double  ms_total;
+    int32_t warm_starts;       /* solves continued from a coarse guess      */
+    int32_t pad;
 } sgipc_advance_stats;
 
 /* Advance the integration by exactly n Newton iterations that solve (the
@@ -218,6 +220,12 @@ int sgipc_profile_query(sgipc_t ctx, const char* kernel, int32_t* launches,
 #define SGIPC_MLP_OUT 12
 /* mode 0 off, 1 affine Galerkin guess only, 2 guess corrected by the network */
 int sgipc_learned_mode(sgipc_t ctx, int32_t mode);
+/* on (the default): each warm start's net gain is its predicted saving (from
+ * the guess's preconditioned residual and PCG's linear convergence) minus its
+ * own device time; while the running average (weight 1/2) of the net gains is
+ * negative, the guesses are skipped for the next 2 solves, the pause doubling
+ * up to 32; off: every solve computes and judges the guesses */
+int sgipc_warm_start_adaptive(sgipc_t ctx, int32_t on);
 /* training samples from the last solving Newton iteration: features
  * [n][SGIPC_MLP_IN] and targets [n][SGIPC_MLP_OUT] ((q* - q_c) / |q_c| with q*
  * the least-squares affine fit of the solution on the aggregate) */

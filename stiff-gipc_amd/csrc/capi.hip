@@ -400,6 +400,11 @@ int sgipc_learned_mode(sgipc_t h, int32_t mode)
     return guard([&] { ctx_of(h).set_learned(mode); });
 }
 
+int sgipc_warm_start_adaptive(sgipc_t h, int32_t on)
+{
+    return guard([&] { ctx_of(h).set_warm_adaptive(on); });
+}
+
 int sgipc_learned_samples(sgipc_t h, float* feat, float* target, int32_t cap, int32_t* n)
 {
     return guard([&] {

@@ -43,6 +43,7 @@ class Context
     // learned coarse warm start: mode 0 off, 1 Galerkin-Jacobi guess, 2 network
     void set_learned(int mode);
     int  learned_samples(float* feat, float* target, int cap);
+    void set_warm_adaptive(int on);
     double mlp_train(const float* X, const float* T, int n, int epochs, double lr)
     {
         return mlp_.train(X, T, n, epochs, lr, s_);
@@ -116,6 +117,11 @@ class Context
     Mlp          mlp_;
     Coarse       coarse_;
     int          learned_ = 0;
+    // adaptive warm start: solves left to run without the guess, the current
+    // pause length (doubles while the average stays negative), the probes
+    // judged and their averaged net gain in ms
+    int          warm_adaptive_ = 1, warm_skip_ = 0, warm_pause_ = 0, warm_probes_ = 0;
+    double       warm_gain_     = 0.0;
     DBuf<double> x0_, b2_, x1_, b3_, zt_;
     PcgStats     solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
                               int* warm);

@@ -1173,26 +1173,49 @@ void Context::newton_iteration(sgipc_iter_stats* st)
 // their preconditioned residuals, the lower one is used when it beats the
 // zero guess, and the zero-guess solve runs whenever the warm solve does not
 // converge.
+//
+// Adaptive mode (the default): the guesses cost a Galerkin build, a coarse
+// solve and one MAS apply and residual per guess, timed on the device.  From
+// the chosen guess's preconditioned residual q_g against the zero guess's
+// q_0, PCG's linear convergence predicts the fraction of iterations saved,
+// f = log(q_0 / q_g) / log(q_0 / threshold), hence a saving of
+// f / (1 - f) times the solve that ran (none when the zero guess won).  The
+// net gain (saving minus cost) is averaged over the probes with weight 1/2;
+// while that average is negative the warm start is paused for the next 2
+// solves, the pause doubling (up to 32) while it stays negative.  Occasional
+// rejected guesses among paying ones therefore do not pause it.
 PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
                                int* warm)
 {
     *warm = 0;
+    if(learned_ && pc == Precond::Mas && warm_adaptive_ && warm_skip_ > 0)
+    {
+        warm_skip_--;
+        return pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+    }
     if(learned_ && pc == Precond::Mas && coarse_.setup(mas_, s_))
     {
+        hipEvent_t ev[3];
+        for(auto& e : ev)
+            SG_HIP(hipEventCreate(&e));
+        SG_HIP(hipEventRecord(ev[0], s_));
         size_t n = 3 * (size_t)nn_;
         x0_.resize(n);
         b2_.resize(n);
         zt_.resize(n);
-        double* q = scratch_.data() + 8;  // q[0] plain, q[1] affine, q[2] learned
-        SG_HIP(hipMemsetAsync(q, 0, 3 * sizeof(double), s_));
-        // r.z (or r.r) of a right-hand side into q[k]
+        // q[0] plain, q[1] affine, q[2] learned in the stop test's norm (r.z
+        // or r.r); q[3 + k]: r.z of the same when the stop test uses r.r and
+        // the adaptive mode judges (PCG's convergence is measured in r.z)
+        double*    q      = scratch_.data() + 8;
+        const bool zjudge = warm_adaptive_ && !cfg.use_rz;
+        SG_HIP(hipMemsetAsync(q, 0, 6 * sizeof(double), s_));
         auto measure = [&](const double* r, int k) {
-            if(cfg.use_rz)
+            if(cfg.use_rz || zjudge)
             {
                 mas_.apply(r, zt_.data(), s_);
-                launch_dot((int)n, r, zt_.data(), q + k, s_);
+                launch_dot((int)n, r, zt_.data(), q + (cfg.use_rz ? k : 3 + k), s_);
             }
-            else
+            if(!cfg.use_rz)
                 launch_dot((int)n, r, r, q + k, s_);
         };
         coarse_.build(nn_, nfem_, btype_.data(), x_.data(), A_, &coo_, rhs_.data(), s_);
@@ -1209,7 +1232,8 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
             residual(S, rhs_.data(), x1_.data(), b3_.data(), s_);
             measure(b3_.data(), 2);
         }
-        double h[3];
+        SG_HIP(hipEventRecord(ev[1], s_));
+        double h[6];
         SG_HIP(hipMemcpyAsync(h, q, sizeof(h), hipMemcpyDeviceToHost, s_));
         SG_HIP(hipStreamSynchronize(s_));
         int     pick = 1;  // the guess with the lowest preconditioned residual
@@ -1223,6 +1247,25 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
             xg   = x1_.data();
             bg   = b3_.data();
         }
+        // judge a probe: predicted saving against the measured cost
+        auto judge = [&](double f) {
+            SG_HIP(hipEventRecord(ev[2], s_));
+            SG_HIP(hipEventSynchronize(ev[2]));
+            double cost = now_ms(ev[0], ev[1]), solve = now_ms(ev[1], ev[2]);
+            for(auto& e : ev)
+                (void)hipEventDestroy(e);
+            if(!warm_adaptive_)
+                return;
+            double net = solve * f / (1.0 - f) - cost;
+            warm_gain_ = warm_probes_++ ? 0.5 * (warm_gain_ + net) : net;
+            if(warm_gain_ < 0.0)
+            {
+                warm_pause_ = warm_pause_ ? std::min(2 * warm_pause_, 32) : 2;
+                warm_skip_  = warm_pause_;
+            }
+            else
+                warm_pause_ = 0;
+        };
         if(h[0] > 0.0 && qb < h[0])
         {
             PcgConfig c = cfg;
@@ -1233,18 +1276,39 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
                 launch_axpy_d((int)n, 1.0, xg, sol, s_);
                 ps.rel_residual = pcg_.true_residual(S, rhs_.data(), sol, s_);
                 *warm           = pick;
+                double rel2 = cfg.rel_tol * cfg.rel_tol, abs2 = cfg.abs_tol * cfg.abs_tol;
+                double thr  = std::max(rel2 * h[0], abs2);
+                double z0 = zjudge ? h[3] : h[0], zg = zjudge ? h[3 + pick] : qb;
+                double f  = (thr < h[0] && z0 > 0.0 && zg > 0.0)
+                                ? std::log(z0 / zg) / std::log(h[0] / thr)
+                                : 0.0;
+                judge(std::min(std::max(f, 0.0), 0.95));
                 return ps;
             }
         }
+        PcgStats ps = pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+        judge(0.0);
+        return ps;
     }
     return pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+}
+
+void Context::set_warm_adaptive(int on)
+{
+    warm_adaptive_ = on ? 1 : 0;
+    warm_skip_     = 0;
+    warm_pause_    = 0;
+    warm_probes_   = 0;
 }
 
 void Context::set_learned(int mode)
 {
     SG_CHECK(mode >= 0 && mode <= 2, SGIPC_EINVAL, "learned mode must be 0, 1 or 2");
     SG_CHECK(mode < 2 || mlp_.ready(), SGIPC_EINVAL, "no trained network (train or set it first)");
-    learned_ = mode;
+    learned_    = mode;
+    warm_skip_   = 0;  // a new mode is probed afresh
+    warm_pause_  = 0;
+    warm_probes_ = 0;
 }
 
 int Context::learned_samples(float* feat, float* target, int cap)
@@ -1303,6 +1367,7 @@ void Context::advance(int n, sgipc_advance_stats* st)
         loc.newton_iterations++;
         loc.pcg_iterations += is.pcg_iterations;
         loc.solver_fallbacks += is.solver_fallback;
+        loc.warm_starts += is.warm_start > 0 ? 1 : 0;
         loc.max_pcg_rel_residual = std::max(loc.max_pcg_rel_residual, is.pcg_rel_residual);
         if(newton_k_ >= cap)
             end_step();

@@ -63,7 +63,8 @@ class StepStats(C.Structure):
 class AdvanceStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_int32), ("time_steps_begun", C.c_int32),
                 ("pcg_iterations", C.c_int32), ("solver_fallbacks", C.c_int32),
-                ("max_pcg_rel_residual", C.c_double), ("ms_total", C.c_double)]
+                ("max_pcg_rel_residual", C.c_double), ("ms_total", C.c_double),
+                ("warm_starts", C.c_int32), ("pad", C.c_int32)]
 
 
 class SgipcError(RuntimeError):
@@ -82,7 +83,8 @@ EXPORTS = [
     "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
     "sgipc_energy", "sgipc_newton_iteration", "sgipc_time_step", "sgipc_advance",
     "sgipc_profile_enable", "sgipc_profile_query", "sgipc_problem_sizes",
-    "sgipc_set_state", "sgipc_set_pcg", "sgipc_learned_mode", "sgipc_learned_samples",
+    "sgipc_set_state", "sgipc_set_pcg", "sgipc_learned_mode", "sgipc_warm_start_adaptive",
+    "sgipc_learned_samples",
     "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
 ]
 MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
@@ -126,6 +128,7 @@ def lib():
             L.sgipc_problem_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
             fp = C.POINTER(C.c_float)
             L.sgipc_learned_mode.argtypes = [C.c_void_p, C.c_int32]
+            L.sgipc_warm_start_adaptive.argtypes = [C.c_void_p, C.c_int32]
             L.sgipc_learned_samples.argtypes = [C.c_void_p, fp, fp, C.c_int32,
                                                 C.POINTER(C.c_int32)]
             L.sgipc_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int32, C.c_int32, C.c_double,
@@ -340,6 +343,10 @@ class Simulation:
     # ---- learned coarse warm start
     def learned_mode(self, mode):
         check(lib().sgipc_learned_mode(self.h, int(mode)))
+
+    def warm_start_adaptive(self, on):
+        """Pause warm starts that cost more than they save (default on)."""
+        check(lib().sgipc_warm_start_adaptive(self.h, int(bool(on))))
 
     def learned_samples(self, cap=1 << 16):
         f = np.zeros((cap, MLP_IN), np.float32)

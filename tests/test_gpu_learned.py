@@ -108,7 +108,9 @@ def test_affine_guess_matches_numpy(oracle):
     assert rel(t, targs) < 1e-5
 
 
-def _run(mode, seed, steps, weights=None, mixed=False):
+def _run(mode, seed, steps, weights=None, mixed=False, adaptive=False):
+    """adaptive=False: every solve computes and judges the guesses, so the
+    iteration totals measure the guesses themselves"""
     import sgipc
     sc = _cubes(seed=seed, fall=0.02)
     if mixed:
@@ -117,11 +119,12 @@ def _run(mode, seed, steps, weights=None, mixed=False):
     if weights is not None:
         sim.mlp_weights(weights)
     sim.learned_mode(mode)
+    sim.warm_start_adaptive(adaptive)
     pcg = 0
     for k in range(steps):
         st = sim.time_step()
         pcg += st["pcg_iterations"]
-        print(f"[warm-run] mode={mode} step={k} {st}", flush=True)
+        print(f"[warm-run] mode={mode} adaptive={adaptive} step={k} {st}", flush=True)
     return sim, pcg
 
 
@@ -162,3 +165,10 @@ def test_warm_start_gain(oracle):
     assert p1 < 0.8 * p0
     assert p2 <= 1.1 * p1
     assert e1 < 1e-3 and e2 < 1e-3
+    # adaptive pausing changes only which solves start from a guess: the
+    # trajectory stays within the solver's tolerance of the plain one
+    s3, p3 = _run(2, 3, steps, weights=w, adaptive=True)
+    e3 = np.abs(s3.positions() - x0).max() / np.abs(x0).max()
+    print(f"[warm-adaptive] pcg={p3} dev={e3:.2e}", flush=True)
+    assert e3 < 1e-3
+    assert p3 <= 1.05 * p0
