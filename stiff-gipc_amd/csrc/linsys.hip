@@ -317,8 +317,10 @@ void Coo::build_rows(hipStream_t s)
 }
 
 // ------------------------------------------------------------------ SpMV
-// 8 lanes per block row; a row's blocks are contiguous, so the 8 lanes of a
-// group stream 576 contiguous bytes per step (288 in FP32).  The static
+// 4 lanes per block row (measured against 8 and 2: 12.95 / 13.25 / 15.3 us
+// on a 46,656-row system with 12.5 blocks per row, 48.4 / 51.2 / 54.7 us on
+// 175,273 rows with 14.5); a row's blocks are contiguous, so the 4 lanes of a
+// group stream 288 contiguous bytes per step (144 in FP32).  The static
 // pattern and the row-sorted contact blocks of the row are traversed by the
 // same lanes, so y is written once per row without atomics.  V is the
 // precision of the matrix, the vectors and the row sums.
@@ -330,7 +332,7 @@ void Coo::build_rows(hipStream_t s)
 //             or p_j = z_j on the first iteration; the row owner stores p_i,
 //             y = A p and p.y is accumulated into a slot set.  Every read of
 //             p_j evaluates the same fma, so all rows see identical values.
-constexpr int kRowLanes = 8;
+constexpr int kRowLanes = 4;
 enum
 {
     MODE_PLAIN = 0,
