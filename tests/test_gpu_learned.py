@@ -166,9 +166,11 @@ def test_warm_start_gain(oracle):
     assert p2 <= 1.1 * p1
     assert e1 < 1e-3 and e2 < 1e-3
     # adaptive pausing changes only which solves start from a guess: the
-    # trajectory stays within the solver's tolerance of the plain one
+    # trajectory stays within the solver's tolerance of the plain one, and
+    # on this scene, where a guess saves most of the iterations, the policy
+    # keeps using it
     s3, p3 = _run(2, 3, steps, weights=w, adaptive=True)
     e3 = np.abs(s3.positions() - x0).max() / np.abs(x0).max()
     print(f"[warm-adaptive] pcg={p3} dev={e3:.2e}", flush=True)
     assert e3 < 1e-3
-    assert p3 <= 1.05 * p0
+    assert p3 <= 1.2 * p2
