@@ -149,6 +149,22 @@ int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
  * barrier, ground, friction, cloth (membrane + bending), affine shape}. */
 int sgipc_energy(sgipc_t ctx, double* terms);
 
+/* Step-size bounds of a vertex direction dx [3 n_verts] at the current
+ * positions (x_new = x - alpha dx; solve_subIP, GIPC.cu:10786-10822):
+ * ground_largestFeasibleStepSize (GIPC.cu:8857-8915), ACCD
+ * self_largestFeasibleStepSize over the proximity pairs (:8821-8855,
+ * ACCD.cu:373-524), buildBVH_FULLCCD + buildFullCP candidates at that step
+ * (mlbvh.cu:2093-2179) and the CFL bound (:8917-8968).
+ * out[6] = {ground bound, self bound, full-CCD candidate count, CFL max
+ * speed (0 without candidates), alpha_cfl, alpha (the bound the line search
+ * starts from)}.  The contact set is the one of the current positions
+ * (sgipc_contact_query's). */
+int sgipc_ccd_bound(sgipc_t ctx, const double* dx, double* out);
+/* full-CCD candidate codes (signed int4, a1 encoding) for the swept boxes of
+ * x and x - alpha dx; *n = candidates found (may exceed cap) */
+int sgipc_ccd_candidates(sgipc_t ctx, const double* dx, double alpha, int32_t* pairs,
+                         int32_t cap, int32_t* n);
+
 typedef struct sgipc_iter_stats {
     int32_t pcg_iterations;
     int32_t n_pairs;          /* self contact records                  */
@@ -175,6 +191,10 @@ typedef struct sgipc_step_stats {
     int32_t newton_iterations;
     int32_t pcg_iterations;   /* summed over the step */
     double  ms_total;
+    int32_t capped;           /* 1: the step ended at the Newton iteration cap
+                                 (newton_max_iter; reference iterCap = 10000,
+                                 GIPC.cu:10739) without meeting the stop test */
+    int32_t pad;
 } sgipc_step_stats;
 
 /* One time step (IPC_Solver, GIPC.cu:10940-11131): kappa setup, Newton loop,
@@ -189,7 +209,14 @@ typedef struct sgipc_advance_stats {
     double  max_pcg_rel_residual;
     double  ms_total;
     int32_t warm_starts;       /* solves continued from a coarse guess      */
-    int32_t pad;
+    int32_t capped_steps;      /* time steps ended at the Newton cap without
+                                  meeting the stop test                      */
+    /* device time of the counted iterations by phase (HIP events): assembly
+     * + stop test, MAS build + solve, CCD bounds, line search (with the next
+     * contact detection) */
+    double  ms_assembly, ms_solve, ms_ccd, ms_line_search;
+    int32_t line_search_steps; /* Armijo backtracking steps, summed         */
+    int32_t ccd_limited;       /* iterations whose CCD/CFL bound was < 1     */
 } sgipc_advance_stats;
 
 /* Advance the integration by exactly n Newton iterations that solve (the

@@ -27,6 +27,12 @@ void orc_sym_eig(int n, const double* A, double* w, double* V);
 void orc_psd_project(int n, double* A);
 /* in-place inverse by Gauss-Jordan with partial pivoting; returns 0 if ok */
 int orc_inverse(int n, double* A);
+/* in-place Gauss-Jordan inverse without pivoting, in the operation order of
+ * the reference's MAS bank inverse (MASPreconditioner.cu:628-724) */
+void orc_inverse_gj(int n, double* M);
+/* host threads of the parallel loops; results do not depend on the count */
+void orc_set_threads(int n);
+int  orc_get_threads(void);
 
 /* ---------------- FEM (orc_fem.c) ---------------- */
 double orc_snk1_psi(const double F[9], double mu, double lam);
@@ -253,6 +259,21 @@ void orc_world_set_state(orc_world* w, const double* x, const double* v,
                          const double* xt, double kappa);
 /* exactly n solving Newton iterations across step boundaries */
 int  orc_world_advance(orc_world* w, int n, int* steps_out, int* pcg_out);
+/* step-size bounds of a vertex direction dx [3 nv] at the current positions
+ * (solve_subIP, GIPC.cu:10786-10822): out[6] = {ground bound, ACCD bound over
+ * the proximity pairs, full-CCD candidate count, CFL max speed (0 without
+ * candidates), alpha_cfl, resulting alpha} */
+void orc_world_ccd_bound(orc_world* w, const double* dx, double* out);
+/* full-CCD candidate codes for x and x - alpha dx (mlbvh.cu:2093-2179) */
+int  orc_world_ccd_candidates(orc_world* w, const double* dx, double alpha,
+                              int32_t* out, int cap);
+/* second derivative of the barrier in squared distance (GIPC.cu:9746-9750) */
+double orc_barrier_H_b(double d, double dhat);
+/* out[5] = {suggestKappa, upperBoundKappa's cap, mean mass, bbox diag^2,
+ * dHat (squared)} (GIPC.cu:9752-9786) */
+void   orc_world_kappa_info(orc_world* w, double* out);
+/* initKappa (GIPC.cu:9789-9831) from kappa = 0 on the current contact set */
+double orc_world_init_kappa(orc_world* w);
 /* additive CCD time of impact (ACCD.cu:373-524), eta = 1 - slackness */
 double orc_pt_ccd(const double* p, const double* t0, const double* t1,
                   const double* t2, const double* dp, const double* dt0,

@@ -111,6 +111,7 @@ void orc_bsr_free(orc_bsr* A)
 
 void orc_bsr_spmv(const orc_bsr* A, const double* x, double* y)
 {
+#pragma omp parallel for schedule(static)
     for(int i = 0; i < A->nb; i++)
     {
         double acc[3] = {0, 0, 0};
@@ -319,13 +320,23 @@ orc_mas* orc_mas_create(const orc_bsr* A, const int32_t* part_id, int fp32)
                         M[(3 * li + a) * 48 + 3 * lj + b] += blk[a * 3 + b];
             }
         }
+#pragma omp parallel for schedule(dynamic, 16)
     for(int bk = 0; bk < nbanks; bk++)
     {
         double* M = B + (size_t)bk * 48 * 48;
         for(int d = 0; d < 48; d++)
             if(M[d * 48 + d] == 0.0)
                 M[d * 48 + d] = 1.0;
-        orc_inverse(48, M);
+        orc_inverse_gj(48, M);
+        /* the diagonal 3x3 blocks take their upper entries below the diagonal
+         * too (MASPreconditioner.cu:704-708) */
+        for(int I = 0; I < BANK; I++)
+        {
+            double* D = M + (size_t)(3 * I) * 48 + 3 * I;
+            D[1 * 48 + 0] = D[0 * 48 + 1];
+            D[2 * 48 + 1] = D[1 * 48 + 2];
+            D[2 * 48 + 0] = D[0 * 48 + 2];
+        }
         if(fp32)
             for(int e = 0; e < 48 * 48; e++)
                 M[e] = (double)(float)M[e];
@@ -391,6 +402,7 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
     if(m->fp32)
         for(int i = 0; i < 3 * tc; i++)
             R[i] = (double)(float)R[i];
+#pragma omp parallel for schedule(static)
     for(int bk = 0; bk < tc / BANK; bk++)
     {
         const double* M = m->inv + (size_t)bk * 48 * 48;
@@ -402,6 +414,7 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
             Z[48 * bk + i] = s;
         }
     }
+#pragma omp parallel for schedule(static)
     for(int v = 0; v < nv; v++)
     {
         double acc[3] = {0, 0, 0};
@@ -488,6 +501,7 @@ int orc_pcg(const orc_bsr* A, const orc_mas* mas, int pc, const double* b, doubl
     {
         orc_bsr_spmv(A, p, Ap);
         double alpha = rz / dot(n, p, Ap);
+#pragma omp parallel for schedule(static)
         for(int i = 0; i < n; i++)
         {
             x[i] += alpha * p[i];
@@ -523,6 +537,7 @@ int orc_pcg(const orc_bsr* A, const orc_mas* mas, int pc, const double* b, doubl
             }
         }
         double beta = rzn / rz;
+#pragma omp parallel for schedule(static)
         for(int i = 0; i < n; i++)
             p[i] = z[i] + beta * p[i];
         rz = rzn;

@@ -165,3 +165,49 @@ int orc_inverse(int n, double* A)
     free(B);
     return ok;
 }
+
+/* In-place Gauss-Jordan inverse without pivoting in the reference's operation
+ * order (__inverse6_P96x96, MASPreconditioner.cu:628-724): for each pivot j,
+ * rt = M[j][j] and column j are saved, column j is replaced by e_j, row j is
+ * divided by rt, then every row k != j gets rate * row j added with
+ * rate = -column_j[k] (one rounded product and one rounded sum per entry). */
+void orc_inverse_gj(int n, double* M)
+{
+    double* colm = (double*)malloc(sizeof(double) * n);
+    for(int j = 0; j < n; j++)
+    {
+        double rt = M[j * n + j];
+        for(int i = 0; i < n; i++)
+            colm[i] = M[i * n + j];
+        for(int i = 0; i < n; i++)
+            M[i * n + j] = (i == j) ? 1.0 : 0.0;
+        for(int i = 0; i < n; i++)
+            M[j * n + i] /= rt;
+        for(int k = 0; k < n; k++)
+        {
+            if(k == j)
+                continue;
+            double rate = -colm[k];
+            for(int i = 0; i < n; i++)
+            {
+                double t = rate * M[j * n + i];
+                M[k * n + i] += t;
+            }
+        }
+    }
+    free(colm);
+}
+
+/* Host threads of the parallel loops.  Parallel loops only compute per-index
+ * results (element gradients/Hessians/energies, SpMV rows, bank inverses and
+ * products); every sum over indices is then taken sequentially in index
+ * order, so results are identical for any thread count. */
+#include <omp.h>
+void orc_set_threads(int n)
+{
+    omp_set_num_threads(n > 0 ? n : 1);
+}
+int orc_get_threads(void)
+{
+    return omp_get_max_threads();
+}

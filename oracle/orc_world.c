@@ -1041,6 +1041,9 @@ static double friction_eps2(const orc_world* w)
     return 1e-4 * w->bbox_diag2 * w->s.dt * w->s.dt;
 }
 
+/* elements per parallel chunk of the assembly and energy loops */
+enum { kChunk = 16384 };
+
 static void compute_grad_hess(orc_world* w, trip_t* T)
 {
     int nv = w->nv;
@@ -1107,54 +1110,80 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
             push_element(T, 1, ids, H, 1, w);
         }
     }
-    /* FEM */
-    for(int e = 0; e < w->nt; e++)
+    /* FEM, cloth membrane (6 blocks per triangle) and bending (10 per hinge):
+     * element gradients and Hessians of a chunk are computed in parallel,
+     * then accumulated and pushed in element order */
+    double* G  = (double*)malloc(sizeof(double) * 12 * kChunk);
+    double* HH = (double*)malloc(sizeof(double) * 144 * kChunk);
+    for(int e0 = 0; e0 < w->nt; e0 += kChunk)
     {
-        if(w->tet_abd[e])
-            continue;
-        const uint32_t* t = w->tets + 4 * e;
-        double          g[12], H[144];
-        orc_tet_grad_hess(w->x, t, w->dminv + 9 * e, w->vol[e], w->mu[e], w->lam[e], w->s.dt, g,
-                          T ? H : H);
-        for(int a = 0; a < 4; a++)
-            for(int c = 0; c < 3; c++)
-                w->gshape[3 * t[a] + c] += g[3 * a + c];
-        if(T)
+        int e1 = e0 + kChunk < w->nt ? e0 + kChunk : w->nt;
+#pragma omp parallel for schedule(static)
+        for(int e = e0; e < e1; e++)
+            if(!w->tet_abd[e])
+                orc_tet_grad_hess(w->x, w->tets + 4 * e, w->dminv + 9 * e, w->vol[e], w->mu[e],
+                                  w->lam[e], w->s.dt, G + 12 * (e - e0), HH + 144 * (e - e0));
+        for(int e = e0; e < e1; e++)
         {
-            int ids[4] = {(int)t[0], (int)t[1], (int)t[2], (int)t[3]};
-            push_element(T, 4, ids, H, 1, w);
+            if(w->tet_abd[e])
+                continue;
+            const uint32_t* t = w->tets + 4 * e;
+            const double*   g = G + 12 * (e - e0);
+            for(int a = 0; a < 4; a++)
+                for(int c = 0; c < 3; c++)
+                    w->gshape[3 * t[a] + c] += g[3 * a + c];
+            if(T)
+            {
+                int ids[4] = {(int)t[0], (int)t[1], (int)t[2], (int)t[3]};
+                push_element(T, 4, ids, HH + 144 * (e - e0), 1, w);
+            }
         }
     }
-    /* cloth membrane (6 blocks per triangle) and bending (10 per hinge) */
-    for(int f = 0; f < w->ntri; f++)
+    for(int f0 = 0; f0 < w->ntri; f0 += kChunk)
     {
-        const uint32_t* t = w->tris + 3 * f;
-        double          g[9], H[81];
-        orc_tri_grad_hess(w->x, t, w->tri_dminv + 4 * f, w->tri_area[f], w->c_stretch,
-                          w->c_shear, w->c_rate, w->s.dt, g, H);
-        for(int a = 0; a < 3; a++)
-            for(int c = 0; c < 3; c++)
-                w->gshape[3 * t[a] + c] += g[3 * a + c];
-        if(T)
+        int f1 = f0 + kChunk < w->ntri ? f0 + kChunk : w->ntri;
+#pragma omp parallel for schedule(static)
+        for(int f = f0; f < f1; f++)
+            orc_tri_grad_hess(w->x, w->tris + 3 * f, w->tri_dminv + 4 * f, w->tri_area[f],
+                              w->c_stretch, w->c_shear, w->c_rate, w->s.dt, G + 9 * (f - f0),
+                              HH + 81 * (f - f0));
+        for(int f = f0; f < f1; f++)
         {
-            int ids[3] = {(int)t[0], (int)t[1], (int)t[2]};
-            push_element(T, 3, ids, H, 1, w);
+            const uint32_t* t = w->tris + 3 * f;
+            const double*   g = G + 9 * (f - f0);
+            for(int a = 0; a < 3; a++)
+                for(int c = 0; c < 3; c++)
+                    w->gshape[3 * t[a] + c] += g[3 * a + c];
+            if(T)
+            {
+                int ids[3] = {(int)t[0], (int)t[1], (int)t[2]};
+                push_element(T, 3, ids, HH + 81 * (f - f0), 1, w);
+            }
         }
     }
-    for(int h = 0; h < w->nh; h++)
+    for(int h0 = 0; h0 < w->nh; h0 += kChunk)
     {
-        const uint32_t* q = w->hinges + 4 * h;
-        double          g[12], H[144];
-        orc_bend_grad_hess(w->x, q, w->h_theta0[h], w->h_len0[h], w->c_bend, w->s.dt, g, H);
-        for(int a = 0; a < 4; a++)
-            for(int c = 0; c < 3; c++)
-                w->gshape[3 * q[a] + c] += g[3 * a + c];
-        if(T)
+        int h1 = h0 + kChunk < w->nh ? h0 + kChunk : w->nh;
+#pragma omp parallel for schedule(static)
+        for(int h = h0; h < h1; h++)
+            orc_bend_grad_hess(w->x, w->hinges + 4 * h, w->h_theta0[h], w->h_len0[h], w->c_bend,
+                               w->s.dt, G + 12 * (h - h0), HH + 144 * (h - h0));
+        for(int h = h0; h < h1; h++)
         {
-            int ids[4] = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
-            push_element(T, 4, ids, H, 1, w);
+            const uint32_t* q = w->hinges + 4 * h;
+            const double*   g = G + 12 * (h - h0);
+            for(int a = 0; a < 4; a++)
+                for(int c = 0; c < 3; c++)
+                    w->gshape[3 * q[a] + c] += g[3 * a + c];
+            if(T)
+            {
+                int ids[4] = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+                push_element(T, 4, ids, HH + 144 * (h - h0), 1, w);
+            }
         }
     }
+    free(G);
+    free(HH);
     /* mass diagonal of the FEM vertices (never zeroed) */
     if(T)
         for(int v = 0; v < w->nfem; v++)
@@ -1304,10 +1333,21 @@ void orc_world_energy(orc_world* w, double* terms)
         }
         kin += d2 * w->mass[v] * 0.5;
     }
-    for(int e = 0; e < w->nt; e++)
-        if(!w->tet_abd[e])
-            ela += orc_tet_energy(w->x, w->tets + 4 * e, w->dminv + 9 * e, w->vol[e], w->mu[e],
-                              w->lam[e]);
+    /* element energies in parallel, summed in element order */
+    double* ebuf = (double*)malloc(sizeof(double) * kChunk);
+    for(int e0 = 0; e0 < w->nt; e0 += kChunk)
+    {
+        int e1 = e0 + kChunk < w->nt ? e0 + kChunk : w->nt;
+#pragma omp parallel for schedule(static)
+        for(int e = e0; e < e1; e++)
+            ebuf[e - e0] = w->tet_abd[e] ? 0.0
+                                         : orc_tet_energy(w->x, w->tets + 4 * e, w->dminv + 9 * e,
+                                                          w->vol[e], w->mu[e], w->lam[e]);
+        for(int e = e0; e < e1; e++)
+            if(!w->tet_abd[e])
+                ela += ebuf[e - e0];
+    }
+    free(ebuf);
     ela *= w->s.dt * w->s.dt;
     for(int i = 0; i < w->npairs; i++)
         bar += orc_barrier_energy(w->x, w->xr, w->pairs + 4 * i, w->dhat2, w->kappa);
@@ -1554,6 +1594,85 @@ static int proximity_candidates(orc_world* w, int32_t* out, int cap)
     return m;
 }
 
+/* Step-size bounds of the direction w->dx (solve_subIP, GIPC.cu:10786-10822):
+ * ground bound (slackness 0.8), ACCD bound over the proximity pairs, full-CCD
+ * candidates at that step, and when there are any the CFL bound
+ * sqrt(dHat) / max|dx| * 0.5, re-bounded by ACCD over the candidates when the
+ * CCD step exceeds twice the CFL step.  out = {ground, self, candidates,
+ * max speed (0 without candidates), alpha_cfl, alpha}. */
+static void step_bounds(orc_world* w, double out[6])
+{
+    double alpha = 1.0, ag = 1.0;
+    if(w->s.use_ground)
+        ag = ground_bound(w, 0.8);
+    alpha         = dmin(alpha, ag);
+    int32_t* cand = w->ccd;
+    int      np   = proximity_candidates(w, cand, w->cap);
+    double   as   = self_bound(w, 0.8, cand, np);
+    alpha         = dmin(alpha, as);
+    double temp_alpha = alpha, alpha_cfl = alpha, maxSpeed = 0.0;
+    int    nccd = full_ccd_candidates(w, temp_alpha, cand, w->cap);
+    if(nccd > w->cap)
+        nccd = w->cap;
+    w->nccd = nccd;
+    if(nccd > 0)
+    {
+        maxSpeed  = cfl_speed(w);
+        alpha_cfl = sqrt(w->dhat2) / maxSpeed * 0.5;
+        alpha     = dmin(alpha, alpha_cfl);
+        if(temp_alpha > 2 * alpha_cfl)
+        {
+            alpha = dmin(temp_alpha, self_bound(w, 0.8, cand, nccd));
+            alpha = dmax(alpha, alpha_cfl);
+        }
+    }
+    out[0] = ag;
+    out[1] = as;
+    out[2] = nccd;
+    out[3] = maxSpeed;
+    out[4] = alpha_cfl;
+    out[5] = alpha;
+}
+
+void orc_world_ccd_bound(orc_world* w, const double* dx, double* out)
+{
+    memcpy(w->dx, dx, sizeof(double) * 3 * w->nv);
+    step_bounds(w, out);
+}
+
+int orc_world_ccd_candidates(orc_world* w, const double* dx, double alpha, int32_t* out, int cap)
+{
+    memcpy(w->dx, dx, sizeof(double) * 3 * w->nv);
+    int n = full_ccd_candidates(w, alpha, w->ccd, w->cap);
+    if(n > w->cap)
+        n = w->cap;
+    if(out)
+        memcpy(out, w->ccd, sizeof(int32_t) * 4 * (n < cap ? n : cap));
+    return n;
+}
+
+double orc_barrier_H_b(double d, double dhat) { return compute_H_b(d, dhat); }
+
+void orc_world_kappa_info(orc_world* w, double* out)
+{
+    double k = 0.0;
+    suggest_kappa(w, &k);
+    out[0] = k;
+    k      = 1e300;
+    upper_bound_kappa(w, &k);
+    out[1] = k;
+    out[2] = w->mean_mass;
+    out[3] = w->bbox_diag2;
+    out[4] = w->dhat2;
+}
+
+double orc_world_init_kappa(orc_world* w)
+{
+    w->kappa = 0.0;
+    init_kappa(w);
+    return w->kappa;
+}
+
 int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_out,
                                double* xdelta_out, int* converged_out)
 {
@@ -1606,27 +1725,9 @@ int orc_world_newton_iteration(orc_world* w, double* alpha_out, int* pcg_iters_o
     orc_bsr_free(&H);
     free(grad);
 
-    double alpha = 1.0;
-    if(w->s.use_ground)
-        alpha = dmin(alpha, ground_bound(w, 0.8));
-    int32_t* cand = w->ccd;
-    int      np   = proximity_candidates(w, cand, w->cap);
-    alpha         = dmin(alpha, self_bound(w, 0.8, cand, np));
-    double temp_alpha = alpha, alpha_cfl = alpha;
-    int    nccd = full_ccd_candidates(w, temp_alpha, cand, w->cap);
-    if(nccd > w->cap)
-        nccd = w->cap;
-    if(nccd > 0)
-    {
-        double maxSpeed = cfl_speed(w);
-        alpha_cfl       = sqrt(w->dhat2) / maxSpeed * 0.5;
-        alpha           = dmin(alpha, alpha_cfl);
-        if(temp_alpha > 2 * alpha_cfl)
-        {
-            alpha = dmin(temp_alpha, self_bound(w, 0.8, cand, nccd));
-            alpha = dmax(alpha, alpha_cfl);
-        }
-    }
+    double b[6];
+    step_bounds(w, b);
+    double alpha = b[5], alpha_cfl = b[4];
     line_search(w, &alpha, alpha_cfl, NULL);
     if(w->kappa == 0.0)
         init_kappa(w);

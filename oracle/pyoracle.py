@@ -154,8 +154,42 @@ def lib():
         L.orc_pt_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
         L.orc_ee_ccd.restype = C.c_double
         L.orc_ee_ccd.argtypes = [dp] * 8 + [C.c_double, C.c_double]
+        L.orc_inverse_gj.argtypes = [C.c_int, dp]
+        L.orc_world_ccd_bound.argtypes = [C.c_void_p, dp, dp]
+        L.orc_world_ccd_candidates.restype = C.c_int
+        L.orc_world_ccd_candidates.argtypes = [C.c_void_p, dp, C.c_double, ip, C.c_int]
+        L.orc_barrier_H_b.restype = d
+        L.orc_barrier_H_b.argtypes = [d, d]
+        L.orc_world_kappa_info.argtypes = [C.c_void_p, dp]
+        L.orc_world_init_kappa.restype = d
+        L.orc_world_init_kappa.argtypes = [C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Host threads of the oracle's parallel loops (1: sequential).  Every
+    parallel sum uses fixed chunks combined in index order, so results do not
+    depend on the thread count."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def inverse_gj(M):
+    """In-place Gauss-Jordan inverse without pivoting in the reference's
+    operation order (MASPreconditioner.cu:628-724); returns a new array."""
+    A = np.ascontiguousarray(M, np.float64).copy()
+    lib().orc_inverse_gj(A.shape[0], P(A))
+    return A
+
+
+def barrier_H_b(d, dhat):
+    return lib().orc_barrier_H_b(float(d), float(dhat))
 
 
 def P(a, t=C.c_double):
@@ -373,6 +407,30 @@ class World:
         it = C.c_int32()
         done = lib().orc_world_advance(self.h, int(n), C.byref(st), C.byref(it))
         return dict(newton_iterations=done, time_steps_begun=st.value, pcg_iterations=it.value)
+
+    def ccd_bound(self, dx):
+        """step bounds of the vertex direction dx (x_new = x - alpha dx)"""
+        out = np.zeros(6)
+        lib().orc_world_ccd_bound(self.h, P(np.ascontiguousarray(dx, np.float64).ravel()),
+                                  P(out))
+        return dict(alpha_ground=out[0], alpha_self=out[1], n_candidates=int(out[2]),
+                    max_speed=out[3], alpha_cfl=out[4], alpha=out[5])
+
+    def ccd_candidates(self, dx, alpha, cap=1 << 20):
+        out = np.zeros((cap, 4), np.int32)
+        n = lib().orc_world_ccd_candidates(
+            self.h, P(np.ascontiguousarray(dx, np.float64).ravel()), float(alpha),
+            P(out, C.c_int32), cap)
+        return out[:min(n, cap)].copy()
+
+    def kappa_info(self):
+        out = np.zeros(5)
+        lib().orc_world_kappa_info(self.h, P(out))
+        return dict(suggest=out[0], upper=out[1], mean_mass=out[2], bbox_diag2=out[3],
+                    dhat2=out[4])
+
+    def init_kappa(self):
+        return lib().orc_world_init_kappa(self.h)
 
     def time_step(self):
         n = C.c_int32()

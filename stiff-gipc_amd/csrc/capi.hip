@@ -104,6 +104,7 @@ int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr, const int32
             {
                 MasConfig mc;
                 mc.fp32 = (mas_precision == SGIPC_MAS_FP32);
+                mc.blocked_invert = mas_blocked_invert();
                 ls->mas.init(ls->A, part_id, mc, ls->s);
                 ls->mas.build(SysView{&ls->A, nullptr}, ls->s);
             }
@@ -361,6 +362,24 @@ int sgipc_energy(sgipc_t h, double* terms)
     return guard([&] {
         SG_CHECK(terms, SGIPC_EINVAL, "null output");
         ctx_of(h).energy_terms(terms);
+    });
+}
+
+int sgipc_ccd_bound(sgipc_t h, const double* dx, double* out)
+{
+    return guard([&] {
+        SG_CHECK(dx && out, SGIPC_EINVAL, "null argument");
+        ctx_of(h).ccd_bound(dx, out);
+    });
+}
+
+int sgipc_ccd_candidates(sgipc_t h, const double* dx, double alpha, int32_t* pairs, int32_t cap,
+                         int32_t* n)
+{
+    return guard([&] {
+        SG_CHECK(dx && n && cap >= 0 && (pairs || cap == 0), SGIPC_EINVAL, "null argument");
+        SG_CHECK(alpha >= 0.0 && alpha <= 1.0, SGIPC_EINVAL, "alpha must lie in [0, 1]");
+        *n = ctx_of(h).ccd_candidates(dx, alpha, pairs, cap);
     });
 }
 

@@ -34,6 +34,10 @@ class Context
     void assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols, double* vals,
                       int cap);
     void energy_terms(double* terms);
+    // step-size bounds of a vertex direction dx (x_new = x - alpha dx) and the
+    // full-CCD candidate set at a step (sgipc_ccd_bound / sgipc_ccd_candidates)
+    void ccd_bound(const double* dx, double out[6]);
+    int  ccd_candidates(const double* dx, double alpha, int32_t* out, int cap);
     void newton_iteration(sgipc_iter_stats* st);
     void time_step(sgipc_step_stats* st);
     void advance(int n, sgipc_advance_stats* st);
@@ -68,6 +72,7 @@ class Context
     double self_bound(const int4* cand, const unsigned* count, int cap, double slack);
     int    full_ccd(double alpha);         // buildBVH_FULLCCD + buildFullCP
     double cfl_speed();
+    void   step_bounds(double out[6]);
     void   line_search(double& alpha, double cfl_alpha, int* nls, double* e0, double* e1);
     void   suggest_kappa(double& k);
     void   upper_bound_kappa(double& k);

@@ -672,6 +672,7 @@ Context::Context(const sgipc_scene_desc& d)
     {
         MasConfig mc;
         mc.fp32 = d.mas_precision == SGIPC_MAS_FP32;
+        mc.blocked_invert = mas_blocked_invert();
         // MAS parts per node: FEM vertices keep theirs, each body is one part
         std::vector<int> npart(nn_);
         int              maxp = -1;
@@ -1074,6 +1075,61 @@ void Context::init_kappa()
     upper_bound_kappa(kappa_);
 }
 
+// Step-size bounds of the direction dx_ (solve_subIP, GIPC.cu:10786-10822):
+// ground bound (slackness 0.8), ACCD over the proximity pairs, full-CCD
+// candidates at that step and, when there are any, the CFL bound
+// sqrt(dHat) / max|dx| * 0.5, re-bounded by ACCD over the candidates when the
+// CCD step exceeds twice the CFL step.  out = {ground, self, candidates, max
+// speed (0 without candidates), alpha_cfl, alpha}.
+void Context::step_bounds(double out[6])
+{
+    double alpha = 1.0, ag = 1.0;
+    if(desc_.use_ground)
+        ag = ground_bound(0.8);
+    alpha     = std::min(alpha, ag);
+    double as = self_bound(pairs_raw_.data(), pcount_.data(), pair_cap_, 0.8);
+    alpha     = std::min(alpha, as);
+    double temp_alpha = alpha, alpha_cfl = alpha, maxSpeed = 0.0;
+    int    nccd       = full_ccd(temp_alpha);
+    if(nccd > 0)
+    {
+        maxSpeed  = cfl_speed();
+        alpha_cfl = std::sqrt(dHat_) / maxSpeed * 0.5;
+        alpha     = std::min(alpha, alpha_cfl);
+        if(temp_alpha > 2 * alpha_cfl)
+        {
+            alpha = std::min(temp_alpha, self_bound(cand_.data(), ccount_.data(), cand_cap_, 0.8));
+            alpha = std::max(alpha, alpha_cfl);
+        }
+    }
+    out[0] = ag;
+    out[1] = as;
+    out[2] = nccd;
+    out[3] = maxSpeed;
+    out[4] = alpha_cfl;
+    out[5] = alpha;
+}
+
+void Context::ccd_bound(const double* dx, double out[6])
+{
+    dx_.upload(dx, 3 * (size_t)nv_, s_);
+    step_bounds(out);
+}
+
+int Context::ccd_candidates(const double* dx, double alpha, int32_t* out, int cap)
+{
+    dx_.upload(dx, 3 * (size_t)nv_, s_);
+    int n = full_ccd(alpha);
+    int m = std::min(n, std::min(cap, cand_cap_));
+    if(out && m > 0)
+    {
+        SG_HIP(hipMemcpyAsync(out, cand_.data(), sizeof(int4) * (size_t)m, hipMemcpyDeviceToHost,
+                              s_));
+        SG_HIP(hipStreamSynchronize(s_));
+    }
+    return n;
+}
+
 // ------------------------------------------------------------- iteration
 void Context::newton_iteration(sgipc_iter_stats* st)
 {
@@ -1127,23 +1183,9 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     loc.solver_fallback  = ps.outer_iterations > 1 ? 1 : 0;
     SG_HIP(hipEventRecord(ev[2], s_));
     // ---- step bounds
-    double alpha = 1.0;
-    if(desc_.use_ground)
-        alpha = std::min(alpha, ground_bound(0.8));
-    alpha = std::min(alpha, self_bound(pairs_raw_.data(), pcount_.data(), pair_cap_, 0.8));
-    double temp_alpha = alpha, alpha_cfl = alpha;
-    int    nccd       = full_ccd(temp_alpha);
-    if(nccd > 0)
-    {
-        double maxSpeed = cfl_speed();
-        alpha_cfl       = std::sqrt(dHat_) / maxSpeed * 0.5;
-        alpha           = std::min(alpha, alpha_cfl);
-        if(temp_alpha > 2 * alpha_cfl)
-        {
-            alpha = std::min(temp_alpha, self_bound(cand_.data(), ccount_.data(), cand_cap_, 0.8));
-            alpha = std::max(alpha, alpha_cfl);
-        }
-    }
+    double bnd[6];
+    step_bounds(bnd);
+    double alpha = bnd[5], alpha_cfl = bnd[4];
     loc.alpha_ccd = alpha;
     SG_HIP(hipEventRecord(ev[3], s_));
     // ---- line search
@@ -1369,8 +1411,17 @@ void Context::advance(int n, sgipc_advance_stats* st)
         loc.solver_fallbacks += is.solver_fallback;
         loc.warm_starts += is.warm_start > 0 ? 1 : 0;
         loc.max_pcg_rel_residual = std::max(loc.max_pcg_rel_residual, is.pcg_rel_residual);
+        loc.ms_assembly += is.ms_assembly;
+        loc.ms_solve += is.ms_solve;
+        loc.ms_ccd += is.ms_ccd;
+        loc.ms_line_search += is.ms_line_search;
+        loc.line_search_steps += is.line_search_steps;
+        loc.ccd_limited += is.alpha_ccd < 1.0 ? 1 : 0;
         if(newton_k_ >= cap)
+        {
+            loc.capped_steps++;
             end_step();
+        }
     }
     SG_HIP(hipEventRecord(e1, s_));
     SG_HIP(hipStreamSynchronize(s_));
@@ -1419,13 +1470,17 @@ void Context::time_step(sgipc_step_stats* st)
     SG_HIP(hipEventRecord(e0, s_));
     begin_step();
     int cap = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
-    int k = 0, pcg_total = 0;
+    int  k = 0, pcg_total = 0;
+    bool met = false;
     for(k = 0; k < cap; k++)
     {
         sgipc_iter_stats is{};
         newton_iteration(&is);
         if(is.converged)
+        {
+            met = true;
             break;
+        }
         pcg_total += is.pcg_iterations;
     }
     end_step();
@@ -1436,6 +1491,7 @@ void Context::time_step(sgipc_step_stats* st)
         st->newton_iterations = k;
         st->pcg_iterations    = pcg_total;
         st->ms_total          = now_ms(e0, e1);
+        st->capped            = met ? 0 : 1;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

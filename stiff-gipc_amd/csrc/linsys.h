@@ -14,6 +14,9 @@
 //   MASPreconditioner (MASPreconditioner.cu) -> Mas below.
 #pragma once
 
+#include <cstdlib>
+#include <cstring>
+
 #include "common.h"
 
 namespace sgipc
@@ -109,7 +112,22 @@ void refresh_fp32(const SysView& S, hipStream_t s);
 struct MasConfig
 {
     bool fp32 = true;  // FP32 inverses and multilevel R/Z (reference default)
+    // bank inverses: false = scalar Gauss-Jordan in the reference's order
+    // (mas_invert.hip, bit-identical to a sequential restatement); true =
+    // eliminated by 16x16 tiles with FP64 MFMA tile products (the same inverse
+    // in exact arithmetic, rounding accumulated in another order)
+    bool blocked_invert = false;
 };
+
+// SGIPC_MAS_INVERT=blocked selects the tiled MFMA bank inverse
+inline bool mas_blocked_invert()
+{
+    const char* e = getenv("SGIPC_MAS_INVERT");
+    return e && strcmp(e, "blocked") == 0;
+}
+
+void launch_mas_invert_gj(const double* B, float* inv, int nbanks, hipStream_t s);
+void launch_mas_invert_gj(const double* B, double* inv, int nbanks, hipStream_t s);
 
 class Mas
 {

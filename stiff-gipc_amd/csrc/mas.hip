@@ -1070,7 +1070,15 @@ void Mas::invert(hipStream_t s)
     unsigned nbanks = total_clusters_ / BANK;
     if(prof)
         prof->begin("mas_invert", s);
-    if(cfg_.fp32)
+    if(!cfg_.blocked_invert)
+    {
+        // scalar Gauss-Jordan in the reference's order (mas_invert.hip)
+        if(cfg_.fp32)
+            launch_mas_invert_gj(B_.data(), inv32_.data(), (int)nbanks, s);
+        else
+            launch_mas_invert_gj(B_.data(), inv64_.data(), (int)nbanks, s);
+    }
+    else if(cfg_.fp32)
         hipLaunchKernelGGL(k_mas_invert<float>, dim3(nbanks), dim3(64), 0, s, B_.data(),
                            inv32_.data());
     else

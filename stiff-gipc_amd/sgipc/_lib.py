@@ -57,14 +57,17 @@ class IterStats(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_int32), ("pcg_iterations", C.c_int32),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("capped", C.c_int32), ("pad", C.c_int32)]
 
 
 class AdvanceStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_int32), ("time_steps_begun", C.c_int32),
                 ("pcg_iterations", C.c_int32), ("solver_fallbacks", C.c_int32),
                 ("max_pcg_rel_residual", C.c_double), ("ms_total", C.c_double),
-                ("warm_starts", C.c_int32), ("pad", C.c_int32)]
+                ("warm_starts", C.c_int32), ("capped_steps", C.c_int32),
+                ("ms_assembly", C.c_double), ("ms_solve", C.c_double), ("ms_ccd", C.c_double),
+                ("ms_line_search", C.c_double), ("line_search_steps", C.c_int32),
+                ("ccd_limited", C.c_int32)]
 
 
 class SgipcError(RuntimeError):
@@ -86,6 +89,7 @@ EXPORTS = [
     "sgipc_set_state", "sgipc_set_pcg", "sgipc_learned_mode", "sgipc_warm_start_adaptive",
     "sgipc_learned_samples",
     "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
+    "sgipc_ccd_bound", "sgipc_ccd_candidates",
 ]
 MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
 
@@ -121,6 +125,8 @@ def lib():
             L.sgipc_assemble.argtypes = [C.c_void_p, dp, ip, ip, ip, dp, C.c_int32]
             L.sgipc_energy.argtypes = [C.c_void_p, dp]
             L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
+            L.sgipc_ccd_bound.argtypes = [C.c_void_p, dp, dp]
+            L.sgipc_ccd_candidates.argtypes = [C.c_void_p, dp, C.c_double, ip, C.c_int32, ip]
             L.sgipc_time_step.argtypes = [C.c_void_p, C.POINTER(StepStats)]
             L.sgipc_advance.argtypes = [C.c_void_p, C.c_int32, C.POINTER(AdvanceStats)]
             L.sgipc_profile_enable.argtypes = [C.c_void_p, C.c_int32]
@@ -309,11 +315,27 @@ class Simulation:
         check(lib().sgipc_newton_iteration(self.h, C.byref(st)))
         return st.as_dict()
 
+    def ccd_bound(self, dx):
+        """step bounds of the vertex direction dx (x_new = x - alpha dx)"""
+        out = np.zeros(6)
+        d = np.ascontiguousarray(dx, np.float64).ravel()
+        check(lib().sgipc_ccd_bound(self.h, d.ctypes.data_as(dp), out.ctypes.data_as(dp)))
+        return dict(alpha_ground=out[0], alpha_self=out[1], n_candidates=int(out[2]),
+                    max_speed=out[3], alpha_cfl=out[4], alpha=out[5])
+
+    def ccd_candidates(self, dx, alpha, cap=1 << 20):
+        out = np.zeros((cap, 4), np.int32)
+        n = C.c_int32()
+        d = np.ascontiguousarray(dx, np.float64).ravel()
+        check(lib().sgipc_ccd_candidates(self.h, d.ctypes.data_as(dp), float(alpha),
+                                         out.ctypes.data_as(ip), cap, C.byref(n)))
+        return out[:min(n.value, cap)].copy()
+
     def time_step(self):
         st = StepStats()
         check(lib().sgipc_time_step(self.h, C.byref(st)))
         return dict(newton_iterations=st.newton_iterations, pcg_iterations=st.pcg_iterations,
-                    ms_total=st.ms_total)
+                    ms_total=st.ms_total, capped=st.capped)
 
     def advance(self, n):
         st = AdvanceStats()

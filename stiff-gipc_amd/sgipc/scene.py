@@ -118,7 +118,7 @@ class Scene:
     armijo_c1: float = 1e-4
     armijo_beta: float = 0.5
     armijo_alpha_min: float = 1e-12
-    newton_max_iter: int = 50
+    newton_max_iter: int = 10000     # reference iterCap (GIPC.cu:10739)
     pcg_rel_tol: float = 1e-4
     pcg_abs_tol: float = 0.0
     pcg_max_iter: int = 0

@@ -64,3 +64,50 @@ def deep_hierarchy_system(nblob=4096, seed=0):
     A = O.bsr_from_upper_triplets(nv, rows, cols, vals)
     part = (np.arange(nv) // 16).astype(np.int32)
     return A, part, rng.normal(size=3 * nv)
+
+
+def bring_into_contact(scene, a, b, method="bvh", gap_frac=0.5):
+    """Positions with body b translated towards body a along the line through
+    their centroids until the bodies are within dhat of each other: bisection
+    (on the oracle's contact pair count, which must exceed its value before
+    the motion) for the first translation t* with a new pair closer than dhat,
+    then gap_frac * dhat further, so the separation lies
+    in [(1 - gap_frac) dhat, dhat) (a translation by s moves every point by s)."""
+    import dataclasses
+    import pyoracle as O
+    x = scene.x.copy()
+    ia, ib = scene.body_id == a, scene.body_id == b
+    # the contact search sees only the primitives of the two bodies
+    keep = ia | ib
+    sub = dataclasses.replace(
+        scene, faces=scene.faces[keep[scene.faces].all(1)],
+        edges=scene.edges[keep[scene.edges].all(1)],
+        surf_verts=scene.surf_verts[keep[scene.surf_verts]])
+    d = x[ia].mean(0) - x[ib].mean(0)
+    length = float(np.linalg.norm(d))
+    u = d / length
+
+    def npairs(t):
+        y = x.copy()
+        y[ib] += t * u
+        _, c = O.contact_pairs(sub, y, method)
+        return int(c[0])
+
+    base = npairs(0.0)  # pairs that do not involve the motion of b
+    lo, hi = 0.0, length
+    assert npairs(hi) > base
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if npairs(mid) > base:
+            hi = mid
+        else:
+            lo = mid
+    x[ib] += (hi + gap_frac * scene.dhat) * u
+    return x
+
+
+def lower_onto_ground(x, dhat, frac=0.5):
+    """Translate every vertex down so the lowest sits frac * dhat above y = 0."""
+    y = x.copy()
+    y[:, 1] -= y[:, 1].min() - frac * dhat
+    return y

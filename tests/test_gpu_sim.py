@@ -40,16 +40,22 @@ def test_lbvh_bit_exact(oracle, name, which):
 @pytest.mark.parametrize("name", ["stack", "cubes8"])
 def test_contact_multiset(oracle, name):
     import sgipc
+    from helpers import bring_into_contact, lower_onto_ground
     sc = scenes()[name]()
     x = sc.x.copy()
-    if name == "cubes8":  # bring two cubes within dhat of each other
-        x = x.copy()
+    if name == "cubes8":
+        # cube 1 translated to within dhat of cube 0, the scene lowered so the
+        # lowest vertex is dhat / 2 above the ground
+        sc.x = bring_into_contact(sc, 0, 1)
+        x = lower_onto_ground(sc.x, sc.dhat)
     sim = sgipc.Simulation(sc)
+    sim.set_positions(x)
     pairs, counts, ground = sim.contacts()
     po, co = oracle.contact_pairs(sc, x, "bvh")
     pb, cb = oracle.contact_pairs(sc, x, "brute")
     print(f"[contacts] {name} gpu={counts.tolist()} oracle={co.tolist()} ground={ground.size}",
           flush=True)
+    assert counts[0] > 0 and ground.size > 0
     assert _pairs_key(pairs) == _pairs_key(po) == _pairs_key(pb)
     np.testing.assert_array_equal(counts, co)
     d = (x[:, 1] - 0.0) ** 2
