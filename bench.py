@@ -298,7 +298,9 @@ def main():
     cpu = None
     if not args.no_cpu and args.cpu_iters > 0:
         try:
-            threads = args.cpu_threads or (os.cpu_count() or 1)
+            # the host cores this process may use (OMP_NUM_THREADS when set)
+            threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) \
+                or (os.cpu_count() or 1)
             cms, used = cpu_baseline(sc, state, args.cpu_iters, threads)
             cpu = {"value": round(cms, 3), "unit": "ms/Newton step", "cores": used,
                    "kind": "port",
