@@ -93,6 +93,13 @@ int sgipc_ls_mas_info(sgipc_ls_t ls, int32_t* levels, int32_t* level_sizes,
 /* coarse node chain per vertex: chain[v*(levels-1) + l-1] = global id of
  * vertex v's node at level l (reference itable, _aggregationKernel:592) */
 int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain);
+/* one bank's assembled 48x48 matrix mat (FP64, row-major, the Galerkin
+ * product of its level restricted to the bank: the reference's MasMatrixSymT
+ * unpacked, PrepareHessian_bcoo MASPreconditioner.cu:1810-2084) and the
+ * inverse the apply uses, inv (__inverse6_P96x96 :628-724; widened to FP64
+ * when the MAS is FP32).  Banks are numbered level by level (level 0 first)
+ * as the clusters are (total_clusters / 16 banks); either output may be NULL */
+int sgipc_ls_mas_bank(sgipc_ls_t ls, int32_t bank, double* mat, double* inv);
 
 /* ------------------------------------------------------------------------ */
 /* Simulation context (GIPC counterpart)                                     */

@@ -229,6 +229,14 @@ int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain)
     });
 }
 
+int sgipc_ls_mas_bank(sgipc_ls_t ls, int32_t bank, double* mat, double* inv)
+{
+    return guard([&] {
+        SG_CHECK(ls && ls->precond == SGIPC_PRECOND_MAS && ls->mas.ready(), SGIPC_ESTATE,
+                 "no MAS hierarchy on this system");
+        ls->mas.bank_host(bank, mat, inv, ls->s);
+    });
+}
 
 // ------------------------------------------------------------------ context
 struct sgipc_context

@@ -162,6 +162,9 @@ class Mas
     const int*    chain_dev() const { return chain_.data(); }  // nv x (L - 1) global ids
     const double* bank_mats() const { return B_.data(); }      // nbanks x 48 x 48 Galerkin
     int  banks() const { return total_clusters_ / 16; }
+    // assembled 48 x 48 matrix and inverse of one bank (any level), FP64 on
+    // the host (an FP32 inverse is widened)
+    void bank_host(int bank, double* mat, double* inv, hipStream_t s) const;
     Profiler* prof = nullptr;
 
   private:

@@ -24,6 +24,7 @@
 // inside a single bank of level l, so restriction is a bank-local LDS
 // reduction done by the same wave that applies that bank's inverse.
 #include "linsys.h"
+#include "sgipc.h"
 
 namespace sgipc
 {
@@ -1063,6 +1064,29 @@ void Mas::assemble(const SysView& S, hipStream_t s)
                        coo ? coo->v.data() : nullptr, L_, nv_,
                        slot_of_.data(), chain_.data(), B_.data());
     SG_LAUNCH_CHECK();
+}
+
+void Mas::bank_host(int bank, double* mat, double* inv, hipStream_t s) const
+{
+    SG_CHECK(built_ && bank >= 0 && bank < total_clusters_ / BANK, SGIPC_EINVAL,
+             "bank index out of range");
+    const size_t off = (size_t)bank * 48 * 48;
+    if(mat)
+        SG_HIP(hipMemcpyAsync(mat, B_.data() + off, 48 * 48 * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    if(inv && !cfg_.fp32)
+        SG_HIP(hipMemcpyAsync(inv, inv64_.data() + off, 48 * 48 * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    std::vector<float> f;
+    if(inv && cfg_.fp32)
+    {
+        f.resize(48 * 48);
+        SG_HIP(hipMemcpyAsync(f.data(), inv32_.data() + off, 48 * 48 * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+    }
+    SG_HIP(hipStreamSynchronize(s));
+    for(size_t i = 0; i < f.size(); i++)
+        inv[i] = f[i];
 }
 
 void Mas::invert(hipStream_t s)

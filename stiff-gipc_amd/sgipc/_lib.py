@@ -80,7 +80,7 @@ _lib = None
 EXPORTS = [
     "sgipc_last_error", "sgipc_device_info",
     "sgipc_ls_create", "sgipc_ls_destroy", "sgipc_ls_spmv", "sgipc_ls_precond_apply",
-    "sgipc_ls_pcg", "sgipc_ls_mas_info", "sgipc_ls_mas_chain",
+    "sgipc_ls_pcg", "sgipc_ls_mas_info", "sgipc_ls_mas_chain", "sgipc_ls_mas_bank",
     "sgipc_create", "sgipc_destroy", "sgipc_set_positions", "sgipc_get_positions",
     "sgipc_get_velocities", "sgipc_set_xtilde", "sgipc_get_xtilde", "sgipc_get_kappa",
     "sgipc_set_kappa", "sgipc_lbvh_build", "sgipc_contact_query", "sgipc_assemble",
@@ -110,6 +110,7 @@ def lib():
         L.sgipc_ls_pcg.argtypes = [C.c_void_p, dp, dp, C.POINTER(PcgCfg), C.POINTER(PcgStats)]
         L.sgipc_ls_mas_info.argtypes = [C.c_void_p, ip, ip, ip]
         L.sgipc_ls_mas_chain.argtypes = [C.c_void_p, ip]
+        L.sgipc_ls_mas_bank.argtypes = [C.c_void_p, C.c_int32, dp, dp]
         if hasattr(L, "sgipc_create"):
             L.sgipc_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
             L.sgipc_destroy.argtypes = [C.c_void_p]
@@ -218,6 +219,13 @@ class LinearSystem:
         c = np.zeros(self.nb * (L - 1), np.int32)
         check(lib().sgipc_ls_mas_chain(self.h, P(c, C.c_int32)))
         return c.reshape(self.nb, L - 1)
+
+    def mas_bank(self, bank):
+        """(48x48 assembled bank matrix, the inverse the apply uses), FP64"""
+        m = np.zeros((48, 48))
+        inv = np.zeros((48, 48))
+        check(lib().sgipc_ls_mas_bank(self.h, int(bank), P(m), P(inv)))
+        return m, inv
 
 
 class Simulation:
