@@ -237,6 +237,7 @@ typedef struct orc_world orc_world;
 orc_world* orc_world_create(const sgipc_scene_desc* s);
 void       orc_world_free(orc_world* w);
 void       orc_world_set_positions(orc_world* w, const double* x);
+void       orc_world_set_positions_exact(orc_world* w, const double* x);
 void       orc_world_get_positions(const orc_world* w, double* x);
 void       orc_world_set_xtilde(orc_world* w, const double* xt);
 void       orc_world_get_xtilde(const orc_world* w, double* xt);

@@ -963,6 +963,16 @@ void orc_world_set_positions(orc_world* w, const double* x)
     }
     build_cp(w);
 }
+/* vertex positions exactly as given, affine bodies refitted to them but their
+ * vertices left unprojected: for comparing position-level quantities (contact
+ * sets, step bounds) with another implementation at identical positions */
+void orc_world_set_positions_exact(orc_world* w, const double* x)
+{
+    memcpy(w->x, x, sizeof(double) * 3 * w->nv);
+    for(int b = 0; b < w->nabd; b++)
+        body_fit(w, b, w->x, w->bq + 12 * b);
+    build_cp(w);
+}
 int orc_world_nodes(const orc_world* w) { return w->nn; }
 void orc_world_get_positions(const orc_world* w, double* x)
 {

@@ -95,6 +95,7 @@ def lib():
         L.orc_world_create.argtypes = [C.c_void_p]
         L.orc_world_free.argtypes = [C.c_void_p]
         L.orc_world_set_positions.argtypes = [C.c_void_p, dp]
+        L.orc_world_set_positions_exact.argtypes = [C.c_void_p, dp]
         L.orc_world_get_positions.argtypes = [C.c_void_p, dp]
         L.orc_world_set_xtilde.argtypes = [C.c_void_p, dp]
         L.orc_world_get_xtilde.argtypes = [C.c_void_p, dp]
@@ -358,6 +359,11 @@ class World:
 
     def set_positions(self, x):
         lib().orc_world_set_positions(self.h, P(np.ascontiguousarray(x, np.float64).ravel()))
+
+    def set_positions_exact(self, x):
+        """vertex positions kept exactly as given (affine bodies refitted,
+        their vertices not projected): position-level comparisons"""
+        lib().orc_world_set_positions_exact(self.h, P(np.ascontiguousarray(x, np.float64).ravel()))
 
     def xtilde(self):
         x = np.zeros(3 * self.nv)
