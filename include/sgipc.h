@@ -86,6 +86,20 @@ int sgipc_ls_precond_apply(sgipc_ls_t ls, const double* r, double* z);
 /* solve A x = b from x0 = 0 */
 int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x,
                  const sgipc_pcg_cfg* cfg, sgipc_pcg_stats* stats);
+/* sgipc_ls_pcg without allocating.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same solve (FP64 PCG
+ * with the system's preconditioner, same kernels) with every work buffer --
+ * device copies of b and x, PCG vectors, reductions, control words, the
+ * block-Jacobi inverse -- in one fixed device array per GPU (about 22 MB,
+ * held until the process ends).  Limits: at most SGIPC_NOALLOC_MAX_ROWS block
+ * rows (SGIPC_ECAPACITY beyond), FP64 solver precision only (SGIPC_EINVAL
+ * for the mixed mode), called with the system's device current.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race.
+ * stats->ms is host wall time of the call. */
+#define SGIPC_NOALLOC_MAX_ROWS 65536
+int sgipc_ls_pcg_noalloc(sgipc_ls_t ls, const double* b, double* x,
+                         const sgipc_pcg_cfg* cfg, sgipc_pcg_stats* stats);
 /* MAS hierarchy: levels, and level_sizes[2*(levels+1)] = (count, offset)
  * pairs in the reference's d_levelSize layout (MASPreconditioner.cu:1768). */
 int sgipc_ls_mas_info(sgipc_ls_t ls, int32_t* levels, int32_t* level_sizes,

@@ -5,57 +5,15 @@
 #include <cstring>
 
 #include "../../include/sgipc.h"
+#include "capi.h"
 #include "context.h"
-#include "linsys.h"
 
 using namespace sgipc;
 
-namespace
+namespace sgipc
 {
 thread_local std::string g_err;
-
-template <typename F>
-int guard(F&& f)
-{
-    try
-    {
-        f();
-        g_err.clear();
-        return SGIPC_OK;
-    }
-    catch(const Error& e)
-    {
-        g_err = e.what();
-        return e.code;
-    }
-    catch(const std::exception& e)
-    {
-        g_err = e.what();
-        return SGIPC_EHIP;
-    }
-    catch(...)
-    {
-        g_err = "unknown error";
-        return SGIPC_EHIP;
-    }
 }
-}  // namespace
-
-struct sgipc_linear_system
-{
-    hipStream_t  s = nullptr;
-    Bsr          A;
-    Mas          mas;
-    Pcg          pcg;
-    int          precond = 1;
-    DBuf<double> b, x, y;
-    ~sgipc_linear_system()
-    {
-        if(s)
-            (void)hipStreamDestroy(s);
-    }
-};
-
 extern "C" {
 
 const char* sgipc_last_error(void) { return g_err.c_str(); }
@@ -95,6 +53,7 @@ int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr, const int32
         auto* ls = new sgipc_linear_system();
         try
         {
+            SG_HIP(hipGetDevice(&ls->dev));
             SG_HIP(hipStreamCreateWithFlags(&ls->s, hipStreamNonBlocking));
             std::vector<int> rp(rowptr, rowptr + nb + 1), cl(cols, cols + nnzb);
             ls->A.set_pattern(nb, rp, cl, ls->s);

@@ -63,30 +63,36 @@ class DBuf
         : p_(o.p_)
         , n_(o.n_)
         , cap_(o.cap_)
+        , owned_(o.owned_)
     {
-        o.p_   = nullptr;
-        o.n_   = 0;
-        o.cap_ = 0;
+        o.p_     = nullptr;
+        o.n_     = 0;
+        o.cap_   = 0;
+        o.owned_ = true;
     }
     DBuf& operator=(DBuf&& o) noexcept
     {
         if(this != &o)
         {
             release();
-            p_     = o.p_;
-            n_     = o.n_;
-            cap_   = o.cap_;
-            o.p_   = nullptr;
-            o.n_   = 0;
-            o.cap_ = 0;
+            p_       = o.p_;
+            n_       = o.n_;
+            cap_     = o.cap_;
+            owned_   = o.owned_;
+            o.p_     = nullptr;
+            o.n_     = 0;
+            o.cap_   = 0;
+            o.owned_ = true;
         }
         return *this;
     }
-    // grow-only reallocation; contents are not preserved
+    // grow-only reallocation; contents are not preserved.  A borrowed buffer
+    // never reallocates: a size beyond its capacity is an error.
     void resize(size_t n)
     {
         if(n > cap_)
         {
+            SG_CHECK(owned_, -3, "a borrowed device buffer is too small");
             release();
             size_t bytes = (n ? n : 1) * sizeof(T);
             SG_HIP(hipMalloc(&p_, bytes));
@@ -94,13 +100,23 @@ class DBuf
         }
         n_ = n;
     }
+    // use n elements of memory owned elsewhere (never freed here)
+    void borrow(T* p, size_t n)
+    {
+        release();
+        p_     = p;
+        n_     = n;
+        cap_   = n;
+        owned_ = false;
+    }
     void release()
     {
-        if(p_)
+        if(p_ && owned_)
             (void)hipFree(p_);
-        p_   = nullptr;
-        n_   = 0;
-        cap_ = 0;
+        p_     = nullptr;
+        n_     = 0;
+        cap_   = 0;
+        owned_ = true;
     }
     void zero(hipStream_t s = 0)
     {
@@ -131,9 +147,10 @@ class DBuf
     size_t   size() const { return n_; }
 
   private:
-    T*     p_   = nullptr;
-    size_t n_   = 0;
-    size_t cap_ = 0;
+    T*     p_     = nullptr;
+    size_t n_     = 0;
+    size_t cap_   = 0;
+    bool   owned_ = true;
 };
 
 // ---------------------------------------------------------------- profiler

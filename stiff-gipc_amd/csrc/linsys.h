@@ -270,6 +270,16 @@ class Pcg
     double   true_residual(const SysView& S, const double* b, const double* x, hipStream_t s);
     Profiler* prof = nullptr;
 
+    // Work memory in a device region the caller owns instead of the
+    // library's allocations, for FP64 solves of n DOFs: the reduction sets,
+    // the control words, the FP64 work vectors and the block-Jacobi inverse
+    // are carved from base (workspace_doubles(n) doubles, 256-byte aligned),
+    // and the stop flag is read back into host memory held by this object.
+    // After binding, no call of this solver allocates; a mixed-precision
+    // solve is refused.
+    static size_t workspace_doubles(int n);
+    void          bind_workspace(double* base, int n);
+
   private:
     // b64/x64 non-null: FP64 residual replacement at every check (mixed mode)
     template <typename V>
@@ -297,6 +307,10 @@ class Pcg
     int*         h_ctl_ = nullptr;  // pinned: two slots of 4 ints for the stop flag
     hipEvent_t   ev_[2] = {nullptr, nullptr};
     int          n_ = 0;
+    // bound to a caller's workspace: the stop flag goes to h_ctl_local_ with a
+    // stream synchronisation in place of the pinned copy and its event
+    bool         external_ = false;
+    int          h_ctl_local_[8] = {};
 };
 
 }  // namespace sgipc

@@ -834,7 +834,7 @@ void Pcg::apply_diag(int nb, const double* r, double* z, hipStream_t s)
 
 Pcg::~Pcg()
 {
-    if(h_ctl_)
+    if(h_ctl_ && !external_)
         (void)hipHostFree(h_ctl_);
     for(auto& e : ev_)
         if(e)
@@ -862,8 +862,48 @@ const float* Pcg::dinv<float>() const
     return dinv32_.data();
 }
 
+size_t Pcg::workspace_doubles(int n)
+{
+    // reductions, 4 control ints (2 doubles), 4 control doubles, 9 FP64 work
+    // vectors and the 3x3 block-Jacobi inverse (9 per block row = 3 n), each
+    // region rounded up to 256 bytes
+    auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+    return up((size_t)(2 * kRing + 4) * kSlotSet) + up(2) + up(4) + 9 * up((size_t)n)
+           + up(3 * (size_t)n);
+}
+
+void Pcg::bind_workspace(double* base, int n)
+{
+    auto    up = [](size_t d) { return (d + 31) / 32 * 32; };
+    double* p  = base;
+    red_.borrow(p, (size_t)(2 * kRing + 4) * kSlotSet);
+    p += up((size_t)(2 * kRing + 4) * kSlotSet);
+    ctl_.borrow(reinterpret_cast<int*>(p), 4);
+    p += up(2);
+    ctl_d_.borrow(p, 4);
+    p += up(4);
+    for(DBuf<double>* b : {&v64_.r, &v64_.z, &v64_.p[0], &v64_.p[1], &v64_.Ap, &v64_.tmp,
+                           &v64_.res, &v64_.rhs, &v64_.sol})
+    {
+        b->borrow(p, (size_t)n);
+        p += up((size_t)n);
+    }
+    v64_.n = n;
+    dinv_.borrow(p, 3 * (size_t)n);
+    if(h_ctl_ && !external_)
+        (void)hipHostFree(h_ctl_);
+    h_ctl_    = h_ctl_local_;
+    external_ = true;
+    n_        = n;
+}
+
 void Pcg::ensure(int n)
 {
+    if(external_)
+    {
+        SG_CHECK(n == n_, -1, "solver workspace bound for another size");
+        return;
+    }
     red_.resize((size_t)(2 * kRing + 4) * kSlotSet);
     ctl_.resize(4);
     ctl_d_.resize(4);
@@ -995,10 +1035,14 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
             if(reliable && cfg.stall_checks > 0)
                 SG_HIP(hipMemcpyAsync(h_ctl_ + 4 * slot + 2, ctl_d_.data() + 1, sizeof(double),
                                       hipMemcpyDeviceToHost, s));
-            SG_HIP(hipEventRecord(ev_[slot], s));
+            if(external_)
+                SG_HIP(hipStreamSynchronize(s));  // the flag is in pageable memory
+            else
+                SG_HIP(hipEventRecord(ev_[slot], s));
             if(have_pending)
             {
-                SG_HIP(hipEventSynchronize(ev_[pend_slot]));
+                if(!external_)
+                    SG_HIP(hipEventSynchronize(ev_[pend_slot]));
                 if(h_ctl_[4 * pend_slot] != 0)
                     break;
                 if(reliable && cfg.stall_checks > 0)
@@ -1098,7 +1142,10 @@ PcgStats Pcg::solve(const SysView& S, Precond pc, Mas* mas, const double* b, dou
                     const PcgConfig& cfg, hipStream_t s)
 {
     if(cfg.mixed)
+    {
+        SG_CHECK(!external_, -1, "a solver on a caller's workspace runs FP64 solves only");
         return solve_mixed(S, pc, mas, b, x, cfg, s);
+    }
     PcgStats st          = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
     st.outer_iterations  = 1;
     st.rel_residual      = true_residual(S, b, x, s);

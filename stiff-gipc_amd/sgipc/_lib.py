@@ -89,7 +89,7 @@ EXPORTS = [
     "sgipc_set_state", "sgipc_set_pcg", "sgipc_learned_mode", "sgipc_warm_start_adaptive",
     "sgipc_learned_samples",
     "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
-    "sgipc_ccd_bound", "sgipc_ccd_candidates",
+    "sgipc_ccd_bound", "sgipc_ccd_candidates", "sgipc_ls_pcg_noalloc",
 ]
 MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
 
@@ -108,6 +108,8 @@ def lib():
         L.sgipc_ls_spmv.argtypes = [C.c_void_p, dp, dp]
         L.sgipc_ls_precond_apply.argtypes = [C.c_void_p, dp, dp]
         L.sgipc_ls_pcg.argtypes = [C.c_void_p, dp, dp, C.POINTER(PcgCfg), C.POINTER(PcgStats)]
+        L.sgipc_ls_pcg_noalloc.argtypes = [C.c_void_p, dp, dp, C.POINTER(PcgCfg),
+                                           C.POINTER(PcgStats)]
         L.sgipc_ls_mas_info.argtypes = [C.c_void_p, ip, ip, ip]
         L.sgipc_ls_mas_chain.argtypes = [C.c_void_p, ip]
         L.sgipc_ls_mas_bank.argtypes = [C.c_void_p, C.c_int32, dp, dp]
@@ -205,6 +207,16 @@ class LinearSystem:
         st = PcgStats()
         check(lib().sgipc_ls_pcg(self.h, P(np.ascontiguousarray(b, np.float64)), P(x),
                                  C.byref(cfg), C.byref(st)))
+        return x, st
+
+    def pcg_noalloc(self, b, rel_tol=1e-4, abs_tol=0.0, max_iter=0, use_rz=True,
+                    check_every=10):
+        """the FP64 solve of pcg() with no device allocation (static workspace)"""
+        x = np.zeros(3 * self.nb)
+        cfg = PcgCfg(rel_tol, abs_tol, max_iter, int(use_rz), check_every, SGIPC_SOLVE_FP64)
+        st = PcgStats()
+        check(lib().sgipc_ls_pcg_noalloc(self.h, P(np.ascontiguousarray(b, np.float64)), P(x),
+                                         C.byref(cfg), C.byref(st)))
         return x, st
 
     def mas_info(self):
