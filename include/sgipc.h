@@ -166,9 +166,17 @@ int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
                    int32_t* cols, double* vals, int32_t cap);
 
 /* Incremental potential at the current positions, FP64 (computeEnergy,
- * GIPC.cu:10392-10451).  terms[8] = {total, kinetic, elastic (tets),
- * barrier, ground, friction, cloth (membrane + bending), affine shape}. */
+ * GIPC.cu:10392-10451).  terms[9] = {total, kinetic, elastic (tets),
+ * barrier, ground, friction, cloth (membrane + bending), affine shape,
+ * soft targets}. */
 int sgipc_energy(sgipc_t ctx, double* terms);
+
+/* New positions of the animated soft targets, targets[3 n_soft] in the
+ * order of sgipc_scene_desc.soft_index (the per-frame targetVert upload of
+ * main.cu:1117-1140), and the current animation rate animation_fullRate
+ * (0 before the first time step, 1 after a completed one). */
+int sgipc_set_soft_targets(sgipc_t ctx, const double* targets);
+int sgipc_get_animation_rate(sgipc_t ctx, double* rate);
 
 /* Step-size bounds of a vertex direction dx [3 n_verts] at the current
  * positions (x_new = x - alpha dx; solve_subIP, GIPC.cu:10786-10822):

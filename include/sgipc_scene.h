@@ -107,6 +107,19 @@ typedef struct sgipc_scene_desc {
     int32_t precond;        /* enum sgipc_precond                               */
     int32_t mas_precision;  /* enum sgipc_mas_precision                         */
     int32_t solver_precision; /* enum sgipc_solver_precision                    */
+    /* ---- animated soft targets (reference TetMesh.targetIndex / targetVert,
+     * scene JSON "animation" + settings motion_stiffness / motion_rate;
+     * GIPC.cu:6365-6435, 7527-7560, 10941-11060) ----
+     * vertex soft_index[i] is pulled towards soft_target[i] by the energy
+     * 0.5 k |r (x - t)|^2, k = soft_stiffness, r the animation rate: a time
+     * step runs Newton loops at r = s, 2 s, ... until r > 0.99, with
+     * s = 1 / motion_rate (animation_subRate), the lagged friction sets rebuilt
+     * between loops.  Free FEM vertices only, each at most once.            */
+    int32_t         n_soft;
+    const uint32_t* soft_index;   /* [n_soft]                                 */
+    const double*   soft_target;  /* [3*n_soft] (sgipc_set_soft_targets moves them) */
+    double          soft_stiffness; /* motion_stiffness (softMotionRate)      */
+    double          motion_rate;    /* <= 0 -> 1                               */
 } sgipc_scene_desc;
 
 #ifdef __cplusplus

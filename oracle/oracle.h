@@ -238,6 +238,9 @@ orc_world* orc_world_create(const sgipc_scene_desc* s);
 void       orc_world_free(orc_world* w);
 void       orc_world_set_positions(orc_world* w, const double* x);
 void       orc_world_set_positions_exact(orc_world* w, const double* x);
+void       orc_world_set_soft_targets(orc_world* w, const double* t);
+void       orc_world_set_anim_rate(orc_world* w, double r);
+double     orc_world_get_anim_rate(const orc_world* w);
 void       orc_world_get_positions(const orc_world* w, double* x);
 void       orc_world_set_xtilde(orc_world* w, const double* xt);
 void       orc_world_get_xtilde(const orc_world* w, double* xt);
@@ -248,6 +251,8 @@ int  orc_world_nodes(const orc_world* w);
 /* assembled RHS gradient (3 nodes) and full node BSR Hessian (caller frees) */
 int  orc_world_assemble(orc_world* w, double* grad, orc_bsr* H);
 /* terms[8] as sgipc_energy */
+/* terms[9] = {total, kinetic, elastic, barrier, ground, friction, cloth,
+ * affine shape, soft targets} */
 void orc_world_energy(orc_world* w, double* terms);
 /* one Newton iteration; stats fields mirror sgipc_iter_stats */
 int  orc_world_newton_iteration(orc_world* w, double* alpha_out,

@@ -74,6 +74,12 @@ struct orc_world
     int       newton_k;
     double    xdelta0;
     int       in_step;
+    /* animated soft targets and the animation rate (animation_fullRate,
+     * animation_subRate) */
+    int       nsoft;
+    uint32_t* soft_idx;
+    double*   soft_tgt;
+    double    anim_rate, anim_sub;
 };
 
 static void begin_step(orc_world* w);
@@ -745,6 +751,13 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     w->s.faces = w->faces;
     w->s.edges = w->edges;
     w->s.surf_verts = w->surf;
+    w->nsoft    = s->n_soft > 0 ? s->n_soft : 0;
+    w->soft_idx = (uint32_t*)dup(s->soft_index, sizeof(uint32_t) * w->nsoft);
+    w->soft_tgt = (double*)dup(s->soft_target, sizeof(double) * 3 * w->nsoft);
+    w->s.soft_index  = w->soft_idx;
+    w->s.soft_target = w->soft_tgt;
+    w->anim_sub  = s->motion_rate > 0.0 ? 1.0 / s->motion_rate : 1.0;
+    w->anim_rate = 0.0; /* GIPC.cuh:114 */
     w->dminv  = (double*)malloc(sizeof(double) * 9 * (nt ? nt : 1));
     w->vol    = (double*)malloc(sizeof(double) * (nt ? nt : 1));
     w->mu     = (double*)malloc(sizeof(double) * (nt ? nt : 1));
@@ -950,6 +963,7 @@ void orc_world_free(orc_world* w)
     free(w->abd_of); free(w->tet_abd); free(w->bstart); free(w->bfix); free(w->xbar);
     free(w->bM); free(w->bq); free(w->bqp); free(w->bqv); free(w->bqt); free(w->bqs);
     free(w->bG); free(w->bV); free(w->npart); free(w->dy);
+    free(w->soft_idx); free(w->soft_tgt);
     free(w);
 }
 
@@ -1063,6 +1077,22 @@ static void compute_grad_hess(orc_world* w, trip_t* T)
         for(int c = 0; c < 3; c++)
             w->gshape[3 * v + c] =
                 v < w->nfem ? (w->x[3 * v + c] - w->xt[3 * v + c]) * w->mass[v] : 0.0;
+    /* soft targets (_computeSoftConstraintGradientAndHessian, GIPC.cu:6365-6409):
+     * gradient d rate rate (x - t), Hessian rate rate d I (d = motion stiffness) */
+    for(int i = 0; i < w->nsoft; i++)
+    {
+        int    v  = (int)w->soft_idx[i];
+        double d  = w->s.soft_stiffness, r = w->anim_rate;
+        double gk = d * r * r, hk = r * r * d;
+        for(int c = 0; c < 3; c++)
+            w->gshape[3 * v + c] += gk * (w->x[3 * v + c] - w->soft_tgt[3 * i + c]);
+        if(T)
+        {
+            double H[9] = {hk, 0, 0, 0, hk, 0, 0, 0, hk};
+            int    ids[1] = {v};
+            push_element(T, 1, ids, H, 1, w);
+        }
+    }
     /* barrier */
     for(int i = 0; i < w->npairs; i++)
     {
@@ -1384,7 +1414,21 @@ void orc_world_energy(orc_world* w, double* terms)
     for(int h = 0; h < w->nh; h++)
         cloth += orc_bend_energy(w->x, w->hinges + 4 * h, w->h_theta0[h], w->h_len0[h], w->c_bend);
     cloth *= w->s.dt * w->s.dt;
-    terms[0] = kin + ela + bar + gnd + fric + cloth + shape;
+    /* _computeSoftConstraintEnergy_Reduction (GIPC.cu:7527-7560): d |rate (x - t)|^2 / 2 */
+    double soft = 0.0;
+    for(int i = 0; i < w->nsoft; i++)
+    {
+        int    v   = (int)w->soft_idx[i];
+        double dis = 0.0;
+        for(int c = 0; c < 3; c++)
+        {
+            double e = (w->x[3 * v + c] - w->soft_tgt[3 * i + c]) * w->anim_rate;
+            dis += e * e;
+        }
+        soft += w->s.soft_stiffness * dis * 0.5;
+    }
+    terms[0] = kin + ela + bar + gnd + fric + cloth + shape + soft;
+    terms[8] = soft;
     terms[1] = kin;
     terms[2] = ela;
     terms[3] = bar;
@@ -1396,7 +1440,7 @@ void orc_world_energy(orc_world* w, double* terms)
 
 static double energy(orc_world* w)
 {
-    double t[8];
+    double t[9];
     orc_world_energy(w, t);
     return t[0];
 }
@@ -1448,6 +1492,14 @@ static void init_kappa(orc_world* w)
         for(int a = 0; a < 4; a++)
             for(int c = 0; c < 3; c++)
                 GE[3 * t[a] + c] += g[3 * a + c];
+    }
+    /* computeSoftConstraintGradient at the current animation rate */
+    for(int i = 0; i < w->nsoft; i++)
+    {
+        int    v  = (int)w->soft_idx[i];
+        double gk = w->s.soft_stiffness * w->anim_rate * w->anim_rate;
+        for(int c = 0; c < 3; c++)
+            GE[3 * v + c] += gk * (w->x[3 * v + c] - w->soft_tgt[3 * i + c]);
     }
     for(int i = 0; i < w->ngp; i++)
     {
@@ -1780,6 +1832,7 @@ static void begin_step(orc_world* w)
     if(w->kappa < 1e-16)
         suggest_kappa(w, &w->kappa);
     init_kappa(w);
+    w->anim_rate = w->anim_sub; /* animation_fullRate = animation_subRate (GIPC.cu:11018) */
     build_friction_sets(w);
     memset(w->dx, 0, sizeof(double) * 3 * w->nv);
     memset(w->dy, 0, sizeof(double) * 3 * (w->nn ? w->nn : 1));
@@ -1824,25 +1877,56 @@ static void end_step(orc_world* w)
     w->in_step = 0;
 }
 
+/* after a Newton loop: 1 when the animation is complete (animation_fullRate >
+ * 0.99) and the step ends, else the next loop at the next rate with the lagged
+ * friction sets rebuilt and a fresh solve_subIP state (GIPC.cu:11038-11079) */
+static int next_newton_loop(orc_world* w)
+{
+    if(w->anim_rate > 0.99)
+        return 1;
+    w->anim_rate += w->anim_sub;
+    build_friction_sets(w);
+    memset(w->dx, 0, sizeof(double) * 3 * w->nv);
+    memset(w->dy, 0, sizeof(double) * 3 * (w->nn ? w->nn : 1));
+    w->newton_k = 0;
+    w->xdelta0  = 0.0;
+    return 0;
+}
+
 int orc_world_time_step(orc_world* w, int* newton_iters_out, int* pcg_iters_out)
 {
     begin_step(w);
-    int total_pcg = 0, k = 0;
+    int total_pcg = 0, total_k = 0;
     int cap = w->s.newton_max_iter > 0 ? w->s.newton_max_iter : 10000;
-    for(k = 0; k < cap; k++)
+    do
     {
-        int it = 0, conv = 0;
-        orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
-        if(conv)
-            break;
-        total_pcg += it;
-    }
+        for(int k = 0; k < cap; k++)
+        {
+            int it = 0, conv = 0;
+            orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
+            if(conv)
+                break;
+            total_pcg += it;
+            total_k++;
+        }
+    } while(!next_newton_loop(w));
     end_step(w);
     if(newton_iters_out)
-        *newton_iters_out = k;
+        *newton_iters_out = total_k;
     if(pcg_iters_out)
         *pcg_iters_out = total_pcg;
     return 0;
+}
+
+/* animation rate of the soft targets (animation_fullRate), for tests */
+void orc_world_set_anim_rate(orc_world* w, double r) { w->anim_rate = r; }
+double orc_world_get_anim_rate(const orc_world* w) { return w->anim_rate; }
+
+/* new targets of the soft constraints (the frame update of main.cu:1117-1140) */
+void orc_world_set_soft_targets(orc_world* w, const double* t)
+{
+    if(w->nsoft)
+        memcpy(w->soft_tgt, t, sizeof(double) * 3 * w->nsoft);
 }
 
 /* exported ACCD entry points for the oracle's own tests (displacements are
@@ -1891,12 +1975,13 @@ int orc_world_advance(orc_world* w, int n, int* steps_out, int* pcg_out)
         orc_world_newton_iteration(w, NULL, &it, NULL, &conv);
         if(conv)
         {
-            end_step(w);
+            if(next_newton_loop(w))
+                end_step(w);
             continue;
         }
         done++;
         pcg += it;
-        if(w->newton_k >= cap)
+        if(w->newton_k >= cap && next_newton_loop(w))
             end_step(w);
     }
     if(steps_out)

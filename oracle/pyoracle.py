@@ -96,6 +96,10 @@ def lib():
         L.orc_world_free.argtypes = [C.c_void_p]
         L.orc_world_set_positions.argtypes = [C.c_void_p, dp]
         L.orc_world_set_positions_exact.argtypes = [C.c_void_p, dp]
+        L.orc_world_set_soft_targets.argtypes = [C.c_void_p, dp]
+        L.orc_world_set_anim_rate.argtypes = [C.c_void_p, C.c_double]
+        L.orc_world_get_anim_rate.restype = C.c_double
+        L.orc_world_get_anim_rate.argtypes = [C.c_void_p]
         L.orc_world_get_positions.argtypes = [C.c_void_p, dp]
         L.orc_world_set_xtilde.argtypes = [C.c_void_p, dp]
         L.orc_world_get_xtilde.argtypes = [C.c_void_p, dp]
@@ -391,8 +395,21 @@ class World:
         lib().orc_world_assemble(self.h, P(g), C.byref(raw))
         return g, OBsr(raw)
 
+    def set_soft_targets(self, t):
+        lib().orc_world_set_soft_targets(self.h, P(np.ascontiguousarray(t, np.float64).ravel()))
+
+    @property
+    def anim_rate(self):
+        return lib().orc_world_get_anim_rate(self.h)
+
+    @anim_rate.setter
+    def anim_rate(self, r):
+        lib().orc_world_set_anim_rate(self.h, float(r))
+
     def energy(self):
-        t = np.zeros(8)
+        """{total, kinetic, elastic, barrier, ground, friction, cloth, affine
+        shape, soft targets}"""
+        t = np.zeros(9)
         lib().orc_world_energy(self.h, P(t))
         return t
 

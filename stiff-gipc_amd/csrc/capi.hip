@@ -332,6 +332,22 @@ int sgipc_energy(sgipc_t h, double* terms)
     });
 }
 
+int sgipc_set_soft_targets(sgipc_t ctx, const double* targets)
+{
+    return guard([&] {
+        SG_CHECK(targets, SGIPC_EINVAL, "null targets");
+        ctx_of(ctx).set_soft_targets(targets);
+    });
+}
+
+int sgipc_get_animation_rate(sgipc_t ctx, double* rate)
+{
+    return guard([&] {
+        SG_CHECK(rate, SGIPC_EINVAL, "null argument");
+        *rate = ctx_of(ctx).animation_rate();
+    });
+}
+
 int sgipc_ccd_bound(sgipc_t h, const double* dx, double* out)
 {
     return guard([&] {

@@ -26,6 +26,9 @@ class Context
     void get_xtilde(double* out) { get(xt_, out); }
     double kappa() const { return kappa_; }
     void   set_kappa(double k) { kappa_ = k; }
+    // new positions of the animated soft targets ([3 n_soft])
+    void   set_soft_targets(const double* t);
+    double animation_rate() const { return dev_.soft_rate; }
 
     // hot-path pieces exposed through the C ABI
     void lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs);
@@ -80,6 +83,9 @@ class Context
     void   check_overflow();
     void   begin_step();                   // IPC_Solver prologue (kappa)
     void   end_step();                     // updateVelocities + computeXTilta
+    // after a Newton loop: true when the animation is complete and the step
+    // ends; else the next loop at the next rate (IPC_Solver sub-steps)
+    bool   next_newton_loop();
     double read_d(const double* p);
     void   step_forward(double alpha);
     void   body_fit(int b, const double* f, double* q) const;
@@ -117,6 +123,10 @@ class Context
     DBuf<int4>     pairs_, pairs_raw_, cand_;
     DBuf<int>      overflow_;
     DBuf<double>   epart_, eterms_;
+    DBuf<uint32_t> soft_idx_;                                           // soft targets
+    DBuf<double>   soft_tgt_;
+    int            nsoft_ = 0;
+    double         anim_sub_ = 1.0;  // animation_subRate = 1 / motion_rate
     int            pair_cap_ = 0, cand_cap_ = 0;
 
     Mlp          mlp_;

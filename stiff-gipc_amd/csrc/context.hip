@@ -664,6 +664,32 @@ Context::Context(const sgipc_scene_desc& d)
     f.basis = fr_basis_.data(), f.count = fr_count_.data();
     f.gids = fg_ids_.data(), f.glam = fg_lam_.data(), f.gcount = fr_count_.data() + 1;
 
+    // animated soft targets: free FEM vertices, each at most once (main.cu
+    // rejects a vertex selected by two constraints)
+    nsoft_ = d.n_soft > 0 ? d.n_soft : 0;
+    if(nsoft_)
+    {
+        SG_CHECK(d.soft_index && d.soft_target, SGIPC_EINVAL, "soft target arrays missing");
+        std::vector<char> seen(nv_, 0);
+        for(int i = 0; i < nsoft_; i++)
+        {
+            uint32_t v = d.soft_index[i];
+            SG_CHECK(v < (uint32_t)nfem_, SGIPC_EINVAL,
+                     "soft target vertex out of range or on an affine body");
+            SG_CHECK(bt[v] == 0, SGIPC_EINVAL, "soft target on a fixed vertex");
+            SG_CHECK(!seen[v], SGIPC_EINVAL, "a vertex carries two soft targets");
+            seen[v] = 1;
+        }
+        soft_idx_.upload(d.soft_index, nsoft_, s_);
+        soft_tgt_.upload(d.soft_target, 3 * (size_t)nsoft_, s_);
+    }
+    anim_sub_       = d.motion_rate > 0.0 ? 1.0 / d.motion_rate : 1.0;
+    dev_.nsoft      = nsoft_;
+    dev_.soft_idx   = soft_idx_.data();
+    dev_.soft_tgt   = soft_tgt_.data();
+    dev_.soft_k     = d.soft_stiffness;
+    dev_.soft_rate  = 0.0;  // animation_fullRate before the first step (GIPC.cuh:114)
+
     EnergyLayout L = energy_layout(dev_);
     epart_.resize(L.off[kEnergyTerms - 1] + L.n[kEnergyTerms - 1] + 1);
     eterms_.resize(kEnergyTerms + 1);
@@ -875,6 +901,7 @@ void Context::assemble(bool hessian)
         dev_.hfail  = coo_.hfail.data();
         dev_.hmask  = (unsigned long long)(coo_.hcap - 1);
         launch_init_diag(dev_, s_);
+        launch_soft(dev_, hessian, gs_.data(), s_);
         launch_abd_body(dev_, hessian, s_);
         launch_barrier(dev_, kappa_, hessian, gc_.data(), s_);
         launch_friction(dev_, hessian, gc_.data(), s_);
@@ -1058,6 +1085,7 @@ void Context::init_kappa()
     // gradient out, GIPC.cu:9809), gc = unit-kappa ground + barrier
     launch_init_diag(dev_, s_);
     launch_fem(dev_, false, s_, false);
+    launch_soft(dev_, false, gs_.data(), s_);  // computeSoftConstraintGradient(_GE)
     launch_ground(dev_, 1.0, false, gc_.data(), s_);
     launch_barrier(dev_, 1.0, false, gc_.data(), s_);
     SG_HIP(hipMemsetAsync(scratch_.data() + 4, 0, 2 * sizeof(double), s_));
@@ -1368,6 +1396,7 @@ void Context::begin_step()
     if(kappa_ < 1e-16)
         suggest_kappa(kappa_);
     init_kappa();
+    dev_.soft_rate = anim_sub_;  // animation_fullRate = animation_subRate (GIPC.cu:11018)
     // buildFrictionSets on the current contact sets (GIPC.cu:11016)
     launch_friction_build(dev_, kappa_, s_);
     dx_.zero(s_);
@@ -1376,6 +1405,31 @@ void Context::begin_step()
     newton_k_ = 0;
     xdelta0_  = 0;
     in_step_  = true;
+}
+
+// IPC_Solver's animation loop (GIPC.cu:11038-11079): the step ends once
+// animation_fullRate > 0.99; otherwise the rate advances by the sub-rate, the
+// lagged friction sets are rebuilt and a fresh Newton loop (solve_subIP)
+// starts from a zero direction
+bool Context::next_newton_loop()
+{
+    if(dev_.soft_rate > 0.99)
+        return true;
+    dev_.soft_rate += anim_sub_;
+    launch_friction_build(dev_, kappa_, s_);
+    dx_.zero(s_);
+    if(nabd_)
+        dy_.zero(s_);
+    newton_k_ = 0;
+    xdelta0_  = 0;
+    return false;
+}
+
+void Context::set_soft_targets(const double* t)
+{
+    SG_CHECK(nsoft_ > 0, SGIPC_ESTATE, "the scene has no soft targets");
+    soft_tgt_.upload(t, 3 * (size_t)nsoft_, s_);
+    SG_HIP(hipStreamSynchronize(s_));
 }
 
 void Context::end_step()
@@ -1403,7 +1457,8 @@ void Context::advance(int n, sgipc_advance_stats* st)
         newton_iteration(&is);
         if(is.converged)
         {
-            end_step();
+            if(next_newton_loop())
+                end_step();
             continue;
         }
         loc.newton_iterations++;
@@ -1420,7 +1475,8 @@ void Context::advance(int n, sgipc_advance_stats* st)
         if(newton_k_ >= cap)
         {
             loc.capped_steps++;
-            end_step();
+            if(next_newton_loop())
+                end_step();
         }
     }
     SG_HIP(hipEventRecord(e1, s_));
@@ -1469,20 +1525,26 @@ void Context::time_step(sgipc_step_stats* st)
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, s_));
     begin_step();
-    int cap = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
+    int  cap = desc_.newton_max_iter > 0 ? desc_.newton_max_iter : 10000;
     int  k = 0, pcg_total = 0;
-    bool met = false;
-    for(k = 0; k < cap; k++)
+    bool met = true;
+    do  // one Newton loop per animation rate
     {
-        sgipc_iter_stats is{};
-        newton_iteration(&is);
-        if(is.converged)
+        bool loop_met = false;
+        for(int i = 0; i < cap; i++)
         {
-            met = true;
-            break;
+            sgipc_iter_stats is{};
+            newton_iteration(&is);
+            if(is.converged)
+            {
+                loop_met = true;
+                break;
+            }
+            pcg_total += is.pcg_iterations;
+            k++;
         }
-        pcg_total += is.pcg_iterations;
-    }
+        met = met && loop_met;
+    } while(!next_newton_loop());
     end_step();
     SG_HIP(hipEventRecord(e1, s_));
     SG_HIP(hipStreamSynchronize(s_));
@@ -1512,6 +1574,7 @@ void Context::energy_terms(double* terms)
     terms[5] = t[4] + t[5];
     terms[6] = t[6] + t[7];
     terms[7] = t[8];
+    terms[8] = t[9];
 }
 
 void Context::lbvh_build(int which, uint64_t* keys, uint32_t* nodes, double* aabbs)

@@ -48,6 +48,32 @@ __global__ void k_init_diag(int nv, int nfem, const double* __restrict__ mass,
     }
 }
 
+// animated soft targets (_computeSoftConstraintGradientAndHessian,
+// GIPC.cu:6365-6409): gradient d rate rate (x - t) added to the vertex's
+// shape gradient and rate rate d on its diagonal block (each vertex carries at
+// most one target, so no two lanes touch one vertex)
+__global__ void k_soft(int n, const uint32_t* __restrict__ idx, const double* __restrict__ tgt,
+                       const double* __restrict__ x, double d, double rate,
+                       const int* __restrict__ diag, double* __restrict__ vals,
+                       double* __restrict__ gshape, int hessian)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    const size_t v  = idx[i];
+    const double gk = d * rate * rate;
+    for(int c = 0; c < 3; c++)
+        gshape[3 * v + c] += gk * (x[3 * v + c] - tgt[3 * (size_t)i + c]);
+    if(hessian)
+    {
+        const double hk = rate * rate * d;
+        double*      B  = vals + 9 * (size_t)diag[v];
+        B[0] += hk;
+        B[4] += hk;
+        B[8] += hk;
+    }
+}
+
 // upper-triangle index of (p, q), p <= q < 12
 __device__ __forceinline__ int uidx(int p, int q) { return p * 12 - p * (p - 1) / 2 + (q - p); }
 
@@ -542,7 +568,23 @@ __global__ void __launch_bounds__(256) k_energy_ground(const unsigned* __restric
     block_partial(e, out);
 }
 
-// sum the partial arrays (fixed order) into terms[0..8]; terms[9] = total
+// _computeSoftConstraintEnergy_Reduction (GIPC.cu:7527-7560): d |rate (x - t)|^2 / 2
+__global__ void __launch_bounds__(256) k_energy_soft(int n, const uint32_t* __restrict__ idx,
+                                                     const double* __restrict__ tgt,
+                                                     const double* __restrict__ x, double d,
+                                                     double rate, double* out)
+{
+    int    i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if(i < n)
+    {
+        V3 dv = (ld3(x + 3 * (size_t)idx[i]) - ld3(tgt + 3 * (size_t)i)) * rate;
+        e     = d * sqn(dv) * 0.5;
+    }
+    block_partial(e, out);
+}
+
+// sum the partial arrays (fixed order) into terms[0..9]; terms[10] = total
 __global__ void __launch_bounds__(256) k_energy_final(const double* __restrict__ partial,
                                                       EnergyLayout lay, double* __restrict__ terms)
 {
@@ -637,6 +679,16 @@ void launch_init_diag(const SimDev& d, hipStream_t s)
     SG_LAUNCH_CHECK();
 }
 
+void launch_soft(const SimDev& d, bool hessian, double* gshape, hipStream_t s)
+{
+    if(d.nsoft == 0)
+        return;
+    hipLaunchKernelGGL(k_soft, dim3(grid_for(d.nsoft, 256)), dim3(256), 0, s, d.nsoft,
+                       d.soft_idx, d.soft_tgt, d.x, d.soft_k, d.soft_rate, d.diag, d.vals,
+                       gshape, hessian ? 1 : 0);
+    SG_LAUNCH_CHECK();
+}
+
 void launch_fem(const SimDev& d, bool hessian, hipStream_t s, bool cloth)
 {
     if(d.nt == 0 && (!cloth || (d.cloth.nf == 0 && d.cloth.nh == 0)))
@@ -713,6 +765,7 @@ EnergyLayout energy_layout(const SimDev& d)
     L.n[6]   = d.cloth.nf ? (int)grid_for(d.cloth.nf, 256) : 0;
     L.n[7]   = d.cloth.nh ? (int)grid_for(d.cloth.nh, 256) : 0;
     L.n[8]   = d.abd.nb ? (int)grid_for(d.abd.nb, 256) : 0;
+    L.n[9]   = d.nsoft ? (int)grid_for(d.nsoft, 256) : 0;
     L.off[0] = 0;
     for(int k = 1; k < kEnergyTerms; k++)
         L.off[k] = L.off[k - 1] + L.n[k - 1];
@@ -737,6 +790,9 @@ void launch_energy(const SimDev& d, double kappa, double* partial, double* terms
     launch_energy_friction(d, L, partial, s);
     launch_energy_cloth(d, L, partial, s);
     launch_energy_abd(d, L, partial, s);
+    if(L.n[9])
+        hipLaunchKernelGGL(k_energy_soft, dim3(L.n[9]), dim3(256), 0, s, d.nsoft, d.soft_idx,
+                           d.soft_tgt, d.x, d.soft_k, d.soft_rate, partial + L.off[9]);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial, L, terms);
     SG_LAUNCH_CHECK();
 }

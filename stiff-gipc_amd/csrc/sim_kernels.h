@@ -104,6 +104,12 @@ struct SimDev
     const int*    node_of = nullptr;
     const double* xbar    = nullptr;
     AbdDev        abd;
+    // animated soft targets (free FEM vertices, each at most once), the
+    // motion stiffness and the current animation rate
+    int             nsoft    = 0;
+    const uint32_t* soft_idx = nullptr;
+    const double*   soft_tgt = nullptr;
+    double          soft_k = 0.0, soft_rate = 0.0;
 };
 
 constexpr int kFemStride = 102;  // 10 blocks x 9 + 12 gradient entries per tet
@@ -113,8 +119,8 @@ constexpr int kPairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
 // partial-sum segments of the energy terms: kinetic, elastic, barrier,
 // ground, friction, ground friction, cloth membrane, cloth bending, affine
-// shape
-constexpr int kEnergyTerms = 9;
+// shape, soft targets
+constexpr int kEnergyTerms = 10;
 struct EnergyLayout
 {
     int n[kEnergyTerms];
@@ -123,6 +129,8 @@ struct EnergyLayout
 
 // fem.hip
 void launch_init_diag(const SimDev& d, hipStream_t s);
+// soft-target gradient into gshape and, with hessian, r^2 k I on the diagonal blocks
+void launch_soft(const SimDev& d, bool hessian, double* gshape, hipStream_t s);
 // cloth = false leaves the cloth records out of the gathered gradient
 void launch_fem(const SimDev& d, bool hessian, hipStream_t s, bool cloth = true);
 void launch_ground_detect(const SimDev& d, hipStream_t s);

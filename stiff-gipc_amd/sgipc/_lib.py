@@ -90,6 +90,7 @@ EXPORTS = [
     "sgipc_learned_samples",
     "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
     "sgipc_ccd_bound", "sgipc_ccd_candidates", "sgipc_ls_pcg_noalloc",
+    "sgipc_set_soft_targets", "sgipc_get_animation_rate",
 ]
 MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
 
@@ -127,6 +128,8 @@ def lib():
             L.sgipc_contact_query.argtypes = [C.c_void_p, ip, C.c_int32, ip, up, C.c_int32, ip]
             L.sgipc_assemble.argtypes = [C.c_void_p, dp, ip, ip, ip, dp, C.c_int32]
             L.sgipc_energy.argtypes = [C.c_void_p, dp]
+            L.sgipc_set_soft_targets.argtypes = [C.c_void_p, dp]
+            L.sgipc_get_animation_rate.argtypes = [C.c_void_p, dp]
             L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
             L.sgipc_ccd_bound.argtypes = [C.c_void_p, dp, dp]
             L.sgipc_ccd_candidates.argtypes = [C.c_void_p, dp, C.c_double, ip, C.c_int32, ip]
@@ -326,9 +329,20 @@ class Simulation:
         return g, rowptr, cols, vals
 
     def energy(self):
-        t = np.zeros(8)
+        """{total, kinetic, elastic, barrier, ground, friction, cloth, affine
+        shape, soft targets}"""
+        t = np.zeros(9)
         check(lib().sgipc_energy(self.h, P(t)))
         return t
+
+    def set_soft_targets(self, t):
+        check(lib().sgipc_set_soft_targets(self.h, P(np.ascontiguousarray(t, np.float64).ravel())))
+
+    @property
+    def animation_rate(self):
+        r = C.c_double()
+        check(lib().sgipc_get_animation_rate(self.h, C.byref(r)))
+        return r.value
 
     def newton_iteration(self):
         st = IterStats()

@@ -11,7 +11,7 @@ plus the collision surface (faces, edges, surface vertices).
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -71,6 +71,11 @@ class SceneDesc(C.Structure):
         ("precond", C.c_int32),
         ("mas_precision", C.c_int32),
         ("solver_precision", C.c_int32),
+        ("n_soft", C.c_int32),
+        ("soft_index", C.POINTER(C.c_uint32)),
+        ("soft_target", C.POINTER(C.c_double)),
+        ("soft_stiffness", C.c_double),
+        ("motion_rate", C.c_double),
     ]
 
 
@@ -126,6 +131,10 @@ class Scene:
     precond: int = 1
     mas_precision: int = 0
     solver_precision: int = 0
+    soft_index: np.ndarray | None = None   # (K,) uint32 animated soft-target vertices
+    soft_target: np.ndarray | None = None  # (K,3) float64 their targets
+    soft_stiffness: float = 0.0            # motion_stiffness
+    motion_rate: float = 1.0               # Newton loops per step (animation_subRate = 1/rate)
     name: str = "scene"
     meta: dict = field(default_factory=dict)
 
@@ -156,6 +165,9 @@ class Scene:
             np.ascontiguousarray(self.surf_verts, np.uint32),
             None if self.tris is None else np.ascontiguousarray(self.tris, np.uint32),
             None if self.abd_body is None else np.ascontiguousarray(self.abd_body, np.int32),
+            None if self.soft_index is None else np.ascontiguousarray(self.soft_index, np.uint32),
+            None if self.soft_target is None else np.ascontiguousarray(self.soft_target,
+                                                                         np.float64),
         ]
         k = self._keep
         d = SceneDesc()
@@ -210,6 +222,11 @@ class Scene:
         d.precond = self.precond
         d.mas_precision = self.mas_precision
         d.solver_precision = self.solver_precision
+        d.n_soft = 0 if self.soft_index is None else int(np.asarray(self.soft_index).size)
+        d.soft_index = _ptr(k[15], C.c_uint32)
+        d.soft_target = _ptr(k[16], C.c_double)
+        d.soft_stiffness = self.soft_stiffness
+        d.motion_rate = self.motion_rate
         return d
 
 
@@ -386,6 +403,26 @@ def assemble_bodies(bodies, cloths=(), **kw):
 # --------------------------------------------------------------------------
 # the benchmark scenes (sizes of the metric's configurations)
 # --------------------------------------------------------------------------
+
+def with_soft_targets(sc, index, targets=None, stiffness=1e3, motion_rate=1.0):
+    """A copy of the scene whose vertices `index` are animated soft targets
+    (reference scene JSON "animation" constraints, settings motion_stiffness /
+    motion_rate), pulled towards `targets` (default: their positions)."""
+    idx = np.ascontiguousarray(index, np.uint32)
+    t = sc.x[idx].copy() if targets is None else np.array(targets, np.float64).reshape(-1, 3)
+    return replace(sc, soft_index=idx, soft_target=t, soft_stiffness=float(stiffness),
+                   motion_rate=float(motion_rate))
+
+
+def rotate_axis_angle(p, origin, axis, angle):
+    """Rotate points p about the axis through origin by angle (Rodrigues; the
+    frame update of animated targets, main.cu rotate_axis_angle)."""
+    k = np.asarray(axis, np.float64)
+    k = k / np.linalg.norm(k)
+    q = np.asarray(p, np.float64) - origin
+    c, s = np.cos(angle), np.sin(angle)
+    return origin + q * c + np.cross(k, q) * s + np.outer(q @ k, k) * (1 - c)
+
 
 def bar_scene(cells=(16, 4, 4), h=0.05, drop=0.1, E=1e5, nu=0.49, rho=1000.0, **kw):
     """One tet bar of 16x4x4 hex cells (h = 0.05 m, 425 vertices) dropped from
