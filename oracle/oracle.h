@@ -116,6 +116,10 @@ typedef struct orc_bsr {
     int* rowptr; /* nb+1 */
     int* cols;   /* nnzb */
     double* vals;/* 9*nnzb row-major blocks */
+    /* the last 4 nbody block rows are affine bodies (p, a1, a2, a3): the
+     * block-Jacobi preconditioner inverts each body's 12x12 diagonal block
+     * (ABDPreconditioner) */
+    int nbody;
 } orc_bsr;
 
 /* merge upper triplets (row <= col) into a full symmetric BSR
@@ -138,6 +142,8 @@ void     orc_mas_level_sizes(const orc_mas* m, int32_t* sizes);
 int      orc_mas_total_clusters(const orc_mas* m);
 void     orc_mas_chain(const orc_mas* m, int32_t* chain);
 
+/* block-Jacobi preconditioner: 3x3 node blocks, 12x12 body blocks */
+void orc_block_jacobi(const orc_bsr* A, const double* r, double* z);
 /* PCG (pcg_solver.cu:171-304).  precond: 0 block-Jacobi, 1 MAS, -1 none */
 int orc_pcg(const orc_bsr* A, const orc_mas* mas, int precond,
             const double* b, double* x, double rel_tol, double abs_tol,

@@ -73,6 +73,7 @@ int orc_bsr_from_upper_triplets(int nb, int ntrip, const int* rows, const int* c
         if(i == 0 || e[i].key != e[i - 1].key)
             nu++;
     out->nb     = nb;
+    out->nbody  = 0;
     out->nnzb   = nu;
     out->rowptr = (int*)calloc(nb + 1, sizeof(int));
     out->cols   = (int*)malloc(sizeof(int) * (nu ? nu : 1));
@@ -432,9 +433,41 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
 }
 
 /* ------------------------------------------------------------------ PCG */
+/* DiagPreconditioner (diag_preconditioner.cu:9-66) on the nodes before the
+ * affine bodies, then ABDPreconditioner (abd_preconditioner.cu:15-38,
+ * setup_abd_system_gradient_and_hessian.cu:603-646) on each body's 4 nodes:
+ * the inverse of its 12x12 diagonal block (a singular block: identity),
+ * applied row by row in index order */
 static void block_jacobi(const orc_bsr* A, const double* r, double* z)
 {
-    for(int i = 0; i < A->nb; i++)
+    int nb3 = A->nb - 4 * A->nbody;
+    for(int b = 0; b < A->nbody; b++)
+    {
+        int    n0 = nb3 + 4 * b;
+        double M[144];
+        for(int m = 0; m < 4; m++)
+            for(int n = 0; n < 4; n++)
+            {
+                const double* blk = NULL;
+                for(int k = A->rowptr[n0 + m]; k < A->rowptr[n0 + m + 1]; k++)
+                    if(A->cols[k] == n0 + n)
+                        blk = A->vals + 9 * k;
+                for(int i = 0; i < 3; i++)
+                    for(int j = 0; j < 3; j++)
+                        M[(3 * m + i) * 12 + 3 * n + j] = blk ? blk[3 * i + j] : 0.0;
+            }
+        if(orc_inverse(12, M) != 0)
+            for(int i = 0; i < 144; i++)
+                M[i] = (i % 13 == 0) ? 1.0 : 0.0;
+        for(int i = 0; i < 12; i++)
+        {
+            double zi = 0.0;
+            for(int k = 0; k < 12; k++)
+                zi += M[i * 12 + k] * r[3 * n0 + k];
+            z[3 * n0 + i] = zi;
+        }
+    }
+    for(int i = 0; i < nb3; i++)
     {
         double D[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         for(int k = A->rowptr[i]; k < A->rowptr[i + 1]; k++)
@@ -446,6 +479,8 @@ static void block_jacobi(const orc_bsr* A, const double* r, double* z)
                            + D[a * 3 + 2] * r[3 * i + 2];
     }
 }
+
+void orc_block_jacobi(const orc_bsr* A, const double* r, double* z) { block_jacobi(A, r, z); }
 
 static void precond(const orc_bsr* A, const orc_mas* mas, int pc, const double* r, double* z)
 {

@@ -1336,7 +1336,10 @@ int orc_world_assemble(orc_world* w, double* grad, orc_bsr* H)
     reduce_to_nodes(w, H ? &T : NULL, grad, &Y);
     int rc = 0;
     if(H)
-        rc = orc_bsr_from_upper_triplets(w->nn, Y.n, Y.r, Y.c, Y.v, H);
+    {
+        rc       = orc_bsr_from_upper_triplets(w->nn, Y.n, Y.r, Y.c, Y.v, H);
+        H->nbody = w->nabd;
+    }
     free(Y.r);
     free(Y.c);
     free(Y.v);

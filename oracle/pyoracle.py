@@ -29,7 +29,7 @@ def build():
 
 class Bsr(C.Structure):
     _fields_ = [("nb", C.c_int), ("nnzb", C.c_int), ("rowptr", C.POINTER(C.c_int)),
-                ("cols", C.POINTER(C.c_int)), ("vals", dp)]
+                ("cols", C.POINTER(C.c_int)), ("vals", dp), ("nbody", C.c_int)]
 
 
 def lib():
@@ -89,6 +89,7 @@ def lib():
         L.orc_mas_level_sizes.argtypes = [C.c_void_p, ip]
         L.orc_mas_total_clusters.argtypes = [C.c_void_p]
         L.orc_mas_chain.argtypes = [C.c_void_p, ip]
+        L.orc_block_jacobi.argtypes = [C.POINTER(Bsr), dp, dp]
         L.orc_pcg.argtypes = [C.POINTER(Bsr), C.c_void_p, C.c_int, dp, dp, C.c_double,
                               C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_world_create.restype = C.c_void_p
@@ -252,6 +253,7 @@ class OBsr:
         self.cols = np.ctypeslib.as_array(raw.cols, (max(raw.nnzb, 1),))[: raw.nnzb].copy()
         self.vals = np.ctypeslib.as_array(raw.vals, (max(raw.nnzb, 1) * 9,))[: 9 * raw.nnzb] \
             .reshape(raw.nnzb, 3, 3).copy()
+        self.nbody = raw.nbody
         lib().orc_bsr_free(C.byref(raw))
 
     def raw(self):
@@ -263,6 +265,7 @@ class OBsr:
         r.rowptr = P(self._keep[0], C.c_int)
         r.cols = P(self._keep[1], C.c_int)
         r.vals = P(self._keep[2])
+        r.nbody = getattr(self, "nbody", 0)
         return r
 
     def dense(self):
@@ -328,6 +331,15 @@ class Mas:
         c = np.zeros(self.nb * max(L - 1, 1), np.int32)
         lib().orc_mas_chain(self.h, P(c, C.c_int32))
         return c.reshape(self.nb, max(L - 1, 1))
+
+
+def block_jacobi(A: OBsr, r):
+    """z = block-Jacobi(A) r: 3x3 node blocks, each affine body (the last
+    4 nbody nodes) by its 12x12 diagonal block"""
+    raw = A.raw()
+    z = np.zeros(3 * A.nb)
+    lib().orc_block_jacobi(C.byref(raw), P(np.ascontiguousarray(r, np.float64)), P(z))
+    return z
 
 
 def pcg(A: OBsr, b, mas: Mas | None = None, precond=1, rel_tol=1e-4, abs_tol=0.0, max_iter=0,

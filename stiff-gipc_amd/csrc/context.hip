@@ -714,6 +714,10 @@ Context::Context(const sgipc_scene_desc& d)
     }
     mas_.prof = &prof_;
     pcg_.prof = &prof_;
+    // block-Jacobi mode: each affine body's 12x12 block as one block
+    // (ABDPreconditioner on the body range after the global 3x3 diagonal)
+    if(nabd_ > 0)
+        pcg_.set_bodies(nfem_, nabd_, bslots_.data());
     bf_.init(nf_, 3, faces_.data());
     be_.init(ne_, 2, edges_.data());
     build_bvh();

@@ -247,6 +247,16 @@ struct PcgVecs
     }
 };
 
+// block-Jacobi of affine bodies (precond_abd.hip): binv[144 b] = inverse of
+// body b's 12x12 diagonal block, gathered from the BSR slots slots[16 b + 4 m + n]
+// of its node pairs (m, n); the update applies it to the nodes first + 4 b ..
+void launch_body_inverse(int nbody, const int* slots, const double* vals, double* binv,
+                         hipStream_t s);
+template <typename V>
+void launch_update_body(int nbody, int first, V* x, V* r, const V* p, const V* Ap,
+                        const double* rz_prev, const double* pap, const V* binv, V* z,
+                        double* rz_out, const int* done, hipStream_t s);
+
 enum class Precond
 {
     Diag = 0,
@@ -263,7 +273,17 @@ class Pcg
     // solve A x = b from x = 0 with the reference's iteration structure
     PcgStats solve(const SysView& S, Precond pc, Mas* mas, const double* b, double* x,
                    const PcgConfig& cfg, hipStream_t s);
-    // block-Jacobi inverse of the diagonal blocks (DiagPreconditioner)
+    // affine bodies: nodes first .. first + 4 nbody - 1 in groups of 4, whose
+    // 12x12 diagonal blocks (BSR slots slots16[16 b + 4 m + n]) the block-Jacobi
+    // preconditioner inverts whole (ABDPreconditioner) instead of by 3x3 node
+    void set_bodies(int first, int nbody, const int* slots16)
+    {
+        body_first_ = first;
+        nbody_      = nbody;
+        body_slots_ = slots16;
+    }
+    // block-Jacobi inverse of the diagonal blocks (DiagPreconditioner, and
+    // the affine bodies' 12x12 blocks)
     void build_diag(const Bsr& A, hipStream_t s);
     void apply_diag(int nb, const double* r, double* z, hipStream_t s);
     // FP64 ||b - A x|| / ||b||
@@ -300,8 +320,18 @@ class Pcg
     double* aux_set(int i) { return red_.data() + (size_t)(2 * kRing + i) * kSlotSet; }
     PcgVecs<double> v64_;
     PcgVecs<float>  v32_;
-    DBuf<double> red_, dinv_;
-    DBuf<float>  dinv32_;
+    DBuf<double> red_, dinv_, binv_;
+    DBuf<float>  dinv32_, binv32_;
+    int          body_first_ = 0, nbody_ = 0;
+    const int*   body_slots_ = nullptr;
+    template <typename V>
+    const V*     binv() const;
+    // z = Dinv r over the 3x3 nodes and the bodies, with the optional update
+    template <typename V>
+    void         apply_block_jacobi(int nb, V* x, V* r, const V* p, const V* Ap,
+                                    const double* rz_prev, const double* pap, V* z,
+                                    double* rz_out, double* zero0, double* zero1,
+                                    const int* done, hipStream_t s);
     DBuf<int>    ctl_;         // [0] stopping check iteration (0: running), [1] breakdown
     DBuf<double> ctl_d_;       // [0] r.z at the last stop test
     int*         h_ctl_ = nullptr;  // pinned: two slots of 4 ints for the stop flag

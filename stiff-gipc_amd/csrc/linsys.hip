@@ -820,17 +820,26 @@ void Pcg::build_diag(const Bsr& A, hipStream_t s)
     hipLaunchKernelGGL(k_diag_inverse, dim3(grid_for(A.nb, 256)), dim3(256), 0, s, A.nb,
                        A.diag.data(), A.vals.data(), dinv_.data());
     SG_LAUNCH_CHECK();
+    if(nbody_ > 0)
+    {
+        SG_CHECK(body_first_ + 4 * nbody_ == A.nb, -1, "affine-body nodes must end the system");
+        binv_.resize((size_t)144 * nbody_);
+        launch_body_inverse(nbody_, body_slots_, A.vals.data(), binv_.data(), s);
+    }
 }
 
-void Pcg::apply_diag(int nb, const double* r, double* z, hipStream_t s)
+template <>
+const double* Pcg::binv<double>() const
 {
-    hipLaunchKernelGGL(k_update_diag<double>, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
-                       (double*)nullptr, const_cast<double*>(r), (const double*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                       dinv_.data(), z, (double*)nullptr, (double*)nullptr, (double*)nullptr,
-                       (const int*)nullptr);
-    SG_LAUNCH_CHECK();
+    return binv_.data();
 }
+template <>
+const float* Pcg::binv<float>() const
+{
+    return binv32_.data();
+}
+
+
 
 Pcg::~Pcg()
 {
@@ -860,6 +869,34 @@ template <>
 const float* Pcg::dinv<float>() const
 {
     return dinv32_.data();
+}
+
+// the global 3x3 block Jacobi on the nodes before the bodies, then the
+// bodies' 12x12 blocks (GlobalLinearSystem::apply_preconditioner applies the
+// global preconditioner and then the ABD local one over the body range,
+// global_linear_system.cu:509-522)
+template <typename V>
+void Pcg::apply_block_jacobi(int nb, V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
+                             const double* pap, V* z, double* rz_out, double* zero0,
+                             double* zero1, const int* done, hipStream_t s)
+{
+    const int n3 = nbody_ > 0 ? body_first_ : nb;
+    if(n3 > 0 || zero0 || zero1)
+    {
+        hipLaunchKernelGGL(k_update_diag<V>, dim3(grid_for(n3 > 0 ? n3 : 1, 256)), dim3(256), 0,
+                           s, n3, x, r, p, Ap, rz_prev, pap, dinv<V>(), z, rz_out, zero0, zero1,
+                           done);
+        SG_LAUNCH_CHECK();
+    }
+    if(nbody_ > 0)
+        launch_update_body<V>(nbody_, body_first_, x, r, p, Ap, rz_prev, pap, binv<V>(), z,
+                              rz_out, done, s);
+}
+
+void Pcg::apply_diag(int nb, const double* r, double* z, hipStream_t s)
+{
+    apply_block_jacobi<double>(nb, (double*)nullptr, const_cast<double*>(r), nullptr, nullptr,
+                               nullptr, nullptr, z, nullptr, nullptr, nullptr, nullptr, s);
 }
 
 size_t Pcg::workspace_doubles(int n)
@@ -955,6 +992,11 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         {
             dinv32_.resize(dinv_.size());
             convert<double, float>((long long)dinv_.size(), dinv_.data(), dinv32_.data(), s);
+            if(nbody_ > 0)
+            {
+                binv32_.resize(binv_.size());
+                convert<double, float>((long long)binv_.size(), binv_.data(), binv32_.data(), s);
+            }
         }
     }
     const int* done = ctl_.data();
@@ -969,12 +1011,8 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
             mas->apply_fused(axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(), rz_prev, pap,
                              W.z.data(), rz_set(k), zero0, zero1, dn, s);
         else
-        {
-            hipLaunchKernelGGL(k_update_diag<V>, dim3(grid_for(nb, 256)), dim3(256), 0, s, nb,
-                               axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(), rz_prev, pap,
-                               dinv<V>(), W.z.data(), rz_set(k), zero0, zero1, dn);
-            SG_LAUNCH_CHECK();
-        }
+            apply_block_jacobi<V>(nb, axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(),
+                                  rz_prev, pap, W.z.data(), rz_set(k), zero0, zero1, dn, s);
     };
 
     const bool reliable = (b64 != nullptr);  // FP64 residual replacement (mixed mode)

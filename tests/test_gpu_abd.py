@@ -67,9 +67,15 @@ def test_abd_assembly_and_energy(oracle):
     assert e[0] == pytest.approx(eo[0], rel=1e-10)
 
 
-def test_abd_newton_iterations_match(oracle):
+@pytest.mark.parametrize("precond", ["mas", "diag"])
+def test_abd_newton_iterations_match(oracle, precond):
+    """Newton iterates within 1e-8 with equal PCG counts, under MAS and under
+    block-Jacobi (3x3 FEM node blocks, each body's 12x12 block whole:
+    ABDPreconditioner)"""
     import sgipc
     sc = _scene(n_abd=2, gap=4e-4)
+    if precond == "diag":
+        sc.precond = sgipc.SGIPC_PRECOND_DIAG
     sim = sgipc.Simulation(sc)
     w = oracle.World(sc)
     sim.kappa = 1e4
@@ -79,7 +85,7 @@ def test_abd_newton_iterations_match(oracle):
         so = w.newton_iteration()
         xg, xo = sim.positions(), w.positions()
         err = np.abs(xg - xo).max() / np.abs(xo).max()
-        print(f"[abd-newton] it={it} pcg={st['pcg_iterations']}/{so['pcg_iterations']} "
+        print(f"[abd-newton] {precond} it={it} pcg={st['pcg_iterations']}/{so['pcg_iterations']} "
               f"alpha={st['alpha']:.6g}/{so['alpha']:.6g} err={err:.2e}", flush=True)
         assert st["converged"] == so["converged"]
         if st["converged"]:

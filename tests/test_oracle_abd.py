@@ -178,3 +178,35 @@ def test_abd_stack_on_soft_box_rests(oracle):
     assert x[abd == 0, 1].min() > x[fem, 1].max() - 1e-3   # resting on the soft box
     assert x[abd == 1, 1].min() > x[abd == 0, 1].max()
     assert x[:, 1].min() > 0.0
+
+
+def test_block_jacobi_inverts_body_blocks_whole(oracle):
+    """Block-Jacobi on an affine-body system (DiagPreconditioner on the FEM
+    nodes, ABDPreconditioner on each body, abd_preconditioner.cu:15-38): z on a
+    body's 4 nodes = inverse of its 12x12 diagonal block times r (numpy), on
+    FEM nodes the 3x3 inverse; and it is a better preconditioner for the body
+    than its 3x3 node blocks would be"""
+    from sgipc import scene as S
+    sc = S.abd_scene(n_abd=2, gap=4e-4)
+    sc.kappa = 1e4
+    w = oracle.World(sc)
+    g, A = w.assemble()
+    assert A.nbody == 2
+    D = A.dense()
+    rng = np.random.default_rng(0)
+    r = rng.normal(size=3 * A.nb)
+    z = oracle.block_jacobi(A, r)
+    nb3 = A.nb - 8
+    for i in range(nb3):
+        blk = D[3 * i:3 * i + 3, 3 * i:3 * i + 3]
+        np.testing.assert_allclose(z[3 * i:3 * i + 3], np.linalg.solve(blk, r[3 * i:3 * i + 3]),
+                                   rtol=1e-10, atol=1e-14 * np.abs(z).max())
+    for b in range(2):
+        s = slice(3 * (nb3 + 4 * b), 3 * (nb3 + 4 * b + 4))
+        blk = D[s, s]
+        np.testing.assert_allclose(z[s], np.linalg.solve(blk, r[s]), rtol=1e-8,
+                                   atol=1e-12 * np.abs(z[s]).max())
+    x12, it12, _, _ = oracle.pcg(A, g, None, 0, rel_tol=1e-6)
+    A.nbody = 0  # the same system preconditioned by 3x3 node blocks only
+    x3, it3, _, _ = oracle.pcg(A, g, None, 0, rel_tol=1e-6)
+    assert it12 < it3
