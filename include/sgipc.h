@@ -86,6 +86,36 @@ int sgipc_ls_precond_apply(sgipc_ls_t ls, const double* r, double* z);
 /* solve A x = b from x0 = 0 */
 int sgipc_ls_pcg(sgipc_ls_t ls, const double* b, double* x,
                  const sgipc_pcg_cfg* cfg, sgipc_pcg_stats* stats);
+/* Create from the reference's merged upper BCOO (GIPCTripletMatrix after
+ * Converter::convert, global_matrix.h:18-203, converter.cu:23-189): nnz
+ * blocks (rows[k] <= cols[k], duplicates summed), values row-major 3x3 (the
+ * reference stores column-major Eigen blocks: transpose them), every diagonal
+ * block present.  Expanded to the full BSR of sgipc_ls_create. */
+int sgipc_ls_create_bcoo(int32_t nb, int32_t nnz, const int32_t* rows,
+                         const int32_t* cols, const double* vals,
+                         const int32_t* part_id, int32_t precond,
+                         int32_t mas_precision, sgipc_ls_t* out);
+/* Solve-only mode (main.cu:387-646, :1047-1087): create from a MatrixMarket A
+ * (coordinate real; general -- blocks below the diagonal dropped -- or
+ * symmetric -- mirrored; square, rows divisible by 3, every diagonal 3x3
+ * block present), read as to_upper_triangle_block_triplets does. */
+int sgipc_ls_create_mm(const char* A_path, const int32_t* part_id, int32_t precond,
+                       int32_t mas_precision, sgipc_ls_t* out);
+/* the unique upper blocks a MatrixMarket A describes, read as
+ * sgipc_ls_create_mm does (host only): *nb block rows, *nnz blocks; the first
+ * min(nnz, cap) (row, col) pairs in (row, col) order and their row-major
+ * values (9 doubles each) into rows/cols/vals (NULL with cap 0 to query) */
+int sgipc_mm_read_upper_bcoo(const char* A_path, int32_t* nb, int32_t* nnz, int32_t* rows,
+                             int32_t* cols, double* vals, int32_t cap);
+/* block rows of a system */
+int sgipc_ls_block_rows(sgipc_ls_t ls, int32_t* nb);
+/* MatrixMarket array real M x 1 vector (read_mm_array_vector_mx1): *m = M;
+ * the first min(M, cap) values into b (b may be NULL with cap 0) */
+int sgipc_mm_read_vector(const char* path, double* b, int32_t cap, int32_t* m);
+/* work of the system on the caller's stream from now on (hipStream_t;
+ * NULL returns to the system's own stream) */
+int sgipc_ls_set_stream(sgipc_ls_t ls, void* stream);
+
 /* sgipc_ls_pcg without allocating.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  The same solve (FP64 PCG
@@ -177,6 +207,26 @@ int sgipc_energy(sgipc_t ctx, double* terms);
  * (0 before the first time step, 1 after a completed one). */
 int sgipc_set_soft_targets(sgipc_t ctx, const double* targets);
 int sgipc_get_animation_rate(sgipc_t ctx, double* rate);
+
+/* DOF order of the system at this boundary (sgipc_assemble, sgipc_mm_export).
+ * The library solves with FEM vertices first; the reference's
+ * GlobalLinearSystem lays out [ABD bodies x 4 nodes | FEM vertices]
+ * (global_linear_system.cu:263-336, subsystems created ABD first,
+ * gipc/gipc.cu:42-47). */
+#define SGIPC_DOF_FEM_FIRST 0
+#define SGIPC_DOF_ABD_FIRST 1
+int sgipc_set_dof_order(sgipc_t ctx, int32_t order);
+/* MatrixMarket export of the Newton system at the current positions
+ * (GlobalLinearSystem::export_matrix_market_files, global_linear_system.cu:
+ * 98-261): <dir>/linear_system/frame_<frame, 5 digits>_ns_<newton_step>_A.mtx
+ * (coordinate real general: every upper 3x3 block, then its transpose) and
+ * ..._b.mtx (array real general, M x 1, the right-hand side of
+ * sgipc_assemble), values %.17g, in the DOF order set above. */
+int sgipc_mm_export(sgipc_t ctx, const char* dir, int32_t frame, int32_t newton_step);
+/* work of the context on the caller's stream from now on (hipStream_t;
+ * NULL returns to the context's own stream).  The reference runs on the
+ * per-thread default stream. */
+int sgipc_set_stream(sgipc_t ctx, void* stream);
 
 /* Step-size bounds of a vertex direction dx [3 n_verts] at the current
  * positions (x_new = x - alpha dx; solve_subIP, GIPC.cu:10786-10822):

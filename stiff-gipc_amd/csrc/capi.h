@@ -42,7 +42,8 @@ int guard(F&& f)
 
 struct sgipc_linear_system
 {
-    hipStream_t          s       = nullptr;
+    hipStream_t          s       = nullptr;  // stream of the system's work
+    hipStream_t          own     = nullptr;  // the stream it created
     int                  dev     = 0;  // device current at creation
     sgipc::Bsr           A;
     sgipc::Mas           mas;
@@ -52,7 +53,7 @@ struct sgipc_linear_system
     sgipc::DBuf<double>  b, x, y;
     ~sgipc_linear_system()
     {
-        if(s)
-            (void)hipStreamDestroy(s);
+        if(own)
+            (void)hipStreamDestroy(own);
     }
 };

@@ -7,6 +7,7 @@
 #include "../../include/sgipc.h"
 #include "capi.h"
 #include "context.h"
+#include "mm_io.h"
 
 using namespace sgipc;
 
@@ -55,6 +56,7 @@ int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr, const int32
         {
             SG_HIP(hipGetDevice(&ls->dev));
             SG_HIP(hipStreamCreateWithFlags(&ls->s, hipStreamNonBlocking));
+            ls->own = ls->s;
             std::vector<int> rp(rowptr, rowptr + nb + 1), cl(cols, cols + nnzb);
             ls->A.set_pattern(nb, rp, cl, ls->s);
             ls->A.vals.upload(vals, (size_t)9 * nnzb, ls->s);
@@ -77,6 +79,88 @@ int sgipc_ls_create(int32_t nb, int32_t nnzb, const int32_t* rowptr, const int32
             throw;
         }
         *out = ls;
+    });
+}
+
+int sgipc_ls_create_bcoo(int32_t nb, int32_t nnz, const int32_t* rows, const int32_t* cols,
+                         const double* vals, const int32_t* part_id, int32_t precond,
+                         int32_t mas_precision, sgipc_ls_t* out)
+{
+    std::vector<int>    rp, cl;
+    std::vector<double> vl;
+    int rc = guard([&] {
+        SG_CHECK(rows && cols && vals && nb > 0 && nnz > 0, SGIPC_EINVAL,
+                 "sgipc_ls_create_bcoo: null argument or empty matrix");
+        upper_bcoo_to_full_bsr(nb, nnz, rows, cols, vals, rp, cl, vl);
+    });
+    if(rc != SGIPC_OK)
+        return rc;
+    return sgipc_ls_create(nb, (int32_t)cl.size(), rp.data(), cl.data(), vl.data(), part_id,
+                           precond, mas_precision, out);
+}
+
+int sgipc_ls_create_mm(const char* A_path, const int32_t* part_id, int32_t precond,
+                       int32_t mas_precision, sgipc_ls_t* out)
+{
+    int                 nb = 0;
+    std::vector<int>    br, bc;
+    std::vector<double> bv;
+    int rc = guard([&] {
+        SG_CHECK(A_path, SGIPC_EINVAL, "null path");
+        mm_read_upper_blocks(A_path, &nb, br, bc, bv);
+    });
+    if(rc != SGIPC_OK)
+        return rc;
+    return sgipc_ls_create_bcoo(nb, (int32_t)br.size(), br.data(), bc.data(), bv.data(), part_id,
+                                precond, mas_precision, out);
+}
+
+int sgipc_mm_read_upper_bcoo(const char* A_path, int32_t* nb, int32_t* nnz, int32_t* rows,
+                             int32_t* cols, double* vals, int32_t cap)
+{
+    return guard([&] {
+        SG_CHECK(A_path && nb && nnz, SGIPC_EINVAL, "null argument");
+        int                 n = 0;
+        std::vector<int>    br, bc;
+        std::vector<double> bv;
+        mm_read_upper_blocks(A_path, &n, br, bc, bv);
+        *nb       = n;
+        *nnz      = (int32_t)br.size();
+        size_t m  = std::min<size_t>(br.size(), cap > 0 ? (size_t)cap : 0);
+        if(rows)
+            std::copy(br.begin(), br.begin() + m, rows);
+        if(cols)
+            std::copy(bc.begin(), bc.begin() + m, cols);
+        if(vals)
+            std::copy(bv.begin(), bv.begin() + 9 * m, vals);
+    });
+}
+
+int sgipc_ls_block_rows(sgipc_ls_t ls, int32_t* nb)
+{
+    return guard([&] {
+        SG_CHECK(ls && nb, SGIPC_EINVAL, "null argument");
+        *nb = ls->A.nb;
+    });
+}
+
+int sgipc_mm_read_vector(const char* path, double* b, int32_t cap, int32_t* m)
+{
+    return guard([&] {
+        SG_CHECK(path && m, SGIPC_EINVAL, "null argument");
+        std::vector<double> v = mm_read_vector(path);
+        *m                    = (int32_t)v.size();
+        if(b && cap > 0)
+            std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), (size_t)cap), b);
+    });
+}
+
+int sgipc_ls_set_stream(sgipc_ls_t ls, void* stream)
+{
+    return guard([&] {
+        SG_CHECK(ls, SGIPC_EINVAL, "null system");
+        SG_HIP(hipStreamSynchronize(ls->s));
+        ls->s = stream ? static_cast<hipStream_t>(stream) : ls->own;
     });
 }
 
@@ -346,6 +430,28 @@ int sgipc_get_animation_rate(sgipc_t ctx, double* rate)
         SG_CHECK(rate, SGIPC_EINVAL, "null argument");
         *rate = ctx_of(ctx).animation_rate();
     });
+}
+
+int sgipc_set_dof_order(sgipc_t ctx, int32_t order)
+{
+    return guard([&] {
+        SG_CHECK(order == SGIPC_DOF_FEM_FIRST || order == SGIPC_DOF_ABD_FIRST, SGIPC_EINVAL,
+                 "unknown DOF order");
+        ctx_of(ctx).set_dof_order(order);
+    });
+}
+
+int sgipc_mm_export(sgipc_t ctx, const char* dir, int32_t frame, int32_t newton_step)
+{
+    return guard([&] {
+        SG_CHECK(dir, SGIPC_EINVAL, "null directory");
+        ctx_of(ctx).mm_export(dir, frame, newton_step);
+    });
+}
+
+int sgipc_set_stream(sgipc_t ctx, void* stream)
+{
+    return guard([&] { ctx_of(ctx).set_stream(static_cast<hipStream_t>(stream)); });
 }
 
 int sgipc_ccd_bound(sgipc_t h, const double* dx, double* out)

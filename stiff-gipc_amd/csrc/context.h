@@ -28,6 +28,11 @@ class Context
     void   set_kappa(double k) { kappa_ = k; }
     // new positions of the animated soft targets ([3 n_soft])
     void   set_soft_targets(const double* t);
+    // boundary DOF order (SGIPC_DOF_*) of assemble_out and mm_export
+    void   set_dof_order(int order) { dof_order_ = order; }
+    void   mm_export(const std::string& dir, int frame, int newton_step);
+    // the caller's stream (nullptr: the context's own)
+    void   set_stream(hipStream_t s);
     double animation_rate() const { return dev_.soft_rate; }
 
     // hot-path pieces exposed through the C ABI
@@ -92,7 +97,11 @@ class Context
     void   body_apply(int b, const double* q, double* f) const;
     unsigned read_u(const unsigned* p);
 
-    hipStream_t s_ = nullptr;
+    hipStream_t s_ = nullptr, own_s_ = nullptr;
+    int         dof_order_ = 0;
+    // full BSR of the assembled system in the boundary DOF order
+    void        gather_system(std::vector<double>& grad, std::vector<int>& rp,
+                              std::vector<int>& cl, std::vector<double>& vl);
     sgipc_scene_desc desc_{};
     int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0, ntri_ = 0, nh_ = 0;
     int nabd_ = 0, nfem_ = 0, nn_ = 0;  // affine bodies, FEM vertices, system nodes

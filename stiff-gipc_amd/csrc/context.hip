@@ -19,6 +19,7 @@
 
 #include "cloth.h"
 #include "context.h"
+#include "mm_io.h"
 
 namespace sgipc
 {
@@ -96,6 +97,7 @@ Context::Context(const sgipc_scene_desc& d)
     for(int e = 0; e < ns_; e++)
         SG_CHECK(d.surf_verts[e] < (uint32_t)nv_, SGIPC_EINVAL, "surface vertex out of range");
     SG_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    own_s_ = s_;
     dHat_  = d.dhat * d.dhat;
     kappa_ = d.kappa > 0 ? d.kappa : 0.0;
 
@@ -764,8 +766,14 @@ void Context::body_apply(int b, const double* q, double* f) const
 
 Context::~Context()
 {
-    if(s_)
-        (void)hipStreamDestroy(s_);
+    if(own_s_)
+        (void)hipStreamDestroy(own_s_);
+}
+
+void Context::set_stream(hipStream_t s)
+{
+    SG_HIP(hipStreamSynchronize(s_));
+    s_ = s ? s : own_s_;
 }
 
 double Context::read_d(const double* p)
@@ -1621,13 +1629,17 @@ void Context::contact_query(int32_t* pairs, int cap, int32_t* counts, uint32_t* 
     SG_HIP(hipStreamSynchronize(s_));
 }
 
-void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols,
-                           double* vals, int cap)
+// the assembled system on the host: right-hand side and full BSR (static
+// pattern merged with the contact blocks), in the boundary DOF order.  With
+// SGIPC_DOF_ABD_FIRST the body nodes come first (external node of internal
+// node i: i - nfem for a body node, i + 4 nabd for a FEM vertex).
+void Context::gather_system(std::vector<double>& grad, std::vector<int>& rp_out,
+                            std::vector<int>& cl_out, std::vector<double>& vl_out)
 {
     assemble(true);
     check_overflow();
-    if(grad)
-        rhs_.download(grad, 3 * (size_t)nn_, s_);
+    grad.resize(3 * (size_t)nn_);
+    rhs_.download(grad.data(), 3 * (size_t)nn_, s_);
     std::vector<double> v = A_.vals.to_host(s_);
     unsigned            nc = read_u(coo_.count.data());
     std::vector<int>    cr(nc), cc(nc);
@@ -1639,27 +1651,28 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
         coo_.v.download(cv.data(), 9 * (size_t)nc, s_);
     }
     SG_HIP(hipStreamSynchronize(s_));
-    // merge static pattern and COO tail row by row
+    const bool abd_first = dof_order_ == SGIPC_DOF_ABD_FIRST && nabd_ > 0;
+    auto ext = [&](int i) { return !abd_first ? i : (i >= nfem_ ? i - nfem_ : i + 4 * nabd_); };
+    // merge static pattern and COO tail row by row (rows in boundary order)
     std::vector<std::vector<std::pair<int, const double*>>> extra(nn_);
     for(unsigned k = 0; k < nc; k++)
         extra[cr[k]].push_back({cc[k], cv.data() + 9 * (size_t)k});
-    std::vector<int>    rp(nn_ + 1, 0), cl;
-    std::vector<double> vl;
+    std::vector<std::vector<std::pair<int, std::array<double, 9>>>> rows(nn_);
     for(int i = 0; i < nn_; i++)
     {
-        std::vector<std::pair<int, std::array<double, 9>>> row;
+        auto& row = rows[ext(i)];
         for(int k = A_.h_rowptr[i]; k < A_.h_rowptr[i + 1]; k++)
         {
             std::array<double, 9> b;
             for(int e = 0; e < 9; e++)
                 b[e] = v[9 * (size_t)k + e];
-            row.push_back({A_.h_cols[k], b});
+            row.push_back({ext(A_.h_cols[k]), b});
         }
         for(auto& ex : extra[i])
         {
             bool found = false;
             for(auto& r : row)
-                if(r.first == ex.first)
+                if(r.first == ext(ex.first))
                 {
                     for(int e = 0; e < 9; e++)
                         r.second[e] += ex.second[e];
@@ -1671,17 +1684,42 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
                 std::array<double, 9> b;
                 for(int e = 0; e < 9; e++)
                     b[e] = ex.second[e];
-                row.push_back({ex.first, b});
+                row.push_back({ext(ex.first), b});
             }
         }
+    }
+    rp_out.assign(nn_ + 1, 0);
+    cl_out.clear();
+    vl_out.clear();
+    for(int i = 0; i < nn_; i++)
+    {
+        auto& row = rows[i];
         std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
         for(auto& r : row)
         {
-            cl.push_back(r.first);
-            vl.insert(vl.end(), r.second.begin(), r.second.end());
+            cl_out.push_back(r.first);
+            vl_out.insert(vl_out.end(), r.second.begin(), r.second.end());
         }
-        rp[i + 1] = (int)cl.size();
+        rp_out[i + 1] = (int)cl_out.size();
     }
+    if(abd_first)
+    {
+        std::vector<double> g(grad.size());
+        for(int i = 0; i < nn_; i++)
+            for(int c = 0; c < 3; c++)
+                g[3 * (size_t)ext(i) + c] = grad[3 * (size_t)i + c];
+        grad.swap(g);
+    }
+}
+
+void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t* cols,
+                           double* vals, int cap)
+{
+    std::vector<double> g, vl;
+    std::vector<int>    rp, cl;
+    gather_system(g, rp, cl, vl);
+    if(grad)
+        std::copy(g.begin(), g.end(), grad);
     if(nnzb)
         *nnzb = (int32_t)cl.size();
     if(cap <= 0)
@@ -1693,6 +1731,16 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
         std::copy(cl.begin(), cl.end(), cols);
     if(vals)
         std::copy(vl.begin(), vl.end(), vals);
+}
+
+void Context::mm_export(const std::string& dir, int frame, int newton_step)
+{
+    std::vector<double> g, vl;
+    std::vector<int>    rp, cl;
+    gather_system(g, rp, cl, vl);
+    std::string b_path;
+    std::string A_path = mm_system_paths(dir, frame, newton_step, &b_path);
+    mm_write_system(A_path, b_path, nn_, rp, cl, vl, g);
 }
 
 }  // namespace sgipc

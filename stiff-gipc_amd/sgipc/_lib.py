@@ -91,7 +91,12 @@ EXPORTS = [
     "sgipc_mlp_train", "sgipc_mlp_loss_grad", "sgipc_mlp_set", "sgipc_mlp_get",
     "sgipc_ccd_bound", "sgipc_ccd_candidates", "sgipc_ls_pcg_noalloc",
     "sgipc_set_soft_targets", "sgipc_get_animation_rate",
+    "sgipc_ls_create_bcoo", "sgipc_ls_create_mm", "sgipc_ls_block_rows", "sgipc_mm_read_vector",
+    "sgipc_mm_read_upper_bcoo", "sgipc_ls_set_stream", "sgipc_set_dof_order", "sgipc_mm_export",
+    "sgipc_set_stream",
 ]
+SGIPC_DOF_FEM_FIRST = 0
+SGIPC_DOF_ABD_FIRST = 1
 MLP_PARAMS, MLP_IN, MLP_OUT = 21260, 24, 12
 
 
@@ -114,6 +119,14 @@ def lib():
         L.sgipc_ls_mas_info.argtypes = [C.c_void_p, ip, ip, ip]
         L.sgipc_ls_mas_chain.argtypes = [C.c_void_p, ip]
         L.sgipc_ls_mas_bank.argtypes = [C.c_void_p, C.c_int32, dp, dp]
+        L.sgipc_ls_create_bcoo.argtypes = [C.c_int32, C.c_int32, ip, ip, dp, ip, C.c_int32,
+                                           C.c_int32, C.POINTER(C.c_void_p)]
+        L.sgipc_ls_create_mm.argtypes = [C.c_char_p, ip, C.c_int32, C.c_int32,
+                                         C.POINTER(C.c_void_p)]
+        L.sgipc_ls_block_rows.argtypes = [C.c_void_p, ip]
+        L.sgipc_mm_read_vector.argtypes = [C.c_char_p, dp, C.c_int32, ip]
+        L.sgipc_mm_read_upper_bcoo.argtypes = [C.c_char_p, ip, ip, ip, ip, dp, C.c_int32]
+        L.sgipc_ls_set_stream.argtypes = [C.c_void_p, C.c_void_p]
         if hasattr(L, "sgipc_create"):
             L.sgipc_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
             L.sgipc_destroy.argtypes = [C.c_void_p]
@@ -129,6 +142,9 @@ def lib():
             L.sgipc_assemble.argtypes = [C.c_void_p, dp, ip, ip, ip, dp, C.c_int32]
             L.sgipc_energy.argtypes = [C.c_void_p, dp]
             L.sgipc_set_soft_targets.argtypes = [C.c_void_p, dp]
+            L.sgipc_set_dof_order.argtypes = [C.c_void_p, C.c_int32]
+            L.sgipc_mm_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]
+            L.sgipc_set_stream.argtypes = [C.c_void_p, C.c_void_p]
             L.sgipc_get_animation_rate.argtypes = [C.c_void_p, dp]
             L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
             L.sgipc_ccd_bound.argtypes = [C.c_void_p, dp, dp]
@@ -168,23 +184,80 @@ def device_info():
     return buf.value.decode()
 
 
+def read_mm_vector(path):
+    """MatrixMarket array real M x 1 (read_mm_array_vector_mx1, main.cu)"""
+    m = C.c_int32()
+    check(lib().sgipc_mm_read_vector(os.fsencode(path), None, 0, C.byref(m)))
+    b = np.zeros(m.value)
+    check(lib().sgipc_mm_read_vector(os.fsencode(path), P(b), m.value, C.byref(m)))
+    return b
+
+
+def read_mm_upper_bcoo(path):
+    """(nb, rows, cols, vals[k,3,3]) of the unique upper blocks a MatrixMarket
+    A describes (to_upper_triangle_block_triplets, main.cu)"""
+    nb, nnz = C.c_int32(), C.c_int32()
+    f = os.fsencode(path)
+    check(lib().sgipc_mm_read_upper_bcoo(f, C.byref(nb), C.byref(nnz), None, None, None, 0))
+    r = np.zeros(nnz.value, np.int32)
+    c = np.zeros(nnz.value, np.int32)
+    v = np.zeros((nnz.value, 3, 3))
+    check(lib().sgipc_mm_read_upper_bcoo(f, C.byref(nb), C.byref(nnz), P(r, C.c_int32),
+                                         P(c, C.c_int32), P(v), nnz.value))
+    return nb.value, r, c, v
+
+
 class LinearSystem:
     """Stand-alone full symmetric BSR system with MAS or block-Jacobi PCG
     (counterpart of GlobalLinearSystem::solve_loaded_system)."""
 
     def __init__(self, rowptr, cols, vals, part_id=None, precond=SGIPC_PRECOND_MAS,
-                 mas_precision=SGIPC_MAS_FP32):
+                 mas_precision=SGIPC_MAS_FP32, _handle=None, _nb=None):
+        self.part = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
+        if _handle is not None:
+            self.h, self.nb = _handle, _nb
+            return
         self.rowptr = np.ascontiguousarray(rowptr, np.int32)
         self.cols = np.ascontiguousarray(cols, np.int32)
         self.vals = np.ascontiguousarray(vals, np.float64).reshape(-1)
         self.nb = self.rowptr.size - 1
-        self.part = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
         h = C.c_void_p()
         check(lib().sgipc_ls_create(self.nb, self.cols.size, P(self.rowptr, C.c_int32),
                                     P(self.cols, C.c_int32), P(self.vals),
                                     None if self.part is None else P(self.part, C.c_int32),
                                     precond, mas_precision, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def _wrap(cls, h, part_id):
+        nb = C.c_int32()
+        check(lib().sgipc_ls_block_rows(h, C.byref(nb)))
+        return cls(None, None, None, part_id, _handle=h, _nb=nb.value)
+
+    @classmethod
+    def from_bcoo(cls, nb, rows, cols, vals, part_id=None, precond=SGIPC_PRECOND_MAS,
+                  mas_precision=SGIPC_MAS_FP32):
+        """from the reference's merged upper BCOO (row <= col, row-major 3x3)"""
+        r = np.ascontiguousarray(rows, np.int32)
+        c = np.ascontiguousarray(cols, np.int32)
+        v = np.ascontiguousarray(vals, np.float64).reshape(-1)
+        pid = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
+        h = C.c_void_p()
+        check(lib().sgipc_ls_create_bcoo(int(nb), r.size, P(r, C.c_int32), P(c, C.c_int32),
+                                         P(v), None if pid is None else P(pid, C.c_int32),
+                                         precond, mas_precision, C.byref(h)))
+        return cls._wrap(h, pid)
+
+    @classmethod
+    def from_matrix_market(cls, A_path, part_id=None, precond=SGIPC_PRECOND_MAS,
+                           mas_precision=SGIPC_MAS_FP32):
+        """solve-only mode: A from a MatrixMarket file (main.cu --A)"""
+        pid = None if part_id is None else np.ascontiguousarray(part_id, np.int32)
+        h = C.c_void_p()
+        check(lib().sgipc_ls_create_mm(os.fsencode(A_path),
+                                       None if pid is None else P(pid, C.c_int32), precond,
+                                       mas_precision, C.byref(h)))
+        return cls._wrap(h, pid)
 
     def close(self):
         if getattr(self, "h", None):
@@ -334,6 +407,19 @@ class Simulation:
         t = np.zeros(9)
         check(lib().sgipc_energy(self.h, P(t)))
         return t
+
+    def set_dof_order(self, order):
+        """SGIPC_DOF_FEM_FIRST (default) or SGIPC_DOF_ABD_FIRST (the reference's
+        GlobalLinearSystem layout) for assemble() and export_matrix_market()"""
+        check(lib().sgipc_set_dof_order(self.h, int(order)))
+
+    def export_matrix_market(self, out_dir, frame=0, newton_step=0):
+        """A and b of the Newton system at the current positions, written as
+        out_dir/linear_system/frame_<frame:05d>_ns_<newton_step>_{A,b}.mtx"""
+        check(lib().sgipc_mm_export(self.h, os.fsencode(str(out_dir)), int(frame),
+                                    int(newton_step)))
+        base = os.path.join(str(out_dir), "linear_system", f"frame_{frame:05d}_ns_{newton_step}")
+        return base + "_A.mtx", base + "_b.mtx"
 
     def set_soft_targets(self, t):
         check(lib().sgipc_set_soft_targets(self.h, P(np.ascontiguousarray(t, np.float64).ravel())))

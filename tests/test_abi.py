@@ -37,6 +37,16 @@ def test_scene_desc_layout_matches_header():
     assert names == fields
 
 
-def test_last_error_is_empty_initially():
+def test_last_error_follows_the_last_call(tmp_path):
+    """a failing call leaves its message, a successful one clears it (host-only
+    entry points, no device)"""
+    import numpy as np
+    import sgipc
     from sgipc import lib
-    assert lib().sgipc_last_error() in (b"",)
+    with __import__("pytest").raises(sgipc.SgipcError):
+        sgipc.read_mm_vector(tmp_path / "missing.mtx")
+    assert b"missing.mtx" in lib().sgipc_last_error()
+    p = tmp_path / "b.mtx"
+    p.write_text("%%MatrixMarket matrix array real general\n2 1\n1.5\n-2\n")
+    assert np.array_equal(sgipc.read_mm_vector(p), [1.5, -2.0])
+    assert lib().sgipc_last_error() == b""
