@@ -223,6 +223,11 @@ int sgipc_set_dof_order(sgipc_t ctx, int32_t order);
  * ..._b.mtx (array real general, M x 1, the right-hand side of
  * sgipc_assemble), values %.17g, in the DOF order set above. */
 int sgipc_mm_export(sgipc_t ctx, const char* dir, int32_t frame, int32_t newton_step);
+/* export the system of every solving Newton iteration of the next time
+ * step as sgipc_mm_export does, with newton_step counting from 0 (the
+ * reference's export_linear_system_frames: enable_matrix_market_export
+ * before IPC_Solver, main.cu:1105-1115, global_linear_system.cu:388-405) */
+int sgipc_mm_export_step(sgipc_t ctx, const char* dir, int32_t frame);
 /* work of the context on the caller's stream from now on (hipStream_t;
  * NULL returns to the context's own stream).  The reference runs on the
  * per-thread default stream. */
@@ -279,6 +284,10 @@ typedef struct sgipc_step_stats {
 /* One time step (IPC_Solver, GIPC.cu:10940-11131): kappa setup, Newton loop,
  * velocity update, next x_tilde. */
 int sgipc_time_step(sgipc_t ctx, sgipc_step_stats* stats);
+/* statistics of every Newton iteration of the last time step (the stats.json
+ * "newton" list, main.cu:1144-1150), the final stop-test pass of each Newton
+ * loop included (converged = 1); *n = count, the first min(n, cap) to out */
+int sgipc_step_log(sgipc_t ctx, sgipc_iter_stats* out, int32_t cap, int32_t* n);
 
 typedef struct sgipc_advance_stats {
     int32_t newton_iterations; /* Newton solves performed (== n requested)  */

@@ -120,6 +120,14 @@ typedef struct sgipc_scene_desc {
     const double*   soft_target;  /* [3*n_soft] (sgipc_set_soft_targets moves them) */
     double          soft_stiffness; /* motion_stiffness (softMotionRate)      */
     double          motion_rate;    /* <= 0 -> 1                               */
+    /* ---- scene extents (GIPC::init / main.cu:853-863) ----
+     * bbox_diag: sqrt(bboxDiagSize2), the diagonal of the largest-volume
+     * object box that scales kappa and the friction epsilon (<= 0: the AABB
+     * of the surface faces); scene_diag: the diagonal of the union box of the
+     * non-obstacle objects that scales abs_xdelta_tol (<= 0: the AABB of the
+     * free vertices) */
+    double          bbox_diag;
+    double          scene_diag;
 } sgipc_scene_desc;
 
 #ifdef __cplusplus

@@ -923,6 +923,11 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     w->scene_diag = any ? sqrt((u2[0] - l2[0]) * (u2[0] - l2[0]) + (u2[1] - l2[1]) * (u2[1] - l2[1])
                                + (u2[2] - l2[2]) * (u2[2] - l2[2]))
                         : 0.0;
+    /* the scene's own extents when given (main.cu:853-863, GIPC::init) */
+    if(s->bbox_diag > 0.0)
+        w->bbox_diag2 = s->bbox_diag * s->bbox_diag;
+    if(s->scene_diag > 0.0)
+        w->scene_diag = s->scene_diag;
     w->kappa = s->kappa > 0 ? s->kappa : 0.0;
     w->cap   = 2 * (s->n_surf_verts + s->n_edges) + 200000;
     w->pairs = (int32_t*)malloc(sizeof(int32_t) * 4 * w->cap);

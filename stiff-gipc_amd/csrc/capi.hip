@@ -449,6 +449,25 @@ int sgipc_mm_export(sgipc_t ctx, const char* dir, int32_t frame, int32_t newton_
     });
 }
 
+int sgipc_mm_export_step(sgipc_t ctx, const char* dir, int32_t frame)
+{
+    return guard([&] {
+        SG_CHECK(dir, SGIPC_EINVAL, "null directory");
+        ctx_of(ctx).mm_export_step(dir, frame);
+    });
+}
+
+int sgipc_step_log(sgipc_t ctx, sgipc_iter_stats* out, int32_t cap, int32_t* n)
+{
+    return guard([&] {
+        SG_CHECK(n, SGIPC_EINVAL, "null argument");
+        const auto& v = ctx_of(ctx).step_log();
+        *n            = (int32_t)v.size();
+        if(out && cap > 0)
+            std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), (size_t)cap), out);
+    });
+}
+
 int sgipc_set_stream(sgipc_t ctx, void* stream)
 {
     return guard([&] { ctx_of(ctx).set_stream(static_cast<hipStream_t>(stream)); });

@@ -31,6 +31,11 @@ class Context
     // boundary DOF order (SGIPC_DOF_*) of assemble_out and mm_export
     void   set_dof_order(int order) { dof_order_ = order; }
     void   mm_export(const std::string& dir, int frame, int newton_step);
+    // export the system of every solving Newton iteration of the next time
+    // step (enable_matrix_market_export before IPC_Solver, main.cu:1105-1115)
+    void   mm_export_step(const std::string& dir, int frame);
+    // per-iteration statistics of the last time step (stats.json "newton")
+    const std::vector<sgipc_iter_stats>& step_log() const { return step_log_; }
     // the caller's stream (nullptr: the context's own)
     void   set_stream(hipStream_t s);
     double animation_rate() const { return dev_.soft_rate; }
@@ -101,7 +106,12 @@ class Context
     int         dof_order_ = 0;
     // full BSR of the assembled system in the boundary DOF order
     void        gather_system(std::vector<double>& grad, std::vector<int>& rp,
-                              std::vector<int>& cl, std::vector<double>& vl);
+                              std::vector<int>& cl, std::vector<double>& vl,
+                              bool reassemble = true);
+    std::vector<sgipc_iter_stats> step_log_;
+    std::string mm_dir_;
+    int         mm_frame_ = -1, mm_ns_ = 0;
+    bool        mm_armed_ = false;
     sgipc_scene_desc desc_{};
     int nv_ = 0, nt_ = 0, nf_ = 0, ne_ = 0, ns_ = 0, ntri_ = 0, nh_ = 0;
     int nabd_ = 0, nfem_ = 0, nn_ = 0;  // affine bodies, FEM vertices, system nodes

@@ -583,6 +583,10 @@ Context::Context(const sgipc_scene_desc& d)
         for(int c = 0; c < 3; c++)
             scene_diag_ += (u2[c] - l2[c]) * (u2[c] - l2[c]);
     scene_diag_ = std::sqrt(scene_diag_);
+    if(d.bbox_diag > 0.0)
+        bbox_diag2_ = d.bbox_diag * d.bbox_diag;
+    if(d.scene_diag > 0.0)
+        scene_diag_ = d.scene_diag;
 
     // ---- device state
     size_t n3 = 3 * (size_t)nv_;
@@ -1201,6 +1205,16 @@ void Context::newton_iteration(sgipc_iter_stats* st)
             *st = loc;
         return;
     }
+    // the system passed to the solver, when an export is armed
+    // (GlobalLinearSystem::solve_linear_system, global_linear_system.cu:388-405)
+    if(mm_armed_)
+    {
+        std::vector<double> g, vl;
+        std::vector<int>    rp, cl;
+        gather_system(g, rp, cl, vl, false);
+        std::string b_path, A_path = mm_system_paths(mm_dir_, mm_frame_, mm_ns_++, &b_path);
+        mm_write_system(A_path, b_path, nn_, rp, cl, vl, g);
+    }
     // ---- linear solve
     SysView S{&A_, &coo_};
     Precond pc = desc_.precond == SGIPC_PRECOND_MAS ? Precond::Mas : Precond::Diag;
@@ -1417,6 +1431,7 @@ void Context::begin_step()
     newton_k_ = 0;
     xdelta0_  = 0;
     in_step_  = true;
+    step_log_.clear();
 }
 
 // IPC_Solver's animation loop (GIPC.cu:11038-11079): the step ends once
@@ -1446,6 +1461,7 @@ void Context::set_soft_targets(const double* t)
 
 void Context::end_step()
 {
+    mm_armed_ = false;
     launch_update_velocity(dev_, xprev_.data(), v_.data(), s_);
     in_step_ = false;
 }
@@ -1467,6 +1483,7 @@ void Context::advance(int n, sgipc_advance_stats* st)
         }
         sgipc_iter_stats is{};
         newton_iteration(&is);
+        step_log_.push_back(is);
         if(is.converged)
         {
             if(next_newton_loop())
@@ -1547,6 +1564,7 @@ void Context::time_step(sgipc_step_stats* st)
         {
             sgipc_iter_stats is{};
             newton_iteration(&is);
+            step_log_.push_back(is);
             if(is.converged)
             {
                 loop_met = true;
@@ -1634,10 +1652,14 @@ void Context::contact_query(int32_t* pairs, int cap, int32_t* counts, uint32_t* 
 // SGIPC_DOF_ABD_FIRST the body nodes come first (external node of internal
 // node i: i - nfem for a body node, i + 4 nabd for a FEM vertex).
 void Context::gather_system(std::vector<double>& grad, std::vector<int>& rp_out,
-                            std::vector<int>& cl_out, std::vector<double>& vl_out)
+                            std::vector<int>& cl_out, std::vector<double>& vl_out,
+                            bool reassemble)
 {
-    assemble(true);
-    check_overflow();
+    if(reassemble)
+    {
+        assemble(true);
+        check_overflow();
+    }
     grad.resize(3 * (size_t)nn_);
     rhs_.download(grad.data(), 3 * (size_t)nn_, s_);
     std::vector<double> v = A_.vals.to_host(s_);
@@ -1731,6 +1753,14 @@ void Context::assemble_out(double* grad, int32_t* nnzb, int32_t* rowptr, int32_t
         std::copy(cl.begin(), cl.end(), cols);
     if(vals)
         std::copy(vl.begin(), vl.end(), vals);
+}
+
+void Context::mm_export_step(const std::string& dir, int frame)
+{
+    mm_dir_   = dir;
+    mm_frame_ = frame;
+    mm_ns_    = 0;
+    mm_armed_ = true;
 }
 
 void Context::mm_export(const std::string& dir, int frame, int newton_step)

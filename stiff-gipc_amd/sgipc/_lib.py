@@ -93,7 +93,7 @@ EXPORTS = [
     "sgipc_set_soft_targets", "sgipc_get_animation_rate",
     "sgipc_ls_create_bcoo", "sgipc_ls_create_mm", "sgipc_ls_block_rows", "sgipc_mm_read_vector",
     "sgipc_mm_read_upper_bcoo", "sgipc_ls_set_stream", "sgipc_set_dof_order", "sgipc_mm_export",
-    "sgipc_set_stream",
+    "sgipc_set_stream", "sgipc_mm_export_step", "sgipc_step_log",
 ]
 SGIPC_DOF_FEM_FIRST = 0
 SGIPC_DOF_ABD_FIRST = 1
@@ -145,6 +145,9 @@ def lib():
             L.sgipc_set_dof_order.argtypes = [C.c_void_p, C.c_int32]
             L.sgipc_mm_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]
             L.sgipc_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+            L.sgipc_mm_export_step.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+            L.sgipc_step_log.argtypes = [C.c_void_p, C.POINTER(IterStats), C.c_int32,
+                                         C.POINTER(C.c_int32)]
             L.sgipc_get_animation_rate.argtypes = [C.c_void_p, dp]
             L.sgipc_newton_iteration.argtypes = [C.c_void_p, C.POINTER(IterStats)]
             L.sgipc_ccd_bound.argtypes = [C.c_void_p, dp, dp]
@@ -420,6 +423,19 @@ class Simulation:
                                     int(newton_step)))
         base = os.path.join(str(out_dir), "linear_system", f"frame_{frame:05d}_ns_{newton_step}")
         return base + "_A.mtx", base + "_b.mtx"
+
+    def export_next_step(self, out_dir, frame):
+        """export the system of every solving Newton iteration of the next
+        time step (frame_<frame>_ns_<k>_{A,b}.mtx under out_dir/linear_system)"""
+        check(lib().sgipc_mm_export_step(self.h, os.fsencode(str(out_dir)), int(frame)))
+
+    def step_log(self):
+        """per-Newton-iteration statistics of the last time step"""
+        n = C.c_int32()
+        check(lib().sgipc_step_log(self.h, None, 0, C.byref(n)))
+        buf = (IterStats * max(n.value, 1))()
+        check(lib().sgipc_step_log(self.h, buf, n.value, C.byref(n)))
+        return [buf[i].as_dict() for i in range(n.value)]
 
     def set_soft_targets(self, t):
         check(lib().sgipc_set_soft_targets(self.h, P(np.ascontiguousarray(t, np.float64).ravel())))

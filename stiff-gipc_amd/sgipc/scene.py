@@ -76,6 +76,8 @@ class SceneDesc(C.Structure):
         ("soft_target", C.POINTER(C.c_double)),
         ("soft_stiffness", C.c_double),
         ("motion_rate", C.c_double),
+        ("bbox_diag", C.c_double),
+        ("scene_diag", C.c_double),
     ]
 
 
@@ -135,6 +137,8 @@ class Scene:
     soft_target: np.ndarray | None = None  # (K,3) float64 their targets
     soft_stiffness: float = 0.0            # motion_stiffness
     motion_rate: float = 1.0               # Newton loops per step (animation_subRate = 1/rate)
+    bbox_diag: float = 0.0                 # kappa/friction scale (0: surface AABB)
+    scene_diag: float = 0.0                # Newton tolerance scale (0: free-vertex AABB)
     name: str = "scene"
     meta: dict = field(default_factory=dict)
 
@@ -227,6 +231,8 @@ class Scene:
         d.soft_target = _ptr(k[16], C.c_double)
         d.soft_stiffness = self.soft_stiffness
         d.motion_rate = self.motion_rate
+        d.bbox_diag = self.bbox_diag
+        d.scene_diag = self.scene_diag
         return d
 
 
