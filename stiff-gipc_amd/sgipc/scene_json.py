@@ -25,7 +25,9 @@ the library takes) plus the run controls the CLI loop needs:
 
 Extensions beyond the reference schema (optional keys):
   objects[].body_type "ABD" makes the object an affine body (its tets give
-  mass only); settings.abd_kappa sets the affine shape stiffness.
+  mass only); settings.abd_kappa sets the affine shape stiffness;
+  simulation.mas_precision "fp64" keeps the MAS inverses and apply in FP64
+  (the reference's MAS applies FP32 inverses: "fp32", the default).
 """
 from __future__ import annotations
 
@@ -264,6 +266,7 @@ def load_scene_json(path):
         pcg_max_iter=int(s["pcg_max_iter"]),
         pcg_use_preconditioned_norm=int(bool(s["pcg_use_preconditioned_norm"])),
         precond=1 if pre != 0 else 0, soft_stiffness=float(s["motion_stiffness"]),
+        mas_precision=1 if sim.get("mas_precision", "fp32") == "fp64" else 0,
         motion_rate=float(s["motion_rate"]), bbox_diag=bbox_diag, scene_diag=scene_diag,
         name=os.path.splitext(os.path.basename(path))[0])
     # the settings the reference reads but the CLI does not use (cloth keys,
