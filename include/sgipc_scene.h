@@ -128,6 +128,18 @@ typedef struct sgipc_scene_desc {
      * free vertices) */
     double          bbox_diag;
     double          scene_diag;
+    /* ---- motor bodies (BodyBoundaryType::Motor, body_boundary_type.h;
+     * ABD gradient/Hessian setup_abd_system_gradient_and_hessian.cu:262-339,
+     * kinetic energy cal_abd_energy.cu:53-104) ----
+     * an affine body whose vertices all have btype 2 is driven to turn about
+     * its body x axis by motor_speed * dt per step: the target q_p rotates the
+     * columns of the step-start prediction A~ (A_p = A~ R_x(motor_speed dt),
+     * rows a2, a3 normalised) and the energy 0.5 dq^T P dq with
+     * dq = (0, 0, a2 - a2_p, a3 - a3_p) and P = motor_strength M[6:12, 6:12]
+     * joins the body's kinetic term (reference Assets/scene/
+     * abd_system_config.json: speed -0.02, strength 200) */
+    double          motor_speed;
+    double          motor_strength;
 } sgipc_scene_desc;
 
 #ifdef __cplusplus

@@ -229,6 +229,8 @@ void   orc_abd_shape_hess(const double q[12], double H[81]);
 void   orc_abd_x(const double xbar[3], const double q[12], double x[3]);
 void   orc_abd_jt(const double xbar[3], const double v[3], double out[12]);
 void   orc_abd_jmat(const double xbar[3], double J[36]);
+/* motor target q_p of a body from its prediction q~ and the angle speed * dt */
+void   orc_abd_motor_target(const double qt[12], double theta, double qp[12]);
 
 /* ---------------- learned warm start network (orc_mlp.c) ---------------- */
 float  orc_bf16_round(float f);

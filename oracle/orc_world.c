@@ -76,6 +76,8 @@ struct orc_world
     int       in_step;
     /* animated soft targets and the animation rate (animation_fullRate,
      * animation_subRate) */
+    int*      bmotor;                         /* motor bodies and targets */
+    double*   bqm;
     int       nsoft;
     uint32_t* soft_idx;
     double*   soft_tgt;
@@ -150,6 +152,57 @@ static void vertices_to_bodies(orc_world* w, const double* v, const double* xt)
         body_fit(w, b, xt, w->bqt + 12 * b);
     }
     bodies_to_vertices(w);
+}
+
+/* motor target of a body from its prediction q~ (the reference's
+ * q_p = J^-1 J_delta q~ + q~ with J, J_delta the affine maps of the points
+ * 0, e_x, e_y, e_z and of their displacement under R_x(theta),
+ * setup_abd_system_gradient_and_hessian.cu:264-307, which is A_p = A~ R_x
+ * on the rows a_i), rows a2 and a3 normalised; p is not driven */
+void orc_abd_motor_target(const double qt[12], double theta, double qp[12])
+{
+    double c = cos(theta), s = sin(theta);
+    for(int k = 0; k < 3; k++)
+        qp[k] = 2.0 * qt[k];
+    for(int i = 0; i < 3; i++)
+    {
+        const double* a = qt + 3 + 3 * i;
+        qp[3 + 3 * i]     = a[0];
+        qp[3 + 3 * i + 1] = c * a[1] + s * a[2];
+        qp[3 + 3 * i + 2] = -s * a[1] + c * a[2];
+    }
+    for(int i = 1; i < 3; i++)
+    {
+        double* a = qp + 3 + 3 * i;
+        double  n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        for(int k = 0; k < 3; k++)
+            a[k] /= n;
+    }
+}
+
+/* motor term of body b: dq = q - q_p on a2, a3 (zero on p, a1) and
+ * P = motor_strength M[6:12, 6:12]; g += P dq, returns 0.5 dq^T P dq */
+static double motor_term(const orc_world* w, int b, double* g, double* H)
+{
+    const double* M = w->bM + 144 * b;
+    double        dq[6], e = 0.0, st = w->s.motor_strength;
+    for(int k = 0; k < 6; k++)
+        dq[k] = w->bq[12 * b + 6 + k] - w->bqm[12 * b + 6 + k];
+    for(int r = 0; r < 6; r++)
+    {
+        double pr = 0.0;
+        for(int c = 0; c < 6; c++)
+        {
+            double P = st * M[(6 + r) * 12 + 6 + c];
+            pr += P * dq[c];
+            if(H)
+                H[(6 + r) * 12 + 6 + c] += P;
+        }
+        if(g)
+            g[6 + r] += pr;
+        e += 0.5 * dq[r] * pr;
+    }
+    return e;
 }
 
 /* node map of a vertex: FEM -> itself with C = I; ABD -> the 4 body nodes with
@@ -815,6 +868,8 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
     int nb1    = w->nabd ? w->nabd : 1;
     w->bstart  = (int*)calloc(nb1 + 1, sizeof(int));
     w->bfix    = (int*)calloc(nb1, sizeof(int));
+    w->bmotor  = (int*)calloc(nb1, sizeof(int));
+    w->bqm     = (double*)calloc(12 * nb1, sizeof(double));
     w->xbar    = (double*)calloc(3 * nv, sizeof(double));
     w->bM      = (double*)calloc(144 * nb1, sizeof(double));
     w->bq      = (double*)calloc(12 * nb1, sizeof(double));
@@ -841,6 +896,11 @@ orc_world* orc_world_create(const sgipc_scene_desc* s)
         }
         for(int k = 0; k < 3; k++)
             c[k] /= ms;
+        /* motor bodies (btype 2 on every vertex) move freely otherwise */
+        w->bmotor[b] = bt(w, w->bstart[b]) == 2;
+        if(w->bmotor[b] && w->btype)
+            for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
+                w->btype[v] = 0;
         w->bfix[b] = bt(w, w->bstart[b]) != 0;
         double* M  = w->bM + 144 * b;
         for(int v = w->bstart[b]; v < w->bstart[b + 1]; v++)
@@ -968,7 +1028,7 @@ void orc_world_free(orc_world* w)
     free(w->abd_of); free(w->tet_abd); free(w->bstart); free(w->bfix); free(w->xbar);
     free(w->bM); free(w->bq); free(w->bqp); free(w->bqv); free(w->bqt); free(w->bqs);
     free(w->bG); free(w->bV); free(w->npart); free(w->dy);
-    free(w->soft_idx); free(w->soft_tgt);
+    free(w->soft_idx); free(w->soft_tgt); free(w->bmotor); free(w->bqm);
     free(w);
 }
 
@@ -1281,6 +1341,8 @@ static void reduce_to_nodes(orc_world* w, const trip_t* T, double* grad, trip_t*
         orc_abd_shape_grad(w->bq + 12 * b, sg);
         for(int k = 0; k < 9; k++)
             g[3 + k] += kv * sg[k];
+        if(w->bmotor[b])
+            motor_term(w, b, g, NULL);
     }
     if(!T)
         return;
@@ -1320,6 +1382,8 @@ static void reduce_to_nodes(orc_world* w, const trip_t* T, double* grad, trip_t*
             for(int i = 0; i < 9; i++)
                 for(int j = 0; j < 9; j++)
                     H[(3 + i) * 12 + 3 + j] += S[i * 9 + j];
+            if(w->bmotor[b])
+                motor_term(w, b, NULL, H);
         }
         int base = w->nfem + 4 * b;
         for(int m = 0; m < 4; m++)
@@ -1368,6 +1432,8 @@ void orc_world_energy(orc_world* w, double* terms)
         for(int i = 0; i < 12; i++)
             for(int k = 0; k < 12; k++)
                 kin += 0.5 * d[i] * M[i * 12 + k] * d[k];
+        if(w->bmotor[b])
+            kin += motor_term(w, b, NULL, NULL);
         shape += w->s.abd_kappa * w->bV[b] * orc_abd_shape_energy(w->bq + 12 * b);
     }
     shape *= w->s.dt * w->s.dt;
@@ -1841,6 +1907,9 @@ static void begin_step(orc_world* w)
         suggest_kappa(w, &w->kappa);
     init_kappa(w);
     w->anim_rate = w->anim_sub; /* animation_fullRate = animation_subRate (GIPC.cu:11018) */
+    for(int b = 0; b < w->nabd; b++)
+        if(w->bmotor[b])
+            orc_abd_motor_target(w->bqt + 12 * b, w->s.motor_speed * w->s.dt, w->bqm + 12 * b);
     build_friction_sets(w);
     memset(w->dx, 0, sizeof(double) * 3 * w->nv);
     memset(w->dy, 0, sizeof(double) * 3 * (w->nn ? w->nn : 1));

@@ -90,6 +90,7 @@ def lib():
         L.orc_mas_total_clusters.argtypes = [C.c_void_p]
         L.orc_mas_chain.argtypes = [C.c_void_p, ip]
         L.orc_block_jacobi.argtypes = [C.POINTER(Bsr), dp, dp]
+        L.orc_abd_motor_target.argtypes = [dp, C.c_double, dp]
         L.orc_pcg.argtypes = [C.POINTER(Bsr), C.c_void_p, C.c_int, dp, dp, C.c_double,
                               C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_world_create.restype = C.c_void_p
@@ -331,6 +332,14 @@ class Mas:
         c = np.zeros(self.nb * max(L - 1, 1), np.int32)
         lib().orc_mas_chain(self.h, P(c, C.c_int32))
         return c.reshape(self.nb, max(L - 1, 1))
+
+
+def abd_motor_target(qt, theta):
+    """motor target q_p of an affine body from its prediction q~"""
+    q = np.ascontiguousarray(qt, np.float64)
+    out = np.zeros(12)
+    lib().orc_abd_motor_target(P(q), float(theta), P(out))
+    return out
 
 
 def block_jacobi(A: OBsr, r):

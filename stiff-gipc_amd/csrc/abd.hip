@@ -19,7 +19,10 @@
 //   k_abd_expand  vertex displacements J dq (cal_dx_from_dq)
 //   k_abd_q/x     q = q_s - alpha dq, x = J q (step_forward, cal_x_from_q)
 //   k_energy_abd  kappa V dt^2 ||A A^T - I||^2 (cal_abd_shape_energy)
-// Fixed bodies keep M and get no shape term and a zero RHS.
+// Fixed bodies keep M and get no shape term and a zero RHS.  Motor bodies add
+// 0.5 dq^T P dq with dq = q - q_p on a2, a3 and P = strength M[6:12, 6:12]
+// (setup_abd_system_gradient_and_hessian.cu:262-339; the target q_p is set per
+// time step on the host).
 #include "assembly.h"
 #include "eig9.h"
 
@@ -57,6 +60,26 @@ __global__ void __launch_bounds__(64) k_abd_body(AbdDev a, double dt2, double* _
         for(int c = 0; c < 3; c++)
             gb[3 + 3 * i + c] =
                 fx ? 0.0 : kv * 4.0 * (D[i][0] * A[0][c] + D[i][1] * A[1][c] + D[i][2] * A[2][c]);
+    // motor: dq on a2, a3 and P dq into the node gradient
+    const double* M  = a.M + 144 * (size_t)b;
+    const bool    mo = a.motor && a.motor[b] != 0;
+    if(mo)
+    {
+        const double* qm = a.qm + 12 * (size_t)b;
+        double        dq[6];
+#pragma unroll
+        for(int k = 0; k < 6; k++)
+            dq[k] = q[6 + k] - qm[6 + k];
+#pragma unroll
+        for(int r = 0; r < 6; r++)
+        {
+            double pr = 0.0;
+#pragma unroll
+            for(int c = 0; c < 6; c++)
+                pr += a.motor_strength * M[12 * (6 + r) + 6 + c] * dq[c];
+            gb[6 + r] += pr;
+        }
+    }
     if(!hessian)
         return;
     double S[45];
@@ -109,7 +132,6 @@ __global__ void __launch_bounds__(64) k_abd_body(AbdDev a, double dt2, double* _
                 }
         }
     }
-    const double* M = a.M + 144 * (size_t)b;
     const int*    sl = a.slots + 16 * (size_t)b;
 #pragma unroll
     for(int m = 0; m < 4; m++)
@@ -126,6 +148,8 @@ __global__ void __launch_bounds__(64) k_abd_body(AbdDev a, double dt2, double* _
                     double h = M[12 * R + C];
                     if(m > 0 && n > 0)
                         h += S[pk<9>(R - 3, C - 3)];
+                    if(mo && m > 1 && n > 1)
+                        h += a.motor_strength * M[12 * R + C];
                     dst[3 * r + c] = h;
                 }
         }
@@ -237,6 +261,21 @@ __global__ void __launch_bounds__(256) k_energy_abd(AbdDev a, double dt2, double
                 e += d * d;
             }
         e *= a.kappa * a.vol[b] * dt2;
+        if(a.motor && a.motor[b])  // the motor's share of the body's kinetic energy
+        {
+            const double* M  = a.M + 144 * (size_t)b;
+            const double* qm = a.qm + 12 * (size_t)b;
+            double        dq[6];
+            for(int k = 0; k < 6; k++)
+                dq[k] = q[6 + k] - qm[6 + k];
+            for(int r = 0; r < 6; r++)
+            {
+                double pr = 0.0;
+                for(int c = 0; c < 6; c++)
+                    pr += a.motor_strength * M[12 * (6 + r) + 6 + c] * dq[c];
+                e += 0.5 * dq[r] * pr;
+            }
+        }
     }
     block_partial(e, out);
 }

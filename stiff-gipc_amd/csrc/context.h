@@ -142,6 +142,11 @@ class Context
     DBuf<int4>     pairs_, pairs_raw_, cand_;
     DBuf<int>      overflow_;
     DBuf<double>   epart_, eterms_;
+    std::vector<int> h_bmotor_;                                         // motor bodies
+    DBuf<int>      bmotor_;
+    DBuf<double>   bqm_;
+    bool           any_motor_ = false;
+    void           set_motor_targets();
     DBuf<uint32_t> soft_idx_;                                           // soft targets
     DBuf<double>   soft_tgt_;
     int            nsoft_ = 0;

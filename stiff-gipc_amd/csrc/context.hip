@@ -213,6 +213,21 @@ Context::Context(const sgipc_scene_desc& d)
     h_bMinv_.assign(144 * (size_t)nabd_, 0.0);
     for(int v = nfem_; v < nv_; v++)
         bstart[abd_of[v] + 1] = v + 1;
+    // motor bodies (btype 2 on every vertex, BodyBoundaryType::Motor) move
+    // freely apart from their motor term
+    h_bmotor_.assign(nabd_, 0);
+    for(int b = 0; b < nabd_; b++)
+    {
+        h_bmotor_[b] = bt[bstart[b]] == 2;
+        for(int v = bstart[b]; v < bstart[b + 1]; v++)
+        {
+            SG_CHECK((bt[v] == 2) == (h_bmotor_[b] != 0), SGIPC_EINVAL,
+                     "an affine body is partly a motor");
+            if(h_bmotor_[b])
+                bt[v] = 0;
+        }
+        any_motor_ = any_motor_ || h_bmotor_[b];
+    }
     for(int b = 0; b < nabd_; b++)
     {
         double ms = 0.0, c[3] = {0, 0, 0};
@@ -527,6 +542,12 @@ Context::Context(const sgipc_scene_desc& d)
         a.bstart = bstart_.data(), a.fixed = bfix_.data(), a.M = bM_.data(), a.vol = bV_.data();
         a.q = bq_.data(), a.qs = bqs_.data(), a.gbody = gbody_.data(), a.slots = bslots_.data();
         a.kappa = d.abd_kappa;
+        bmotor_.upload(h_bmotor_, s_);
+        bqm_.resize(12 * (size_t)nabd_);
+        bqm_.zero(s_);
+        a.motor          = any_motor_ ? bmotor_.data() : nullptr;
+        a.qm             = bqm_.data();
+        a.motor_strength = d.motor_strength;
         dev_.nfem = nfem_, dev_.node_of = node_of_.data(), dev_.xbar = xbar_.data();
     }
     tri_.upload(d.tris, 3 * (size_t)ntri_, s_);
@@ -1423,6 +1444,8 @@ void Context::begin_step()
         suggest_kappa(kappa_);
     init_kappa();
     dev_.soft_rate = anim_sub_;  // animation_fullRate = animation_subRate (GIPC.cu:11018)
+    if(any_motor_)
+        set_motor_targets();
     // buildFrictionSets on the current contact sets (GIPC.cu:11016)
     launch_friction_build(dev_, kappa_, s_);
     dx_.zero(s_);
@@ -1450,6 +1473,45 @@ bool Context::next_newton_loop()
     newton_k_ = 0;
     xdelta0_  = 0;
     return false;
+}
+
+// Motor target of each motor body from its step prediction q~ (the affine
+// fit of x~ over the body's vertices, exact since x~ = J q~): the reference's
+// q_p = J^-1 J_delta q~ + q~ (J, J_delta the affine maps of the points 0, e_x,
+// e_y, e_z and of their displacement under R_x(theta),
+// setup_abd_system_gradient_and_hessian.cu:264-307) is A_p = A~ R_x(theta) on
+// the rows a_i; rows a2 and a3 are then normalised.
+void Context::set_motor_targets()
+{
+    std::vector<double> xt = xt_.to_host(s_);
+    std::vector<double> qm(12 * (size_t)nabd_, 0.0);
+    const double        th = desc_.motor_speed * desc_.dt;
+    const double        c = std::cos(th), sn = std::sin(th);
+    for(int b = 0; b < nabd_; b++)
+    {
+        if(!h_bmotor_[b])
+            continue;
+        double qt[12];
+        body_fit(b, xt.data(), qt);
+        double* qp = qm.data() + 12 * (size_t)b;
+        for(int k = 0; k < 3; k++)
+            qp[k] = 2.0 * qt[k];
+        for(int i = 0; i < 3; i++)
+        {
+            const double* a     = qt + 3 + 3 * i;
+            qp[3 + 3 * i]       = a[0];
+            qp[3 + 3 * i + 1]   = c * a[1] + sn * a[2];
+            qp[3 + 3 * i + 2]   = -sn * a[1] + c * a[2];
+        }
+        for(int i = 1; i < 3; i++)
+        {
+            double* a = qp + 3 + 3 * i;
+            double  n = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+            for(int k = 0; k < 3; k++)
+                a[k] /= n;
+        }
+    }
+    bqm_.upload(qm, s_);
 }
 
 void Context::set_soft_targets(const double* t)

@@ -53,6 +53,10 @@ struct AbdDev
     double*       gbody  = nullptr;  // 12 per body: shape gradient
     const int*    slots  = nullptr;  // 16 per body: BSR slot of node block (m, n)
     double        kappa  = 0.0;
+    // motor bodies: flag and target q_p per body, the motor's mass multiple
+    const int*    motor  = nullptr;
+    const double* qm     = nullptr;
+    double        motor_strength = 0.0;
 };
 
 struct SimDev

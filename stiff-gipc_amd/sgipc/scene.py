@@ -78,6 +78,8 @@ class SceneDesc(C.Structure):
         ("motion_rate", C.c_double),
         ("bbox_diag", C.c_double),
         ("scene_diag", C.c_double),
+        ("motor_speed", C.c_double),
+        ("motor_strength", C.c_double),
     ]
 
 
@@ -139,6 +141,8 @@ class Scene:
     motion_rate: float = 1.0               # Newton loops per step (animation_subRate = 1/rate)
     bbox_diag: float = 0.0                 # kappa/friction scale (0: surface AABB)
     scene_diag: float = 0.0                # Newton tolerance scale (0: free-vertex AABB)
+    motor_speed: float = -0.02             # motor bodies (btype 2): rad/s about body x
+    motor_strength: float = 200.0          # ... and the motor's mass multiple
     name: str = "scene"
     meta: dict = field(default_factory=dict)
 
@@ -233,6 +237,8 @@ class Scene:
         d.motion_rate = self.motion_rate
         d.bbox_diag = self.bbox_diag
         d.scene_diag = self.scene_diag
+        d.motor_speed = self.motor_speed
+        d.motor_strength = self.motor_strength
         return d
 
 

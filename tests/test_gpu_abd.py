@@ -245,3 +245,28 @@ def test_contact_table_growth(oracle, monkeypatch):
     err = np.abs(sim2.positions() - w2.positions()).max() / np.abs(w2.positions()).max()
     print(f"[table-growth] 3 steps err={err:.2e}", flush=True)
     assert err < 1e-8
+
+
+def test_motor_body_trajectory(oracle):
+    """an affine motor box (btype 2 on its vertices) turning on a soft box:
+    motor target per step, its gradient and P = strength M[6:12, 6:12] block
+    on the GPU against the oracle, positions within 1e-8 per step"""
+    import sgipc
+    sc = _scene(n_abd=1, gap=2e-4, motor_speed=-3.0, motor_strength=200.0)
+    sc.btype = sc.btype.copy()
+    sc.btype[sc.abd_body == 0] = 2
+    sim = sgipc.Simulation(sc)
+    w = oracle.World(sc)
+    for step in range(6):
+        sg = sim.time_step()
+        so = w.time_step()
+        xg, xo = sim.positions(), w.positions()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[motor] step={step} newton={sg['newton_iterations']}/{so['newton_iterations']} "
+              f"err={err:.2e} kin+motor={sim.energy()[1]:.4e}", flush=True)
+        assert err < 1e-8
+    # the box turned about its x axis
+    abd = sc.abd_body
+    x0, x = sc.x[abd == 0], sim.positions()[abd == 0]
+    A, *_ = np.linalg.lstsq(x0 - x0.mean(0), x - x.mean(0), rcond=None)
+    assert np.arctan2(A.T[2, 1], A.T[1, 1]) < -0.01

@@ -210,3 +210,85 @@ def test_block_jacobi_inverts_body_blocks_whole(oracle):
     A.nbody = 0  # the same system preconditioned by 3x3 node blocks only
     x3, it3, _, _ = oracle.pcg(A, g, None, 0, rel_tol=1e-6)
     assert it12 < it3
+
+
+def _jac(xbar):
+    """ABDJacobi::to_mat (abd_jacobi_matrix.inl:32-43): x = t + (a1.xbar, a2.xbar, a3.xbar)"""
+    J = np.zeros((3, 12))
+    J[:, :3] = np.eye(3)
+    for i in range(3):
+        J[i, 3 + 3 * i:6 + 3 * i] = xbar
+    return J
+
+
+def test_motor_target_is_the_reference_matrix_formula(oracle):
+    """q_p = J^-1 J_delta q~ + q~ with J, J_delta stacked from the points 0,
+    e_x, e_y, e_z and their displacements under R_x(theta)
+    (setup_abd_system_gradient_and_hessian.cu:264-307), rows a2 and a3
+    normalised, restated with numpy matrices"""
+    rng = np.random.default_rng(4)
+    for theta in (-0.02 * 0.01, 0.3, -1.1):
+        qt = _q(rng.normal(size=3), _rot(rng.normal(size=3), 0.8) @ (np.eye(3) + 0.05 *
+                                                                   rng.normal(size=(3, 3))))
+        e = np.eye(3)
+        R = _rot([1, 0, 0], theta)
+        J = np.vstack([_jac(np.zeros(3)), _jac(e[0]), _jac(e[1]), _jac(e[2])])
+        Jd = np.vstack([_jac(np.zeros(3)), _jac(np.zeros(3)), _jac(R @ e[1] - e[1]),
+                        _jac(R @ e[2] - e[2])])
+        ref = np.linalg.inv(J) @ Jd @ qt + qt
+        for s in (slice(6, 9), slice(9, 12)):
+            ref[s] /= np.linalg.norm(ref[s])
+        got = oracle.abd_motor_target(qt, theta)
+        np.testing.assert_allclose(got[3:], ref[3:], rtol=1e-12, atol=1e-14)
+
+
+def _motor_scene():
+    from sgipc import scene as S
+    sc = S.abd_scene(n_abd=1, fem_base=False, gravity=(0.0, 0.0, 0.0), use_ground=False,
+                     motor_speed=-2.0, motor_strength=200.0)
+    sc.btype = sc.btype.copy()
+    sc.btype[sc.abd_body == 0] = 2
+    return sc
+
+
+def test_motor_gradient_is_energy_derivative(oracle):
+    sc = _motor_scene()
+    w = oracle.World(sc)
+    w.time_step()  # sets the motor target from the step prediction
+    x = w.positions().copy()
+    g, _ = w.assemble()
+    abd = sc.abd_body
+    xbar0 = sc.x[abd == 0] - sc.x[abd == 0].mean(0)
+    J = [oracle.abd_jmat(v) for v in xbar0]
+    h = 1e-7
+    for k in (0, 4, 6, 8, 10, 11):
+        def E(s):
+            y = x.copy()
+            dq = np.zeros(12)
+            dq[k] = s
+            y[abd == 0] = x[abd == 0] + np.array([Jm @ dq for Jm in J])
+            w.set_positions(y)
+            return w.energy()[0]
+        fd = (E(h) - E(-h)) / (2 * h)
+        assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(g).max())
+    w.set_positions(x)
+
+
+def test_motor_turns_the_body_about_its_x_axis(oracle):
+    """a free motor box (no gravity, no contact) turns about its body x axis
+    and stays rigid.  The target rotates the step prediction q~ (which carries
+    the current angular velocity) by theta = speed * dt, so each step adds
+    theta to the angular step: after n steps the angle is theta n (n + 1) / 2"""
+    sc = _motor_scene()
+    w = oracle.World(sc)
+    abd = sc.abd_body
+    x0 = w.positions()[abd == 0]
+    th = sc.motor_speed * sc.dt
+    for n in range(1, 7):
+        w.time_step()
+        x = w.positions()[abd == 0]
+        A, *_ = np.linalg.lstsq(x0 - x0.mean(0), x - x.mean(0), rcond=None)
+        A = A.T
+        assert np.linalg.norm(A @ A.T - np.eye(3)) < 1e-3 and abs(A[0, 0] - 1) < 1e-3
+        ang = np.arctan2(A[2, 1], A[1, 1])
+        assert ang == pytest.approx(th * n * (n + 1) / 2, rel=0.05)
