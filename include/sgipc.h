@@ -197,7 +197,8 @@ int sgipc_assemble(sgipc_t ctx, double* grad, int32_t* nnzb, int32_t* rowptr,
 
 /* Incremental potential at the current positions, FP64 (computeEnergy,
  * GIPC.cu:10392-10451).  terms[9] = {total, kinetic, elastic (tets),
- * barrier, ground, friction, cloth (membrane + bending), affine shape,
+ * barrier, ground, friction, cloth (membrane + bending), affine shape (and
+ * the motor bodies' drive term, which the reference counts as kinetic),
  * soft targets}. */
 int sgipc_energy(sgipc_t ctx, double* terms);
 

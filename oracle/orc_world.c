@@ -1420,7 +1420,7 @@ int orc_world_assemble(orc_world* w, double* grad, orc_bsr* H)
 
 void orc_world_energy(orc_world* w, double* terms)
 {
-    double kin = 0.0, ela = 0.0, bar = 0.0, gnd = 0.0, shape = 0.0;
+    double kin = 0.0, ela = 0.0, bar = 0.0, gnd = 0.0, shape = 0.0, motor = 0.0;
     for(int b = 0; b < w->nabd; b++)
     {
         if(w->bfix[b])
@@ -1433,10 +1433,11 @@ void orc_world_energy(orc_world* w, double* terms)
             for(int k = 0; k < 12; k++)
                 kin += 0.5 * d[i] * M[i * 12 + k] * d[k];
         if(w->bmotor[b])
-            kin += motor_term(w, b, NULL, NULL);
+            motor += motor_term(w, b, NULL, NULL);
         shape += w->s.abd_kappa * w->bV[b] * orc_abd_shape_energy(w->bq + 12 * b);
     }
     shape *= w->s.dt * w->s.dt;
+    shape += motor; /* reported with the affine term (the reference counts it kinetic) */
     for(int v = 0; v < w->nfem; v++)
     {
         double d2 = 0.0;
