@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--n-side", type=int, default=4)
     ap.add_argument("--cells", type=int, default=8)
     ap.add_argument("--mas-fp64", action="store_true", help="all-FP64 MAS instead of FP32 apply")
+    ap.add_argument("--morton", action="store_true",
+                    help="spatially re-sort the scene (Morton vertex order, centroid-key tet "
+                         "order, MAS banks of 16 consecutive vertices) before the run")
     ap.add_argument("--cpu-iters", type=int, default=1, help="oracle Newton iterations timed")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all cores)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -162,6 +165,11 @@ def main():
     sc, settle_default, workload = build_scene(args, S)
     if args.settle is None:
         args.settle = settle_default
+    if args.morton:
+        t0 = time.time()
+        sc = sgipc.spatial_sort(sc).scene
+        workload += "; Morton re-sorted (banks of 16 consecutive vertices)"
+        log(f"spatial sort: {time.time() - t0:.2f} s")
     sc.mas_precision = sgipc.SGIPC_MAS_FP64 if args.mas_fp64 else sgipc.SGIPC_MAS_FP32
     sc.pcg_rel_tol = 1e-4
     nparts = int(sc.part_id.max()) + 1
@@ -284,7 +292,8 @@ def main():
         avg_s = tot / nl * 1e-3
         byt = algorithmic_bytes(dom, sz, nparts, fp32)
         ach = byt / avg_s / 1e9
-        default_shape = (args.settle == settle_default and args.n_side == 4 and args.cells == 8)
+        default_shape = (args.settle == settle_default and args.n_side == 4 and args.cells == 8
+                         and not args.morton)
         traffic, src = measured_traffic(dom + ("_fp32" if fp32 else ""), args.workload) \
             if default_shape else (None, None)
         roof = {"bound": "hbm", "kernel": dom + (" (FP32 inner PCG)" if fp32 else ""),

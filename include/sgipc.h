@@ -146,6 +146,23 @@ int sgipc_ls_mas_chain(sgipc_ls_t ls, int32_t* chain);
 int sgipc_ls_mas_bank(sgipc_ls_t ls, int32_t bank, double* mat, double* inv);
 
 /* ------------------------------------------------------------------------ */
+/* Spatial re-sorting keys (GIPC.cu:279-461 _calcTetMChash / _calcVertMChash /
+ * hash_code, with the sort the reference runs on them).  kind 0: vertices,
+ * the 30-bit Morton code of each position in the vertices' bounding box;
+ * kind 1: tets, their centroid 0.25 ((a + b) + (c + d)) on the 1024^3 grid
+ * packed in the axis order the box's extents select (hash_code types 0..5).
+ * Keys are code << 32 | index, sorted ascending on the device; keys[n] and
+ * order[n] (the index in each key's low word, n = n_verts or n_tets) may be
+ * NULL.  x[3 n_verts] and tets[4 n_tets] are host arrays; a flat axis (zero
+ * extent) contributes cell 0.                                                */
+/* ------------------------------------------------------------------------ */
+#define SGIPC_SORT_VERTICES 0
+#define SGIPC_SORT_TETS 1
+int sgipc_spatial_sort_keys(int32_t kind, const double* x, int32_t n_verts,
+                            const uint32_t* tets, int32_t n_tets, uint64_t* keys,
+                            uint32_t* order);
+
+/* ------------------------------------------------------------------------ */
 /* Simulation context (GIPC counterpart)                                     */
 /* ------------------------------------------------------------------------ */
 int sgipc_create(const sgipc_scene_desc* desc, sgipc_t* out);

@@ -65,6 +65,11 @@ void   orc_ground_grad_hess(const double x[3], const double n[3], double off,
 /* ---------------- LBVH (orc_lbvh.c) ---------------- */
 uint32_t orc_expand_bits(uint32_t v);
 uint32_t orc_morton_code(double x, double y, double z);
+/* orc_sort.c: spatial re-sorting keys (kind 0 vertices, 1 tets) */
+uint32_t orc_hash_code(int type, double x, double y, double z);
+int      orc_tet_hash_type(double sx, double sy, double sz);
+int      orc_spatial_keys(int kind, const double* x, int nv, const uint32_t* tets, int nt,
+                          uint64_t* keys, uint32_t* order);
 /* leaf AABBs {upper xyz, lower xyz} of n primitives of arity 2 or 3 */
 void orc_leaf_aabbs(const double* x, const uint32_t* prims, int arity, int n,
                     double* aabb);

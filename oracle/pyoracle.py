@@ -57,6 +57,10 @@ def lib():
         L.orc_expand_bits.argtypes = [C.c_uint32]
         L.orc_morton_code.restype = C.c_uint32
         L.orc_morton_code.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_hash_code.restype = C.c_uint32
+        L.orc_hash_code.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_tet_hash_type.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_spatial_keys.argtypes = [C.c_int, dp, C.c_int, up, C.c_int, u64p, up]
         L.orc_leaf_aabbs.argtypes = [dp, up, C.c_int, C.c_int, dp]
         L.orc_lbvh_build.argtypes = [dp, up, C.c_int, C.c_int, u64p, up, dp]
         for f in ("orc_d_pp",):
@@ -230,6 +234,25 @@ def lbvh(x, prims):
     lib().orc_lbvh_build(P(x), P(prims, C.c_uint32), ar, n, P(keys, C.c_uint64),
                          P(nodes, C.c_uint32), P(aabbs))
     return keys, nodes, aabbs
+
+
+def spatial_keys(x, tets=None):
+    """sorted keys (code << 32 | index) and the order of the vertices
+    (tets None) or of the tets (GIPC.cu:279-461)"""
+    x = np.ascontiguousarray(x, np.float64)
+    nv = x.shape[0]
+    if tets is None:
+        kind, n, t = 0, nv, np.zeros((1, 4), np.uint32)
+    else:
+        t = np.ascontiguousarray(tets, np.uint32)
+        kind, n = 1, t.shape[0]
+    keys = np.zeros(n, np.uint64)
+    order = np.zeros(n, np.uint32)
+    r = lib().orc_spatial_keys(kind, P(x), nv, P(t, C.c_uint32), t.shape[0] if kind else 0,
+                               P(keys, C.c_uint64), P(order, C.c_uint32))
+    if r != 0:
+        raise ValueError("no vertices or no tets")
+    return keys, order
 
 
 def contact_pairs(scene, x=None, method="bvh", cap=None):

@@ -155,6 +155,30 @@ int sgipc_mm_read_vector(const char* path, double* b, int32_t cap, int32_t* m)
     });
 }
 
+int sgipc_spatial_sort_keys(int32_t kind, const double* x, int32_t n_verts,
+                            const uint32_t* tets, int32_t n_tets, uint64_t* keys,
+                            uint32_t* order)
+{
+    return guard([&] {
+        SG_CHECK(kind == SGIPC_SORT_VERTICES || kind == SGIPC_SORT_TETS, SGIPC_EINVAL,
+                 "kind must be SGIPC_SORT_VERTICES or SGIPC_SORT_TETS");
+        SG_CHECK(x && n_verts > 0, SGIPC_EINVAL, "no vertices");
+        SG_CHECK(kind == SGIPC_SORT_VERTICES || (tets && n_tets > 0), SGIPC_EINVAL, "no tets");
+        hipStream_t s;
+        SG_HIP(hipStreamCreate(&s));
+        try
+        {
+            spatial_sort_keys(kind, x, n_verts, tets, n_tets, keys, order, s);
+        }
+        catch(...)
+        {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        SG_HIP(hipStreamDestroy(s));
+    });
+}
+
 int sgipc_ls_set_stream(sgipc_ls_t ls, void* stream)
 {
     return guard([&] {

@@ -503,6 +503,32 @@ void Lbvh::init(int n_, int arity_, const uint32_t* prims_)
     partial.resize(6 * (size_t)grid_for(n, 256));
 }
 
+// stable LSD radix sort of n 64-bit keys on the 8-bit digits from bit lo to
+// bit hi (exclusive); the result ends in keys (alt: scratch of n keys, hist:
+// 256 * ceil(n / kSortTile) counters)
+void radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int lo, int hi,
+                     hipStream_t s)
+{
+    if(n <= 1)
+        return;
+    int       nblk = (n + kSortTile - 1) / kSortTile;
+    uint64_t* a    = keys;
+    uint64_t* b    = alt;
+    for(int sh = lo; sh < hi; sh += 8)
+    {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, hist, nblk);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
+                           hist, nblk);
+        SG_LAUNCH_CHECK();
+        std::swap(a, b);
+    }
+    if(a != keys)
+        SG_HIP(hipMemcpyAsync(keys, a, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+}
+
+size_t radix_sort_hist_size(int n) { return (size_t)256 * ((n + kSortTile - 1) / kSortTile); }
+
 void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
 {
     if(n <= 0)

@@ -163,6 +163,15 @@ void launch_step_forward(const SimDev& d, const double* xs, const double* dx, do
                          hipStream_t s);
 void launch_update_velocity(const SimDev& d, double* xprev, double* v, hipStream_t s);
 
+// lbvh.hip: stable radix sort of 64-bit keys on bits [lo, hi) (8-bit digits)
+void   radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int lo, int hi,
+                       hipStream_t s);
+size_t radix_sort_hist_size(int n);
+
+// spatial_sort.hip: sorted spatial keys and orders of vertices (kind 0) or tets (1)
+void spatial_sort_keys(int kind, const double* x, int nv, const uint32_t* tets, int nt,
+                       uint64_t* keys_out, uint32_t* order_out, hipStream_t s);
+
 // lbvh.hip: linear BVH over faces (arity 3) or edges (arity 2)
 struct Lbvh
 {

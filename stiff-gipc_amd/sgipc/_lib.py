@@ -93,7 +93,7 @@ EXPORTS = [
     "sgipc_set_soft_targets", "sgipc_get_animation_rate",
     "sgipc_ls_create_bcoo", "sgipc_ls_create_mm", "sgipc_ls_block_rows", "sgipc_mm_read_vector",
     "sgipc_mm_read_upper_bcoo", "sgipc_ls_set_stream", "sgipc_set_dof_order", "sgipc_mm_export",
-    "sgipc_set_stream", "sgipc_mm_export_step", "sgipc_step_log",
+    "sgipc_set_stream", "sgipc_mm_export_step", "sgipc_step_log", "sgipc_spatial_sort_keys",
 ]
 SGIPC_DOF_FEM_FIRST = 0
 SGIPC_DOF_ABD_FIRST = 1
@@ -126,6 +126,7 @@ def lib():
         L.sgipc_ls_block_rows.argtypes = [C.c_void_p, ip]
         L.sgipc_mm_read_vector.argtypes = [C.c_char_p, dp, C.c_int32, ip]
         L.sgipc_mm_read_upper_bcoo.argtypes = [C.c_char_p, ip, ip, ip, ip, dp, C.c_int32]
+        L.sgipc_spatial_sort_keys.argtypes = [C.c_int32, dp, C.c_int32, up, C.c_int32, u64p, up]
         L.sgipc_ls_set_stream.argtypes = [C.c_void_p, C.c_void_p]
         if hasattr(L, "sgipc_create"):
             L.sgipc_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
@@ -194,6 +195,24 @@ def read_mm_vector(path):
     b = np.zeros(m.value)
     check(lib().sgipc_mm_read_vector(os.fsencode(path), P(b), m.value, C.byref(m)))
     return b
+
+
+def spatial_sort_keys(x, tets=None):
+    """(sorted keys, order) of the vertices x (tets None: Morton codes) or of
+    the tets (centroid keys); keys are code << 32 | index (GIPC.cu:279-461)"""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    if tets is None:
+        kind, n, t = 0, x.shape[0], None
+    else:
+        t = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+        kind, n = 1, t.shape[0]
+    keys = np.zeros(n, np.uint64)
+    order = np.zeros(n, np.uint32)
+    check(lib().sgipc_spatial_sort_keys(kind, P(x), x.shape[0],
+                                        None if t is None else P(t, C.c_uint32),
+                                        0 if t is None else n, P(keys, C.c_uint64),
+                                        P(order, C.c_uint32)))
+    return keys, order
 
 
 def read_mm_upper_bcoo(path):

@@ -29,10 +29,12 @@ def _scene_file(tmp_path):
                        "rot_origin": [7.5, 0, 0], "rot_axis": [1, 0, 0], "rot_velocity": 0.5}]
     d["settings"]["motion_stiffness"] = 1e6
     d["simulation"]["export_linear_system_frames"] = [1]
-    # FP64 MAS (an extension key): the JSON's loose Newton and PCG tolerances
-    # (abs_xdelta_tol 1e-3, pcg_rel_threshold 1e-2) stop short of convergence,
-    # so FP32 rounding of the preconditioner would show at the Newton tolerance
+    # FP64 MAS (an extension key) and a tight PCG threshold: with the file's
+    # pcg_rel_threshold 1e-2, the PCG stops on an iterate that rounding steers
+    # (FP64 atomics order on the GPU, sequential sums in the oracle), and the
+    # two trajectories part at 1e-7 within 3 frames
     d["simulation"]["mas_precision"] = "fp64"
+    d["settings"]["pcg_rel_threshold"] = 1e-9
     p = tmp_path / "scene.json"
     p.write_text(json.dumps(d))
     return p
