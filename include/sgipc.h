@@ -121,7 +121,7 @@ int sgipc_ls_set_stream(sgipc_ls_t ls, void* stream);
  * serving loops, harnesses that forbid hipMalloc).  The same solve (FP64 PCG
  * with the system's preconditioner, same kernels) with every work buffer --
  * device copies of b and x, PCG vectors, reductions, control words, the
- * block-Jacobi inverse -- in one fixed device array per GPU (about 22 MB,
+ * block-Jacobi inverse -- in one fixed device array per GPU (about 31 MB,
  * held until the process ends).  Limits: at most SGIPC_NOALLOC_MAX_ROWS block
  * rows (SGIPC_ECAPACITY beyond), FP64 solver precision only (SGIPC_EINVAL
  * for the mixed mode), called with the system's device current.  Every call

@@ -361,18 +361,26 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v)
 }
 
 // ------------------------------------------------------------ slot sets
-// A global FP64 sum is accumulated by one atomic per workgroup into one of
-// kSlots adjacent doubles, chosen by blockIdx % kSlots, so no address sees
-// more than a few dozen atomics (a single address serialises every atomic of
-// the grid).  A consumer sums the kSlots values with one wave, whose load is
-// then 512 contiguous bytes.  Spreading the slots one per 128-byte line
-// instead (kSlotStride 16) made every consumer wave's load touch 64 lines and
-// measured slower: 13.7 against 13.2 us for the PCG SpMV of a 46,656-vertex
-// system, whose every wave reads two sets.  A "slot set" is
-// kSlots * kSlotStride doubles and must be zero before accumulation starts.
-constexpr int kSlots      = 64;
-constexpr int kSlotStride = 1;
-constexpr int kSlotSet    = kSlots * kSlotStride;
+// A global FP64 sum over the workgroups of one launch, in a fixed order so
+// that reruns give identical bits.  Each workgroup reduces its values (wave
+// butterflies, then the waves in index order) and stores the result in its
+// own partial; a one-workgroup launch that follows in stream order
+// (set_finalize) sums the partials in a fixed order and adds the total to
+// slot 0.  A reduction across workgroups inside the launch itself would need
+// an agent-scope release fence per workgroup, and each of those writes back
+// the L2: measured, it tripled the PCG SpMV's time (170 against 55 us on a
+// 349k-row system), where the extra launch costs a few microseconds.
+//
+// Layout of a set (doubles): kSlots result slots (slot 0 holds the sum, the
+// others stay 0: a consumer wave sums all kSlots, one per lane, so a set is
+// read as 512 contiguous bytes), then the workgroup partials.  Launches may
+// add to the same set one after another (each with its finalize; the total
+// accumulates in slot 0); a launch may have at most kSetMaxBlocks workgroups.
+constexpr int kSlots        = 64;
+constexpr int kSlotStride   = 1;
+constexpr int kSetMaxBlocks = 65536;
+constexpr int kSetOffBlock  = kSlots;
+constexpr int kSlotSet      = kSetOffBlock + kSetMaxBlocks;
 
 // every lane of the calling wave receives the total of the set
 __device__ __forceinline__ double slot_set_sum(const double* set)
@@ -380,8 +388,9 @@ __device__ __forceinline__ double slot_set_sum(const double* set)
     return wave_sum(set[__lane_id() * kSlotStride]);
 }
 
-// workgroup-wide sum of v added into the set; call from every thread of the
-// workgroup (contains __syncthreads); blockDim.x a multiple of 64, <= 1024
+// this workgroup's sum of v stored as its partial of the set; call from every
+// thread of every workgroup of the launch (contains __syncthreads);
+// blockDim.x a multiple of 64, <= 1024; gridDim.x <= kSetMaxBlocks
 __device__ __forceinline__ void slot_set_add(double* set, double v)
 {
     __shared__ double part[16];
@@ -396,12 +405,27 @@ __device__ __forceinline__ void slot_set_add(double* set, double v)
         int    nw = blockDim.x >> 6;
         for(int i = 0; i < nw; i++)
             t += part[i];
-        if(t != 0.0)
-            atomicAdd(set + (blockIdx.x % kSlots) * kSlotStride, t);
+        set[kSetOffBlock + blockIdx.x] = t;
     }
 }
 
-// clear one slot set (kSlots threads of any workgroup)
+// set_finalize's body: one workgroup of 256 threads sums the nblocks partials
+// (thread t: partials t, t + 256, ... in order; then the waves in order)
+__device__ __forceinline__ void slot_set_finalize(double* set, int nblocks)
+{
+    __shared__ double part[4];
+    double            t = 0.0;
+    for(int i = threadIdx.x; i < nblocks; i += 256)
+        t += set[kSetOffBlock + i];
+    t = wave_sum(t);
+    if(__lane_id() == 0)
+        part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if(threadIdx.x == 0)
+        set[0] = set[0] + (((part[0] + part[1]) + part[2]) + part[3]);
+}
+
+// clear the result slots of one set (kSlots threads of any workgroup)
 __device__ __forceinline__ void slot_set_clear(double* set, int lane)
 {
     if(lane < kSlots)

@@ -222,12 +222,16 @@ __global__ void __launch_bounds__(1024) k_sort_scan(int total, unsigned* __restr
     }
 }
 
-// stable scatter: keys are visited in tile order (round j, then thread)
+// stable scatter: keys are visited in tile order (round j, then thread);
+// with KV each key's 32-bit value moves with it
+template <bool KV>
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(int n, int shift,
                                                                const uint64_t* __restrict__ in,
                                                                uint64_t* __restrict__ out,
                                                                const unsigned* __restrict__ hist,
-                                                               int nblk)
+                                                               int nblk,
+                                                               const uint32_t* __restrict__ vin,
+                                                               uint32_t* __restrict__ vout)
 {
     __shared__ unsigned base_d[256];
     __shared__ unsigned wcnt[4][256];
@@ -262,6 +266,8 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(int n, int shift,
             for(int q = 0; q < w; q++)
                 off += wcnt[q][d];
             out[off] = k;
+            if(KV)
+                vout[off] = vin[i];
         }
         __syncthreads();
         base_d[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x]
@@ -518,13 +524,48 @@ void radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int l
     {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, hist, nblk);
         hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
-                           hist, nblk);
+        hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
+                           hist, nblk, nullptr, nullptr);
         SG_LAUNCH_CHECK();
         std::swap(a, b);
     }
     if(a != keys)
         SG_HIP(hipMemcpyAsync(keys, a, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+}
+
+void radix_sort_pairs(uint64_t* keys, uint64_t* kalt, uint32_t* vals, uint32_t* valt,
+                      unsigned* hist, int n, int lo, int hi, hipStream_t s)
+{
+    if(n <= 1)
+        return;
+    int       nblk = (n + kSortTile - 1) / kSortTile;
+    uint64_t* a    = keys;
+    uint64_t* b    = kalt;
+    uint32_t* va   = vals;
+    uint32_t* vb   = valt;
+    for(int sh = lo; sh < hi; sh += 8)
+    {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, hist, nblk);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist);
+        hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
+                           hist, nblk, va, vb);
+        SG_LAUNCH_CHECK();
+        std::swap(a, b);
+        std::swap(va, vb);
+    }
+    if(a != keys)
+    {
+        SG_HIP(hipMemcpyAsync(keys, a, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        SG_HIP(hipMemcpyAsync(vals, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+}
+
+int radix_key_bits(uint64_t max_key)
+{
+    int b = 0;
+    while(b < 64 && (max_key >> b))
+        b++;
+    return (b + 7) / 8 * 8;
 }
 
 size_t radix_sort_hist_size(int n) { return (size_t)256 * ((n + kSortTile - 1) / kSortTile); }
@@ -541,32 +582,9 @@ void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
     hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, s, n, leaf.data(),
                        scene.data(), keys.data());
     SG_LAUNCH_CHECK();
-    // radix passes: index bits [0, ib), morton bits [32, 62)
-    int ib = 0;
-    while((1ll << ib) < n)
-        ib++;
-    int nblk = (n + kSortTile - 1) / kSortTile;
-    uint64_t* a = keys.data();
-    uint64_t* b = keys_alt.data();
     // the keys enter in index order and every pass is stable, so sorting the
-    // 30 Morton bits alone orders by (morton, index): the index bits need no
-    // passes
-    (void)ib;
-    std::vector<int> shifts;
-    for(int sh = 32; sh < 62; sh += 8)
-        shifts.push_back(sh);
-    for(int sh : shifts)
-    {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a,
-                           hist.data(), nblk);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist.data());
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
-                           hist.data(), nblk);
-        SG_LAUNCH_CHECK();
-        std::swap(a, b);
-    }
-    if(a != keys.data())
-        SG_HIP(hipMemcpyAsync(keys.data(), a, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    // 30 Morton bits alone orders by (morton, index)
+    radix_sort_keys(keys.data(), keys_alt.data(), hist.data(), n, 32, 62, s);
     hipLaunchKernelGGL(k_leaf_nodes, dim3(grid_for(n, 256)), dim3(256), 0, s, n, keys.data(),
                        nodes.data(), leaf.data(), bvs.data(), flags.data());
     if(n > 1)

@@ -95,6 +95,10 @@ struct SysView
 // (slot sets: common.h)
 // host-side sum of a slot set in the device's butterfly order
 double slot_set_sum_host(const double* h);
+// after a launch of nblocks workgroups that called slot_set_add on set: add
+// the sum of its partials to slot 0 (skipped, like the launch, while
+// *done != 0)
+void set_finalize(double* set, int nblocks, const int* done, hipStream_t s);
 double read_slot_set(const double* d, hipStream_t s);
 
 // y = A x (FP64); optional dot(x, y) accumulated into the slot set `dot`
@@ -128,6 +132,26 @@ inline bool mas_blocked_invert()
 
 void launch_mas_invert_gj(const double* B, float* inv, int nbanks, hipStream_t s);
 void launch_mas_invert_gj(const double* B, double* inv, int nbanks, hipStream_t s);
+
+// the bank matrices of MAS levels >= 1 from the Galerkin operators A_l,
+// summed in a fixed order by sorting (mas_galerkin.hip)
+class MasGalerkin
+{
+  public:
+    void run(const Bsr& A, const Coo* coo, int L, const std::vector<int>& lvl, const int* chain,
+             const int* going_next, double* B, hipStream_t s);
+
+  private:
+    void ensure(int n);
+    int  reduce(int n, uint64_t kdead, int nsrc0, const double* src0, const double* src1,
+                int offl, long long nl, double* B, int out, hipStream_t s);
+    int            cap_ = 0;
+    DBuf<uint64_t> keys_, kalt_;
+    DBuf<uint32_t> vals_, valt_;
+    DBuf<unsigned> flag_, pos_, hist_, tsum_;
+    DBuf<int>      er_[2], ec_[2];
+    DBuf<double>   eb_[2];
+};
 
 class Mas
 {
@@ -199,6 +223,7 @@ class Mas
     DBuf<int>      bank_first_; // global bank -> first next-level global id
     DBuf<int>      bank_ncl_;   // global bank -> number of next-level nodes
     size_t         nbanks_alloc_ = 0;
+    MasGalerkin    galerkin_;
 };
 
 // ------------------------------------------------------------------ PCG

@@ -5,10 +5,12 @@
 //   x += a p, r -= a Ap, (every 10th: r = b - A x), z = M r, rz' = r.z,
 //   (every 10th: stop if rz' <= rel^2 rz0 or <= abs^2), p = z + (rz'/rz) p.
 // Unlike the reference (two device->host copies per iteration, :145) every
-// scalar stays on the device: pAp[k] and rz[k] are accumulated into slot sets
-// (see common.h) and the consuming kernels derive a and b from them; the
-// host reads back only at the convergence test.
+// scalar stays on the device: pAp[k] and rz[k] are summed in a fixed order
+// into slot sets (common.h) and the consuming kernels derive a and b from
+// them; the host reads back only at the convergence test.
 #include "linsys.h"
+
+#include "../../include/sgipc.h"
 
 #include <cstdlib>
 #include <cstring>
@@ -579,11 +581,15 @@ static void launch_spmv(const SysView& S, const V* x, const V* pprev, V* pcur, c
     const int* c2;
     const V*   v2;
     contact_rows<V>(S, rp2, c2, v2);
+    SG_CHECK(grid_for((long long)A.nb * kRowLanes, 256) <= (unsigned)kSetMaxBlocks, SGIPC_ECAPACITY,
+             "system too large for the fixed-order reductions (block rows > 4M)");
     hipExtLaunchKernelGGL((k_spmv<MODE, V>), dim3(grid_for((long long)A.nb * kRowLanes, 256)),
                           dim3(256), 0, s, ev.first, ev.second, 0, A.nb, A.rowptr.data(),
                           A.cols.data(), MatVals<V>::a(S), rp2, c2, v2, x, pprev, pcur, b, y, rz_a,
                           rz_b, dot, pc);
     SG_LAUNCH_CHECK();
+    if(dot)
+        set_finalize(dot, (int)grid_for((long long)A.nb * kRowLanes, 256), pc.ctl, s);
 }
 
 void spmv(const SysView& S, const double* x, double* y, double* dot, hipStream_t s)
@@ -620,11 +626,26 @@ static void dot_slots_ctl(int n, const V* a, const V* b, double* out, const int*
     g          = g > 1024 ? 1024 : g;
     hipLaunchKernelGGL(k_dot_slots<V>, dim3(g), dim3(256), 0, s, n, a, b, out, done);
     SG_LAUNCH_CHECK();
+    set_finalize(out, (int)g, done, s);
 }
 
 void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t s)
 {
     dot_slots_ctl<double>(n, a, b, out, nullptr, s);
+}
+
+__global__ void __launch_bounds__(256) k_set_finalize(double* set, int nblocks,
+                                                      const int* __restrict__ done)
+{
+    if(done && *done)
+        return;
+    slot_set_finalize(set, nblocks);
+}
+
+void set_finalize(double* set, int nblocks, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_finalize, dim3(1), dim3(256), 0, s, set, nblocks, done);
+    SG_LAUNCH_CHECK();
 }
 
 // clear a slot set in stream order (a kernel, so the sequence stays free of
@@ -883,10 +904,14 @@ void Pcg::apply_block_jacobi(int nb, V* x, V* r, const V* p, const V* Ap, const 
     const int n3 = nbody_ > 0 ? body_first_ : nb;
     if(n3 > 0 || zero0 || zero1)
     {
+        SG_CHECK(grid_for(n3 > 0 ? n3 : 1, 256) <= (unsigned)kSetMaxBlocks, SGIPC_ECAPACITY,
+                 "system too large for the fixed-order reductions");
         hipLaunchKernelGGL(k_update_diag<V>, dim3(grid_for(n3 > 0 ? n3 : 1, 256)), dim3(256), 0,
                            s, n3, x, r, p, Ap, rz_prev, pap, dinv<V>(), z, rz_out, zero0, zero1,
                            done);
         SG_LAUNCH_CHECK();
+        if(rz_out)
+            set_finalize(rz_out, (int)grid_for(n3 > 0 ? n3 : 1, 256), done, s);
     }
     if(nbody_ > 0)
         launch_update_body<V>(nbody_, body_first_, x, r, p, Ap, rz_prev, pap, binv<V>(), z,

@@ -267,25 +267,26 @@ __global__ void k_chain(int level, int L, int nv, const int* __restrict__ lvl,
 }
 
 // ------------------------------------------------------------- assembly
-// Every stored block (i, j) of A (both triangles, plus the COO tail) adds
-// into the bank matrix of each level at which i and j share a bank.
+// Level 0: every stored block (i, j) of A (both triangles, plus the COO tail)
+// whose nodes share a level-0 bank is that bank's (i, j) block; every
+// (row, col) block appears once (static pattern and merged contact blocks
+// are disjoint), so each bank entry has a single writer.  The coarse levels'
+// bank matrices come from MasGalerkin (mas_galerkin.hip).
 __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restrict__ rows,
                                                     const int* __restrict__ cols,
                                                     const double* __restrict__ vals,
                                                     const unsigned* __restrict__ coo_count,
                                                     int coo_cap, const int* __restrict__ cr,
                                                     const int* __restrict__ cc,
-                                                    const double* __restrict__ cv, int L, int nv,
+                                                    const double* __restrict__ cv,
                                                     const int* __restrict__ slot_of,
-                                                    const int* __restrict__ chain,
                                                     double* __restrict__ B)
 {
     int  t    = blockIdx.x * blockDim.x + threadIdx.x;
     int  ncoo = coo_count ? (int)*coo_count : 0;
     ncoo      = ncoo < coo_cap ? ncoo : coo_cap;
-    int  i = 0, j = 0;
+    int           i = 0, j = 0;
     const double* blk = nullptr;
-    bool live = true;
     if(t < nnz)
     {
         i   = rows[t];
@@ -299,74 +300,16 @@ __global__ void __launch_bounds__(256) k_mas_gather(int nnz, const int* __restri
         blk = cv + 9 * (size_t)(t - nnz);
     }
     else
-        live = false;
-    double b[9];
+        return;
+    int si = slot_of[i], sj = slot_of[j];
+    if((si >> 4) != (sj >> 4))
+        return;
+    double* M = B + (size_t)(si >> 4) * BD * BD + (3 * (si & 15)) * BD + 3 * (sj & 15);
 #pragma unroll
-    for(int e = 0; e < 9; e++)
-        b[e] = live ? blk[e] : 0.0;
-    if(live)
-    {
-        int si = slot_of[i], sj = slot_of[j];
-        if((si >> 4) == (sj >> 4))
-        {
-            // level 0: every (row, col) block appears once (static pattern
-            // and merged contact blocks are disjoint), so each bank entry
-            // has a single writer
-            double* M = B + (size_t)(si >> 4) * BD * BD + (3 * (si & 15)) * BD + 3 * (sj & 15);
+    for(int a = 0; a < 3; a++)
 #pragma unroll
-            for(int a = 0; a < 3; a++)
-#pragma unroll
-                for(int c = 0; c < 3; c++)
-                    M[a * BD + c] = b[3 * a + c];
-        }
-    }
-    for(int l = 1; l < L; l++)
-    {
-        long long key = -1;
-        int       gi = 0, gj = 0;
-        if(live)
-        {
-            gi = chain[(size_t)i * (L - 1) + l - 1];
-            gj = chain[(size_t)j * (L - 1) + l - 1];
-            if((gi >> 4) == (gj >> 4))
-                key = ((long long)gi << 32) | (unsigned)gj;
-        }
-        // Lanes sharing a target block are summed across the wave first (one
-        // atomic per entry and group) when the group is large; the rounds
-        // are capped, and lanes of small groups add their own block, so a
-        // wave with many distinct targets (fine levels) does not pay a wave
-        // reduction per target.
-        unsigned long long pending = __ballot(key != -1), direct = 0;
-        for(int round = 0; round < 6 && pending; round++)
-        {
-            int                leader = __ffsll((long long)pending) - 1;
-            long long          lk     = __shfl(key, leader, 64);
-            bool               mine   = (key == lk);
-            unsigned long long mm     = __ballot(mine);
-            pending &= ~mm;
-            if(__popcll(mm) < 4)
-            {
-                direct |= mm;  // small group: its lanes add their own blocks
-                continue;
-            }
-            int     lgi = (int)(lk >> 32), lgj = (int)(lk & 0xffffffffll);
-            double* M   = B + (size_t)(lgi >> 4) * BD * BD + (3 * (lgi & 15)) * BD + 3 * (lgj & 15);
-#pragma unroll
-            for(int e = 0; e < 9; e++)
-            {
-                double s = wave_sum(mine ? b[e] : 0.0);
-                if((int)__lane_id() == leader)
-                    atomicAdd(M + (e / 3) * BD + (e % 3), s);
-            }
-        }
-        if((pending | direct) & (1ull << __lane_id()))
-        {
-            double* M = B + (size_t)(gi >> 4) * BD * BD + (3 * (gi & 15)) * BD + 3 * (gj & 15);
-#pragma unroll
-            for(int e = 0; e < 9; e++)
-                atomicAdd(M + (e / 3) * BD + (e % 3), b[e]);
-        }
-    }
+        for(int c = 0; c < 3; c++)
+            M[a * BD + c] = blk[3 * a + c];
 }
 
 // Ordering of LDS between the lanes of a single-wave workgroup: the LDS
@@ -567,41 +510,33 @@ __device__ __forceinline__ T bank_product(const T* __restrict__ inv, int gb, T r
     return bank_apply(pv, rv);
 }
 
-// Cluster sums of one bank's R into the next level (plain stores: the bank
-// owns those nodes), and optionally into the level after it (atomics: the
-// kernel two levels up reads them after this launch).  rv: lane t < 48 holds
-// component t % 3 of bank node t / 3; node_up: global id of that node's
-// parent (or -1).
-// up2_node: going_next[first + t / 3] for t < 3 ncl when up2 (the caller may
-// have loaded it early), else unused.  Single-wave workgroups only.
+// Cluster sums of one bank's R into the next level, stored (the bank owns
+// those nodes).  rv: lane t < 48 holds component t % 3 of bank node t / 3;
+// node_up: global id of that node's parent (or -1).  Lane t < 3 ncl sums
+// component t % 3 of the bank's nodes whose parent is cluster first + t / 3,
+// in node order, so the sums do not depend on timing.  Single-wave
+// workgroups only.
 template <typename T>
 __device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, int first,
-                                              int ncl, int up2_node, T* __restrict__ R, bool up2)
+                                              int ncl, T* __restrict__ R)
 {
-    const int t = __lane_id();
+    __shared__ int up[BANK];
+    const int      t = __lane_id();
     if(t < BD)
-        Rc[t] = (T)0;
-    wave_lds_sync();
-    if(node_up >= 0)
-        atomicAdd(&Rc[3 * (node_up - first) + t % 3], rv);
+        Rc[t] = rv;
+    if(t < BD && t % 3 == 0)
+        up[t / 3] = node_up;
     wave_lds_sync();
     if(t < 3 * ncl)
     {
-        T v = Rc[t];
+        const int p = first + t / 3, c = t % 3;
+        T         v = 0;
+#pragma unroll
+        for(int u = 0; u < BANK; u++)
+            if(up[u] == p)
+                v += Rc[3 * u + c];
         R[3 * (size_t)first + t] = v;
-        if(up2)
-            atomicAdd(&R[3 * (size_t)up2_node + t % 3], v);
     }
-}
-
-template <typename T>
-__device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, int first,
-                                              int ncl, const int* __restrict__ going_next,
-                                              T* __restrict__ R, bool up2)
-{
-    const int t   = __lane_id();
-    int       up2n = (up2 && t < 3 * ncl) ? going_next[first + t / 3] : 0;
-    restrict_bank(Rc, rv, node_up, first, ncl, up2n, R, up2);
 }
 
 // Level-0 banks (one wave each): optional PCG update of x and r
@@ -619,7 +554,7 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
                                                const int* __restrict__ going_next,
                                                const int* __restrict__ bank_first,
                                                const int* __restrict__ bank_ncl, T* __restrict__ R,
-                                               T* __restrict__ Z, int has_coarse, int up2,
+                                               T* __restrict__ Z, int has_coarse,
                                                const int* __restrict__ done)
 {
     if(done && *done)
@@ -639,10 +574,9 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
         ra = rz_prev[t * kSlotStride];
         pa = pap[t * kSlotStride];
     }
-    // restriction targets: this node's parent and (up2) the parents of the
-    // bank's clusters, independent of everything before the restriction
-    // (loads are unconditional from a valid fallback address, bank_ncl + b,
-    // so no branch joins force an early wait on them)
+    // restriction target: this node's parent, independent of everything
+    // before the restriction (the load is unconditional from a valid
+    // fallback address, bank_ncl + b, so no branch join forces an early wait)
     int first = 0, ncl = 0;
     if(has_coarse)
     {
@@ -652,9 +586,6 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
     const int gn = *((has_coarse && t < BD) ? going_next + s : bank_ncl + b);
     T         pv[BD];
     bank_load(inv, b, pv);
-    // after the inverse loads: the address waits for `first`
-    const int up2n =
-        *((has_coarse && up2 && t < 3 * ncl) ? going_next + first + t / 3 : bank_ncl + b);
     double a = 0.0;
     if(x)
         a = wave_sum(ra) / wave_sum(pa);  // slot_set_sum of both sets
@@ -689,18 +620,18 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
         r[d] = rv;
     }
     if(has_coarse)
-        restrict_bank(Rc, rt, v >= 0 ? gn : -1, first, ncl, up2n, R, up2 != 0);
+        restrict_bank(Rc, rt, v >= 0 ? gn : -1, first, ncl, R);
 }
 
 // Banks of one coarse level, one wave each: z = Binv R and the next level's
-// (and with up2 the level after's) sums.
+// sums.
 template <typename T>
 __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
                                                const T* __restrict__ inv,
                                                const int* __restrict__ going_next,
                                                const int* __restrict__ bank_first,
                                                const int* __restrict__ bank_ncl, T* __restrict__ R,
-                                               T* __restrict__ Z, int has_next, int up2,
+                                               T* __restrict__ Z, int has_next,
                                                const int* __restrict__ done)
 {
     if(done && *done)
@@ -715,18 +646,19 @@ __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
     T         z     = bank_product(inv, gb, rv);
     Z[(size_t)gb * ZB + t] = z;
     if(has_next)
-        restrict_bank(Rc, rv, valid ? going_next[g] : -1, bank_first[gb], bank_ncl[gb],
-                      going_next, R, up2 != 0);
+        restrict_bank(Rc, rv, valid ? going_next[g] : -1, bank_first[gb], bank_ncl[gb], R);
 }
 
 // The coarse levels left after the separate launches, in one launch.
 // Workgroups [0, nwg_b) give one wave to each bank of level lb (R complete
 // from earlier launches, no restriction needed).  The remaining workgroups
-// (when lt < L) own the levels from lt up: each loads R_lt (complete: stored
-// by the level lt-1 kernel, or accumulated by the level lt-2 kernel and
-// cleared by the collect kernel afterwards), sums the higher levels' R in
-// LDS along goingNext, and gives one wave to each of its share of the banks
-// of those levels.
+// (when lt < L) own the levels from lt up: each forms R_lt (loaded when the
+// level lt-1 kernel stored it; summed from R_lb when level lb = lt - 1 runs
+// in this launch), sums the higher levels' R in LDS, and gives one wave to
+// each of its share of the banks of those levels.  Every cluster sum adds
+// the cluster's member nodes in node order: the members of cluster p of
+// level l+1 are the nodes of the one level-l bank whose cluster range
+// [bank_first, bank_first + bank_ncl) holds p, whose parent is p.
 constexpr int kCoarseWaves = 16;
 constexpr int kTailBanks   = 64;    // a level with at most this many banks may start the tail
 constexpr int kTailNodes   = 2048;  // node span of the tail levels (a multiple of 16)
@@ -736,11 +668,28 @@ struct LevelTable
     int off[24];
 };
 
+// the level-l bank (global id) whose cluster range holds next-level node p
+__device__ __forceinline__ int member_bank(const LevelTable& lv, int l,
+                                           const int* __restrict__ bank_first, int p)
+{
+    int lo = lv.off[l] / BANK, hi = (lv.off[l] + lv.count[l] + BANK - 1) / BANK - 1;
+    while(lo < hi)
+    {
+        int mid = (lo + hi + 1) >> 1;
+        if(bank_first[mid] <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64 * kCoarseWaves)
     k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt,
                  const T* __restrict__ inv, const int* __restrict__ going_next,
-                 T* __restrict__ R, T* __restrict__ Z, const int* __restrict__ done)
+                 const int* __restrict__ bank_first, T* __restrict__ R, T* __restrict__ Z,
+                 const int* __restrict__ done)
 {
     if(done && *done)
         return;
@@ -777,18 +726,62 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
     T         pv[BD];
     bank_load(inv, b0 + (bi < nb ? bi : 0), pv);
     const int nc = lv.count[lt];
-    for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
-        Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
+    if(lb >= 0)
+    {
+        // R_lt summed from R_lb (level lb's banks run in this launch)
+        for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
+            if(i >= 3 * nc)
+                Rs[i] = (T)0;
+        const int e0 = lv.off[lb] + lv.count[lb];
+        for(int q = threadIdx.x; q < nc; q += blockDim.x)
+        {
+            const int p  = base + q;
+            const int gb = member_bank(lv, lb, bank_first, p);
+            T         s0 = 0, s1 = 0, s2 = 0;
+            for(int u = 0; u < BANK; u++)
+            {
+                const int g = gb * BANK + u;
+                if(g < e0 && going_next[g] == p)
+                {
+                    s0 += R[3 * (size_t)g];
+                    s1 += R[3 * (size_t)g + 1];
+                    s2 += R[3 * (size_t)g + 2];
+                }
+            }
+            Rs[3 * q]     = s0;
+            Rs[3 * q + 1] = s1;
+            Rs[3 * q + 2] = s2;
+        }
+    }
+    else
+    {
+        for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
+            Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
+    }
     for(int i = threadIdx.x; i < span; i += blockDim.x)
         up[i] = going_next[base + i];
     __syncthreads();
     for(int l = lt; l + 1 < L; l++)
     {
-        const int n0 = lv.off[l] - base, nl = lv.count[l];
-        for(int i = threadIdx.x; i < 3 * nl; i += blockDim.x)
+        const int e0 = lv.off[l] + lv.count[l], n1 = lv.off[l + 1];
+        for(int q = threadIdx.x; q < lv.count[l + 1]; q += blockDim.x)
         {
-            int g = n0 + i / 3;
-            atomicAdd(&Rs[3 * (up[g] - base) + i % 3], Rs[3 * g + i % 3]);
+            const int p  = n1 + q;
+            const int gb = member_bank(lv, l, bank_first, p);
+            T         s0 = 0, s1 = 0, s2 = 0;
+            for(int u = 0; u < BANK; u++)
+            {
+                const int g = gb * BANK + u;
+                if(g < e0 && up[g - base] == p)
+                {
+                    s0 += Rs[3 * (g - base)];
+                    s1 += Rs[3 * (g - base) + 1];
+                    s2 += Rs[3 * (g - base) + 2];
+                }
+            }
+            Rs[3 * (p - base)]     = s0;
+            Rs[3 * (p - base) + 1] = s1;
+            Rs[3 * (p - base) + 2] = s2;
         }
         __syncthreads();
     }
@@ -816,14 +809,12 @@ template <typename T, typename V, int NL>
 __global__ void __launch_bounds__(256)
     k_mas_collect(int nv, int L, const int* __restrict__ slot_of, const int* __restrict__ chain,
                   const T* __restrict__ Z, const V* __restrict__ r, V* __restrict__ z,
-                  double* rz_out, double* zero0, double* zero1, T* __restrict__ clr, int nclr,
+                  double* rz_out, double* zero0, double* zero1,
                   const int* __restrict__ done)
 {
     if(done && *done)
         return;
     int    v = blockIdx.x * blockDim.x + threadIdx.x;
-    if(v < nclr)
-        clr[v] = (T)0;  // R of a level accumulated by atomics, for the next apply
     double d = 0.0;
     if(v < nv)
     {
@@ -1042,7 +1033,7 @@ void Mas::build(const SysView& S, hipStream_t s)
     }
     assemble(S, s);
     invert(s);
-    // R of a level fed by atomics from two levels below must start at zero
+    // every R entry is stored before it is read; zeroed once all the same
     if(cfg_.fp32)
         R32_.zero(s);
     else
@@ -1061,9 +1052,9 @@ void Mas::assemble(const SysView& S, hipStream_t s)
                        A.rows.data(), A.cols.data(), A.vals.data(),
                        coo ? coo->count.data() : nullptr, coo ? coo->cap : 0,
                        coo ? coo->r.data() : nullptr, coo ? coo->c.data() : nullptr,
-                       coo ? coo->v.data() : nullptr, L_, nv_,
-                       slot_of_.data(), chain_.data(), B_.data());
+                       coo ? coo->v.data() : nullptr, slot_of_.data(), B_.data());
     SG_LAUNCH_CHECK();
+    galerkin_.run(A, coo, L_, h_lvl_, chain_.data(), going_next_.data(), B_.data(), s);
 }
 
 void Mas::bank_host(int bank, double* mat, double* inv, hipStream_t s) const
@@ -1140,14 +1131,11 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         }
         lb = (lt >= 2) ? lt - 1 : -1;
     }
-    const int up2_level = (lb >= 1 && lt < L) ? lb - 1 : -1;  // level whose kernel feeds R_lt
-    T*        clr       = nullptr;                            // R_lt, cleared by collect
-    int       nclr      = 0;
     auto ev = prof ? prof->kernel_events("mas_apply_l0")
                    : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
     hipExtLaunchKernelGGL((k_mas_l0<T, V>), dim3(nparts), dim3(64), 0, s, ev.first, ev.second, 0,
                           vert_of_slot, x, r, p, Ap, rz_prev, pap, inv, going_next, bank_first,
-                          bank_ncl, R, Z, L > 1 ? 1 : 0, up2_level == 0 ? 1 : 0, done);
+                          bank_ncl, R, Z, L > 1 ? 1 : 0, done);
     SG_LAUNCH_CHECK();
     if(L > 1)
     {
@@ -1156,8 +1144,7 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
             int count = lvl[2 * l], off = lvl[2 * l + 1];
             int nbk   = (count + BANK - 1) / BANK;
             hipLaunchKernelGGL(k_mas_lx<T>, dim3(nbk), dim3(64), 0, s, off / BANK, off, count, inv,
-                               going_next, bank_first, bank_ncl, R, Z, 1,
-                               up2_level == l ? 1 : 0, done);
+                               going_next, bank_first, bank_ncl, R, Z, 1, done);
             SG_LAUNCH_CHECK();
         }
         SG_CHECK(L <= 24, -1, "MAS: more than 24 levels");
@@ -1181,19 +1168,15 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         if(grid > 0)
         {
             hipLaunchKernelGGL(k_mas_coarse<T>, dim3(grid), dim3(64 * kCoarseWaves), 0, s, lv, L,
-                               lb, nwg_b, lt, inv, going_next, R, Z, done);
+                               lb, nwg_b, lt, inv, going_next, bank_first, R, Z, done);
             SG_LAUNCH_CHECK();
         }
-        if(up2_level >= 0)
-        {
-            clr  = R + 3 * (size_t)lvl[2 * lt + 1];
-            nclr = 3 * lvl[2 * lt];
-        }
     }
-    SG_CHECK(nclr <= (long long)grid_for(nv, 256) * 256, -1, "MAS: tail level larger than the vertex count");
+    SG_CHECK(grid_for(nv, 256) <= (unsigned)kSetMaxBlocks, SGIPC_ECAPACITY,
+             "MAS: system too large for the fixed-order reductions");
     auto collect = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, L, slot_of, chain,
-                           Z, r, z, rz_out, zero0, zero1, clr, nclr, done);
+                           Z, r, z, rz_out, zero0, zero1, done);
     };
     static_assert(kChainFast == 8, "collect dispatch below covers NL = 0 .. 8");
     switch(L - 1)
@@ -1210,6 +1193,8 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
     default: collect(k_mas_collect<T, V, -1>); break;
     }
     SG_LAUNCH_CHECK();
+    if(rz_out)
+        set_finalize(rz_out, (int)grid_for(nv, 256), done, s);
 }
 
 template <typename V>

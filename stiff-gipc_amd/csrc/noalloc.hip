@@ -28,9 +28,10 @@ namespace
 {
 constexpr int    kMaxDevices = 64;
 constexpr size_t kMaxDofs    = 3 * (size_t)SGIPC_NOALLOC_MAX_ROWS;
-// b, x and the solver workspace of the largest system, with margin for the
-// 256-byte rounding of each region
-constexpr size_t kScratchDoubles = 14 * kMaxDofs + 4096;
+// b, x and the solver workspace of the largest system (12 reduction sets,
+// vectors, block inverses), with margin for the 256-byte rounding of each
+// region
+constexpr size_t kScratchDoubles = 14 * kMaxDofs + 16 * (size_t)kSlotSet + 4096;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
 

@@ -12,6 +12,8 @@
 // inverse bit for bit.
 #include "linsys.h"
 
+#include "../../include/sgipc.h"
+
 namespace sgipc
 {
 
@@ -157,9 +159,13 @@ void launch_update_body(int nbody, int first, V* x, V* r, const V* p, const V* A
 {
     if(nbody < 1)
         return;
+    SG_CHECK(grid_for(nbody, 256) <= (unsigned)kSetMaxBlocks, SGIPC_ECAPACITY,
+             "too many affine bodies for the fixed-order reductions");
     hipLaunchKernelGGL(k_update_body<V>, dim3(grid_for(nbody, 256)), dim3(256), 0, s, nbody,
                        first, x, r, p, Ap, rz_prev, pap, binv, z, rz_out, done);
     SG_LAUNCH_CHECK();
+    if(rz_out)
+        set_finalize(rz_out, (int)grid_for(nbody, 256), done, s);
 }
 
 template void launch_update_body<double>(int, int, double*, double*, const double*,

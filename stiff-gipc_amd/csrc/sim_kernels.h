@@ -167,6 +167,16 @@ void launch_update_velocity(const SimDev& d, double* xprev, double* v, hipStream
 void   radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int lo, int hi,
                        hipStream_t s);
 size_t radix_sort_hist_size(int n);
+// ... of (key, 32-bit value) pairs: the values move with their keys (stable)
+void radix_sort_pairs(uint64_t* keys, uint64_t* kalt, uint32_t* vals, uint32_t* valt,
+                      unsigned* hist, int n, int lo, int hi, hipStream_t s);
+// the digit-rounded bit count that covers keys up to max_key
+int radix_key_bits(uint64_t max_key);
+
+// mas_galerkin.hip: exclusive scan of n unsigned values into out; tsum holds
+// scan_tmp_size(n) entries, the total ends in tsum[scan_tmp_size(n) - 1]
+void   scan_exclusive_u32(int n, const unsigned* in, unsigned* out, unsigned* tsum, hipStream_t s);
+size_t scan_tmp_size(int n);
 
 // spatial_sort.hip: sorted spatial keys and orders of vertices (kind 0) or tets (1)
 void spatial_sort_keys(int kind, const double* x, int nv, const uint32_t* tets, int nt,
@@ -192,6 +202,7 @@ struct Lbvh
 };
 
 // contact.hip
+
 void launch_query_proximity(const SimDev& d, const Lbvh& bf, const Lbvh& be, const double* xrest,
                             hipStream_t s);
 // swept candidates into cand (count in *ccount, capacity cap)
