@@ -409,20 +409,40 @@ __device__ __forceinline__ void slot_set_add(double* set, double v)
     }
 }
 
-// set_finalize's body: one workgroup of 256 threads sums the nblocks partials
-// (thread t: partials t, t + 256, ... in order; then the waves in order)
+// set_finalize's body: one workgroup of 1024 threads sums the nblocks
+// partials: thread t adds partials t + 1024 j in four interleaved chains
+// (j mod 4), whose loads are all issued before the adds, then the chains,
+// the lanes (butterfly) and the waves, each in a fixed order
+constexpr int kFinalizeThreads = 1024;
 __device__ __forceinline__ void slot_set_finalize(double* set, int nblocks)
 {
-    __shared__ double part[4];
-    double            t = 0.0;
-    for(int i = threadIdx.x; i < nblocks; i += 256)
-        t += set[kSetOffBlock + i];
-    t = wave_sum(t);
+    __shared__ double part[kFinalizeThreads / 64];
+    const double*     p = set + kSetOffBlock;
+    double            c[4] = {0.0, 0.0, 0.0, 0.0};
+    for(int b = 0; b < nblocks; b += 4 * kFinalizeThreads)
+    {
+        double v[4];
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+        {
+            const int i = b + q * kFinalizeThreads + (int)threadIdx.x;
+            v[q]        = i < nblocks ? p[i] : 0.0;
+        }
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+            c[q] += v[q];
+    }
+    double t = wave_sum((c[0] + c[1]) + (c[2] + c[3]));
     if(__lane_id() == 0)
         part[threadIdx.x >> 6] = t;
     __syncthreads();
     if(threadIdx.x == 0)
-        set[0] = set[0] + (((part[0] + part[1]) + part[2]) + part[3]);
+    {
+        double tot = 0.0;
+        for(int w = 0; w < kFinalizeThreads / 64; w++)
+            tot += part[w];
+        set[0] = set[0] + tot;
+    }
 }
 
 // clear the result slots of one set (kSlots threads of any workgroup)

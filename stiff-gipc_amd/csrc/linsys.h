@@ -222,6 +222,7 @@ class Mas
     DBuf<int>      going_next_; // global id -> global id one level up
     DBuf<int>      bank_first_; // global bank -> first next-level global id
     DBuf<int>      bank_ncl_;   // global bank -> number of next-level nodes
+    DBuf<int>      child_bank_; // global id (levels >= 1) -> bank holding its members
     size_t         nbanks_alloc_ = 0;
     MasGalerkin    galerkin_;
 };

@@ -634,8 +634,8 @@ void dot_slots(int n, const double* a, const double* b, double* out, hipStream_t
     dot_slots_ctl<double>(n, a, b, out, nullptr, s);
 }
 
-__global__ void __launch_bounds__(256) k_set_finalize(double* set, int nblocks,
-                                                      const int* __restrict__ done)
+__global__ void __launch_bounds__(kFinalizeThreads) k_set_finalize(double* set, int nblocks,
+                                                                   const int* __restrict__ done)
 {
     if(done && *done)
         return;
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256) k_set_finalize(double* set, int nblocks,
 
 void set_finalize(double* set, int nblocks, const int* done, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_set_finalize, dim3(1), dim3(256), 0, s, set, nblocks, done);
+    hipLaunchKernelGGL(k_set_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, set, nblocks, done);
     SG_LAUNCH_CHECK();
 }
 

@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(256) k_assign(int level, int total_map, const 
                                                 const int* __restrict__ bank_prefix,
                                                 const int* __restrict__ bank_cnt,
                                                 int* __restrict__ newid, int* __restrict__ going_next,
-                                                int* __restrict__ bank_first, int* __restrict__ bank_ncl)
+                                                int* __restrict__ bank_first, int* __restrict__ bank_ncl,
+                                                int* __restrict__ child_bank)
 {
     int  t      = blockIdx.x * blockDim.x + threadIdx.x;
     int  lane16 = t & 15;
@@ -250,6 +251,9 @@ __global__ void __launch_bounds__(256) k_assign(int level, int total_map, const 
     int id   = bank_prefix[bank] + __popc(grp & ((1u << e) - 1u));
     newid[t] = id;
     going_next[off_l + t] = off_n + id;
+    // the bank holding the members of the new node (every member stores the
+    // same value)
+    child_bank[off_n + id] = (off_l >> 4) + bank;
 }
 
 // per fine vertex: node id at level l+1 and its chain entry
@@ -657,8 +661,8 @@ __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
 // in this launch), sums the higher levels' R in LDS, and gives one wave to
 // each of its share of the banks of those levels.  Every cluster sum adds
 // the cluster's member nodes in node order: the members of cluster p of
-// level l+1 are the nodes of the one level-l bank whose cluster range
-// [bank_first, bank_first + bank_ncl) holds p, whose parent is p.
+// level l+1 are the nodes of the level-l bank child_bank[p] whose parent is
+// p.
 constexpr int kCoarseWaves = 16;
 constexpr int kTailBanks   = 64;    // a level with at most this many banks may start the tail
 constexpr int kTailNodes   = 2048;  // node span of the tail levels (a multiple of 16)
@@ -668,27 +672,11 @@ struct LevelTable
     int off[24];
 };
 
-// the level-l bank (global id) whose cluster range holds next-level node p
-__device__ __forceinline__ int member_bank(const LevelTable& lv, int l,
-                                           const int* __restrict__ bank_first, int p)
-{
-    int lo = lv.off[l] / BANK, hi = (lv.off[l] + lv.count[l] + BANK - 1) / BANK - 1;
-    while(lo < hi)
-    {
-        int mid = (lo + hi + 1) >> 1;
-        if(bank_first[mid] <= p)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(64 * kCoarseWaves)
     k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt,
                  const T* __restrict__ inv, const int* __restrict__ going_next,
-                 const int* __restrict__ bank_first, T* __restrict__ R, T* __restrict__ Z,
+                 const int* __restrict__ child_bank, T* __restrict__ R, T* __restrict__ Z,
                  const int* __restrict__ done)
 {
     if(done && *done)
@@ -716,8 +704,9 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
     // bank inverse (each wave owns at most one tail bank: the launch gives
     // ceil(banks / kCoarseWaves) tail workgroups), so the chain is one
     // memory round trip, the LDS restriction, and the product.
-    __shared__ T   Rs[3 * kTailNodes];
-    __shared__ int up[kTailNodes];
+    __shared__ T             Rs[3 * kTailNodes];
+    __shared__ int           up[kTailNodes];
+    __shared__ unsigned char cb[kTailNodes];  // member bank of a node above lt, minus b0
     const int base = lv.off[lt];
     const int span = lv.off[L - 1] + lv.count[L - 1] - base;
     const int nb   = (span + BANK - 1) / BANK;  // banks of the tail levels
@@ -732,20 +721,34 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
             if(i >= 3 * nc)
                 Rs[i] = (T)0;
+        // the bank's 16 parents, then its 48 values, each as one batch of
+        // independent loads (addresses of nodes past the level's end fall
+        // back to the bank's first node, which always exists; their values
+        // are not used)
         const int e0 = lv.off[lb] + lv.count[lb];
         for(int q = threadIdx.x; q < nc; q += blockDim.x)
         {
             const int p  = base + q;
-            const int gb = member_bank(lv, lb, bank_first, p);
-            T         s0 = 0, s1 = 0, s2 = 0;
+            const int gb = child_bank[p];
+            int       par[BANK];
+#pragma unroll
             for(int u = 0; u < BANK; u++)
             {
                 const int g = gb * BANK + u;
-                if(g < e0 && going_next[g] == p)
+                par[u]      = g < e0 ? going_next[g] : -1;
+            }
+            T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+            for(int u = 0; u < BANK; u++)
+            {
+                const int    g  = gb * BANK + u;
+                const size_t gg = 3 * (size_t)(g < e0 ? g : gb * BANK);
+                const T      r0 = R[gg], r1 = R[gg + 1], r2 = R[gg + 2];
+                if(par[u] == p)
                 {
-                    s0 += R[3 * (size_t)g];
-                    s1 += R[3 * (size_t)g + 1];
-                    s2 += R[3 * (size_t)g + 2];
+                    s0 += r0;
+                    s1 += r1;
+                    s2 += r2;
                 }
             }
             Rs[3 * q]     = s0;
@@ -758,8 +761,13 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
             Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
     }
+    const int above = (lt + 1 < L) ? lv.off[lt + 1] - base : span;
     for(int i = threadIdx.x; i < span; i += blockDim.x)
+    {
         up[i] = going_next[base + i];
+        if(i >= above)
+            cb[i] = (unsigned char)(child_bank[base + i] - b0);
+    }
     __syncthreads();
     for(int l = lt; l + 1 < L; l++)
     {
@@ -767,7 +775,7 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         for(int q = threadIdx.x; q < lv.count[l + 1]; q += blockDim.x)
         {
             const int p  = n1 + q;
-            const int gb = member_bank(lv, l, bank_first, p);
+            const int gb = b0 + cb[p - base];
             T         s0 = 0, s1 = 0, s2 = 0;
             for(int u = 0; u < BANK; u++)
             {
@@ -946,6 +954,7 @@ void Mas::init(const Bsr& A, const int* part_id, const MasConfig& cfg, hipStream
     lvl_.resize(2 * (L_ + 2));
     size_t bound = (size_t)L_ * (total_map_ + BANK) + BANK;
     going_next_.resize(bound);
+    child_bank_.resize(bound);
     bank_first_.resize(bound / BANK + 1);
     bank_ncl_.resize(bound / BANK + 1);
     std::vector<int> lvl0(2 * (L_ + 2), 0);
@@ -966,7 +975,8 @@ void Mas::cluster_level(int level, hipStream_t s)
     SG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_assign, dim3(g), dim3(256), 0, s, level, total_map_, lvl_.data(),
                        vert_of_slot_.data(), comp_.data(), bank_prefix_.data(), bank_cnt_.data(),
-                       newid_.data(), going_next_.data(), bank_first_.data(), bank_ncl_.data());
+                       newid_.data(), going_next_.data(), bank_first_.data(), bank_ncl_.data(),
+                       child_bank_.data());
     SG_LAUNCH_CHECK();
     if(level + 1 <= L_ - 1)
     {
@@ -1108,7 +1118,7 @@ template <typename T, typename V>
 static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::vector<int>& lvl,
                         const int* vert_of_slot, const int* slot_of, const int* chain,
                         const int* going_next, const int* bank_first, const int* bank_ncl,
-                        const T* inv, T* R, T* Z, V* x, V* r, const V* p,
+                        const int* child_bank, const T* inv, T* R, T* Z, V* x, V* r, const V* p,
                         const V* Ap, const double* rz_prev, const double* pap, V* z,
                         double* rz_out, double* zero0, double* zero1, const int* done,
                         hipStream_t s)
@@ -1168,7 +1178,7 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
         if(grid > 0)
         {
             hipLaunchKernelGGL(k_mas_coarse<T>, dim3(grid), dim3(64 * kCoarseWaves), 0, s, lv, L,
-                               lb, nwg_b, lt, inv, going_next, bank_first, R, Z, done);
+                               lb, nwg_b, lt, inv, going_next, child_bank, R, Z, done);
             SG_LAUNCH_CHECK();
         }
     }
@@ -1206,13 +1216,15 @@ void Mas::apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_pr
     if(cfg_.fp32)
         mas_apply_t<float, V>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(),
                               slot_of_.data(), chain_.data(), going_next_.data(),
-                              bank_first_.data(), bank_ncl_.data(), inv32_.data(), R32_.data(),
+                              bank_first_.data(), bank_ncl_.data(), child_bank_.data(),
+                              inv32_.data(), R32_.data(),
                               Z32_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
                               done, s);
     else
         mas_apply_t<double, V>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(),
                                slot_of_.data(), chain_.data(), going_next_.data(),
-                               bank_first_.data(), bank_ncl_.data(), inv64_.data(), R64_.data(),
+                               bank_first_.data(), bank_ncl_.data(), child_bank_.data(),
+                               inv64_.data(), R64_.data(),
                                Z64_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
                                done, s);
 }
