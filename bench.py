@@ -150,8 +150,15 @@ def main():
     ap.add_argument("--train-iters", type=int, default=12,
                     help="Newton solves whose coarse systems train the network")
     ap.add_argument("--train-epochs", type=int, default=400)
+    ap.add_argument("--train-crop", default=None,
+                    help="cubes workload: collect the training solves on a crop of the scene, "
+                         "NA,NB,NC cubes along x, y (vertical), z (e.g. 4,2,4: 32 cubes), "
+                         "settled for the same number of steps, instead of the scene itself")
     args = ap.parse_args()
 
+    if args.train_crop and args.workload != "cubes64":
+        print("bench.py: --train-crop applies to the cubes64 workload", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU decomposition is design-only", file=sys.stderr)
@@ -198,25 +205,41 @@ def main():
     train = None
     if args.modes != "ref" and args.train_iters > 0:
         t0 = time.time()
-        sim.set_pcg(rel_tol=1e-8, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
-        sim.learned_mode(1)
-        sim.warm_start_adaptive(False)
+        tsim = sim
+        if args.train_crop:
+            crop = tuple(int(v) for v in args.train_crop.split(","))
+            csc = S.cubes_scene(n_side=args.n_side, cells=args.cells, size=0.2, spacing=0.6,
+                                fall=0.5, seed=0, crop=crop)
+            csc.mas_precision = sc.mas_precision
+            tsim = sgipc.Simulation(csc)
+            for _ in range(args.settle):
+                tsim.time_step()
+            log(f"training crop {crop}: {csc.n_verts} verts, {csc.n_tets} tets, settled "
+                f"{args.settle} steps ({time.time() - t0:.1f} s)")
+        tsim.set_pcg(rel_tol=1e-8, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
+        tsim.learned_mode(1)
+        tsim.warm_start_adaptive(False)
         F, T = [], []
         for _ in range(args.train_iters):
-            sim.advance(1)
-            f, t = sim.learned_samples()
+            tsim.advance(1)
+            f, t = tsim.learned_samples()
             F.append(f)
             T.append(t)
         F, T = np.concatenate(F), np.concatenate(T)
         t1 = time.time()
-        loss = sim.mlp_train(F, T, epochs=args.train_epochs, lr=1e-3)
+        loss = tsim.mlp_train(F, T, epochs=args.train_epochs, lr=1e-3)
         t2 = time.time()
+        if tsim is not sim:
+            sim.mlp_weights(tsim.mlp_weights())
+            tsim.close()
         sim.learned_mode(0)
         sim.warm_start_adaptive(True)
         train = {"samples": int(F.shape[0]), "collect_s": round(t1 - t0, 2),
                  "train_s": round(t2 - t1, 2), "epochs": args.train_epochs,
                  "loss_untrained": float((T.astype(np.float64) ** 2).mean()),
                  "loss_trained": loss}
+        if args.train_crop:
+            train["crop"] = args.train_crop
         log(f"learned correction: {train}")
 
     ref = dict(rel_tol=1e-4, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
@@ -260,6 +283,7 @@ def main():
              "ccd_limited_iterations": adv["ccd_limited"]}
         if lmode:
             r["warm_starts"] = adv["warm_starts"]
+            r["warm_fallbacks"] = adv["warm_fallbacks"]
         results[name] = r
         advs[name] = (adv, ms)
         log(f"timed [{name}]: {n} Newton iterations in {(t1 - t0):.3f} s -> {ms:.3f} ms/iter; "

@@ -276,7 +276,9 @@ typedef struct sgipc_iter_stats {
     int32_t solver_fallback;  /* 1: the mixed solve was redone in FP64 */
     int32_t warm_start;       /* solve continued from the affine (1) or the
                                  network-corrected (2) coarse guess        */
-    int32_t reserved;
+    int32_t warm_fallback;    /* 1: the warm-started solve was abandoned at
+                                 twice the recent zero-guess PCG iteration
+                                 count and solved again from zero          */
     double  pcg_rel_residual; /* FP64 ||b-Ax||/||b||                   */
     double  alpha;            /* accepted step                         */
     double  alpha_ccd;        /* CCD / ground bound before line search */
@@ -323,6 +325,9 @@ typedef struct sgipc_advance_stats {
     double  ms_assembly, ms_solve, ms_ccd, ms_line_search;
     int32_t line_search_steps; /* Armijo backtracking steps, summed         */
     int32_t ccd_limited;       /* iterations whose CCD/CFL bound was < 1     */
+    int32_t warm_fallbacks;    /* warm-started solves abandoned (see
+                                  sgipc_iter_stats.warm_fallback)           */
+    int32_t pad;
 } sgipc_advance_stats;
 
 /* Advance the integration by exactly n Newton iterations that solve (the

@@ -161,9 +161,13 @@ class Context
     // judged and their averaged net gain in ms
     int          warm_adaptive_ = 1, warm_skip_ = 0, warm_pause_ = 0, warm_probes_ = 0;
     double       warm_gain_     = 0.0;
+    // PCG iterations of the recent zero-guess MAS solves (averaged with
+    // weight 1/2): a warm-started solve is abandoned at twice this count
+    double       plain_iters_   = 0.0;
     DBuf<double> x0_, b2_, x1_, b3_, zt_;
     PcgStats     solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
-                              int* warm);
+                              int* warm, int* warm_fallback);
+    PcgStats     solve_plain(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol);
 
     Bsr   A_;
     Coo   coo_;

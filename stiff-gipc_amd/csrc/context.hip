@@ -1248,7 +1248,7 @@ void Context::newton_iteration(sgipc_iter_stats* st)
     cfg.use_rz       = desc_.pcg_use_preconditioned_norm != 0;
     cfg.mixed        = desc_.solver_precision == SGIPC_SOLVE_MIXED;
     double*  sol     = nabd_ ? dy_.data() : dx_.data();
-    PcgStats ps      = solve_linear(S, pc, cfg, sol, &loc.warm_start);
+    PcgStats ps      = solve_linear(S, pc, cfg, sol, &loc.warm_start, &loc.warm_fallback);
     SG_CHECK(ps.rel_residual == ps.rel_residual, SGIPC_ENUMERIC,
              "the linear solve produced a non-finite direction");
     if(nabd_)  // vertex displacements J dq of the affine-body vertices
@@ -1289,7 +1289,9 @@ void Context::newton_iteration(sgipc_iter_stats* st)
 // affine guess and, in mode 2, the network-corrected one are measured by
 // their preconditioned residuals, the lower one is used when it beats the
 // zero guess, and the zero-guess solve runs whenever the warm solve does not
-// converge.
+// converge.  The warm solve is abandoned once it has run twice the PCG
+// iterations that the recent zero-guess solves needed (their average with
+// weight 1/2); its iterations are counted in the solve's total.
 //
 // Adaptive mode (the default): the guesses cost a Galerkin build, a coarse
 // solve and one MAS apply and residual per guess, timed on the device.  From
@@ -1301,14 +1303,23 @@ void Context::newton_iteration(sgipc_iter_stats* st)
 // while that average is negative the warm start is paused for the next 2
 // solves, the pause doubling (up to 32) while it stays negative.  Occasional
 // rejected guesses among paying ones therefore do not pause it.
-PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
-                               int* warm)
+PcgStats Context::solve_plain(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol)
 {
-    *warm = 0;
+    PcgStats ps = pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+    if(ps.converged)
+        plain_iters_ = plain_iters_ > 0.0 ? 0.5 * (plain_iters_ + ps.iterations) : ps.iterations;
+    return ps;
+}
+
+PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol,
+                               int* warm, int* warm_fallback)
+{
+    *warm          = 0;
+    *warm_fallback = 0;
     if(learned_ && pc == Precond::Mas && warm_adaptive_ && warm_skip_ > 0)
     {
         warm_skip_--;
-        return pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+        return solve_plain(S, pc, cfg, sol);
     }
     if(learned_ && pc == Precond::Mas && coarse_.setup(mas_, s_))
     {
@@ -1383,11 +1394,24 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
             else
                 warm_pause_ = 0;
         };
+        int wasted = 0;
         if(h[0] > 0.0 && qb < h[0])
         {
             PcgConfig c = cfg;
             c.q_ref     = h[0];
+            // abandon the guess at twice the iterations the recent solves
+            // from zero needed
+            if(plain_iters_ > 0.0)
+            {
+                int cap    = std::max(2 * (int)std::ceil(plain_iters_), 2 * cfg.check_every);
+                c.max_iter = cfg.max_iter > 0 ? std::min(cfg.max_iter, cap) : cap;
+            }
             PcgStats ps = pcg_.solve(S, pc, &mas_, bg, sol, c, s_);
+            if(!ps.converged)
+            {
+                wasted         = ps.iterations;
+                *warm_fallback = 1;
+            }
             if(ps.converged)
             {
                 launch_axpy_d((int)n, 1.0, xg, sol, s_);
@@ -1403,11 +1427,12 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
                 return ps;
             }
         }
-        PcgStats ps = pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+        PcgStats ps = solve_plain(S, pc, cfg, sol);
+        ps.iterations += wasted;
         judge(0.0);
         return ps;
     }
-    return pcg_.solve(S, pc, &mas_, rhs_.data(), sol, cfg, s_);
+    return solve_plain(S, pc, cfg, sol);
 }
 
 void Context::set_warm_adaptive(int on)
@@ -1556,6 +1581,7 @@ void Context::advance(int n, sgipc_advance_stats* st)
         loc.pcg_iterations += is.pcg_iterations;
         loc.solver_fallbacks += is.solver_fallback;
         loc.warm_starts += is.warm_start > 0 ? 1 : 0;
+        loc.warm_fallbacks += is.warm_fallback;
         loc.max_pcg_rel_residual = std::max(loc.max_pcg_rel_residual, is.pcg_rel_residual);
         loc.ms_assembly += is.ms_assembly;
         loc.ms_solve += is.ms_solve;

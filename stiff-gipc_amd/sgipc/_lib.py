@@ -44,7 +44,7 @@ class IterStats(C.Structure):
     _fields_ = [("pcg_iterations", C.c_int32), ("n_pairs", C.c_int32), ("n_ground", C.c_int32),
                 ("line_search_steps", C.c_int32), ("converged", C.c_int32),
                 ("solver_fallback", C.c_int32), ("warm_start", C.c_int32),
-                ("reserved", C.c_int32),
+                ("warm_fallback", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("alpha", C.c_double),
                 ("alpha_ccd", C.c_double), ("xdelta", C.c_double), ("energy0", C.c_double),
                 ("energy1", C.c_double), ("ms_assembly", C.c_double), ("ms_solve", C.c_double),
@@ -52,7 +52,7 @@ class IterStats(C.Structure):
                 ("ms_total", C.c_double)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f not in ("pad", "reserved")}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f not in ("pad",)}
 
 
 class StepStats(C.Structure):
@@ -67,7 +67,7 @@ class AdvanceStats(C.Structure):
                 ("warm_starts", C.c_int32), ("capped_steps", C.c_int32),
                 ("ms_assembly", C.c_double), ("ms_solve", C.c_double), ("ms_ccd", C.c_double),
                 ("ms_line_search", C.c_double), ("line_search_steps", C.c_int32),
-                ("ccd_limited", C.c_int32)]
+                ("ccd_limited", C.c_int32), ("warm_fallbacks", C.c_int32), ("pad", C.c_int32)]
 
 
 class SgipcError(RuntimeError):

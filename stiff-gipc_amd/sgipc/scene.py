@@ -448,10 +448,12 @@ def bar_scene(cells=(16, 4, 4), h=0.05, drop=0.1, E=1e5, nu=0.49, rho=1000.0, **
 
 
 def cubes_scene(n_side=4, cells=8, size=0.2, spacing=0.6, fall=0.5, seed=0, nu=0.4,
-                rho=1000.0, E_range=(1e5, 1e9), **kw):
+                rho=1000.0, E_range=(1e5, 1e9), crop=None, **kw):
     """n_side^3 tet cubes of cells^3 hex cells placed on a grid with seeded
     +/-15% jitter and uniform random rotations, Young's modulus log-uniform in
-    E_range, lowest cube 'fall' metres above the ground."""
+    E_range, lowest cube 'fall' metres above the ground.  crop = (na, nb, nc)
+    keeps only the cubes at grid index (a < na, b < nb, c < nc) (b vertical),
+    each with the same position, rotation and modulus as in the full scene."""
     rng = np.random.default_rng(seed)
     x0, t0, p0 = hex_grid((cells,) * 3, size / cells)
     x0 = x0 - x0.mean(0)
@@ -465,8 +467,10 @@ def cubes_scene(n_side=4, cells=8, size=0.2, spacing=0.6, fall=0.5, seed=0, nu=0
                 ctr = np.array([a * spacing, fall + half_diag + b * spacing, c * spacing]) + jit
                 ctr[1] = max(ctr[1], fall + half_diag)
                 E = float(np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]))))
+                if crop is not None and not (a < crop[0] and b < crop[1] and c < crop[2]):
+                    continue
                 bodies.append((x0 @ R.T + ctr, t0, p0, E, nu, rho, 0))
-    defaults = dict(dt=0.01, dhat=1e-3, name=f"cubes_{n_side ** 3}x{cells}^3")
+    defaults = dict(dt=0.01, dhat=1e-3, name=f"cubes_{len(bodies)}x{cells}^3")
     defaults.update(kw)
     sc = assemble_bodies(bodies, **defaults)
     sc.abs_xdelta_tol = 1e-2 * sc.dt / _diag(sc.x)

@@ -62,11 +62,11 @@ def test_cloth_newton_iterations_match(oracle):
     iteration counts; step sizes and positions identical to rounding before
     contact.  Once the CCD bound limits the step, additive CCD's stopping
     test on directions that differ by PCG rounding moves alpha by a few 1e-6
-    relative and the mid-step iterate by up to ~1e-7 (GPU runs differ from
-    each other by the same amount: atomic summation order; the cloth's bank
-    matrices are ill-conditioned, so the two sides' bank inverses, eliminated
-    in different orders, differ well above FP64 rounding), so mid-step
-    iterates are held to 1e-5 in alpha and 1e-6 in positions"""
+    relative and the mid-step iterate by up to ~1e-7 (the two sides sum
+    element and contact terms in different orders, so their bank matrices
+    differ in the last bits, and the cloth's ill-conditioned banks amplify
+    that in the inverses), so mid-step iterates are held to 1e-5 in alpha and
+    1e-6 in positions"""
     import sgipc
     sc = _scene(height=0.1022)
     sim = sgipc.Simulation(sc)
@@ -163,3 +163,67 @@ def test_cloth_drape_over_box(oracle):
         assert err < (1e-8 if step < 2 else 5 * _NEWTON_SCALE)
         inside = np.all((xg[cloth] > lo) & (xg[cloth] < hi), axis=1)
         assert not inside.any()
+
+
+def _cloth_over_abd(prec):
+    """reduced cloth-over-affine-boxes configuration: a 12 x 12 sheet (0.24 m,
+    membrane E 1e6, bending 1e-4, thickness 1e-3) dropped onto three free
+    affine boxes (kappa 1e11) on the ground; PCG stops on the FP64 true
+    residual 1e-6 and Newton on |dx|_inf < 1e-8, so each step's end point is
+    the incremental potential's minimiser to well below 1e-6"""
+    import sgipc
+    from sgipc import scene as S
+    sc = S.cloth_scene(n=12, size=0.24, height=0.075, n_boxes=3, box_size=0.06, box_cells=2,
+                       abd_boxes=True, abd_kappa=1e11, seed=1)
+    sc.mas_precision = sgipc.SGIPC_MAS_FP64
+    sc.solver_precision = prec
+    sc.pcg_rel_tol = 1e-6
+    sc.pcg_use_preconditioned_norm = 0
+    sc.abs_xdelta_tol = 1e-8
+    return sc
+
+
+def test_cloth_over_affine_boxes_mixed_tracks_fp64(oracle):
+    """sheet over three affine boxes, 6 time steps (free fall, first contact,
+    draping over the box edges with up to ~170 Newton iterations per step):
+    the FP64 GPU solve against the oracle, and the mixed-precision solve (FP32
+    inner PCG with FP64 residual refinement) against the FP64 GPU solve, both
+    within 1e-8 of the scene scale per step (the Newton stop |dx|_inf < 1e-8
+    bounds how far either end point is from the step's minimiser); the boxes
+    stay rigid and the sheet lands on them"""
+    import sgipc
+    sims = {}
+    for prec in (sgipc.SGIPC_SOLVE_FP64, sgipc.SGIPC_SOLVE_MIXED):
+        sims[prec] = sgipc.Simulation(_cloth_over_abd(prec))
+    sc = _cloth_over_abd(sgipc.SGIPC_SOLVE_FP64)
+    w = oracle.World(sc)
+    abd = sc.abd_body
+    cloth = np.unique(sc.tris)
+    x0 = sc.x.copy()
+    box_top = x0[abd >= 0, 1].max()
+    fell = False
+    for step in range(6):
+        s64 = sims[sgipc.SGIPC_SOLVE_FP64].time_step()
+        s32 = sims[sgipc.SGIPC_SOLVE_MIXED].time_step()
+        so = w.time_step()
+        x64 = sims[sgipc.SGIPC_SOLVE_FP64].positions()
+        x32 = sims[sgipc.SGIPC_SOLVE_MIXED].positions()
+        xo = w.positions()
+        scale = np.abs(xo).max()
+        e_or = np.abs(x64 - xo).max() / scale
+        e_mx = np.abs(x32 - x64).max() / scale
+        print(f"[cloth-abd] step={step} newton={s64['newton_iterations']}/"
+              f"{s32['newton_iterations']}/{so['newton_iterations']} "
+              f"pcg={s64['pcg_iterations']}/{s32['pcg_iterations']}/{so['pcg_iterations']} "
+              f"err_oracle={e_or:.2e} err_mixed={e_mx:.2e} "
+              f"ymin={x64[cloth, 1].min():.5f}", flush=True)
+        assert s64["capped"] == 0 and s32["capped"] == 0
+        assert e_or < 1e-8
+        assert e_mx < 1e-8
+        fell |= x64[cloth, 1].min() < box_top
+    assert fell
+    for x in (x64, x32):
+        for b in range(int(abd.max()) + 1):
+            d0 = np.linalg.norm(x0[abd == b][:, None] - x0[abd == b][None], axis=2)
+            d = np.linalg.norm(x[abd == b][:, None] - x[abd == b][None], axis=2)
+            assert np.abs(d - d0).max() < 1e-6

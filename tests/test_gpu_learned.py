@@ -174,3 +174,68 @@ def test_warm_start_gain(oracle):
     print(f"[warm-adaptive] pcg={p3} dev={e3:.2e}", flush=True)
     assert e3 < 1e-3
     assert p3 <= 1.2 * p2
+
+
+def test_crop_trained_correction_on_a_larger_scene(oracle):
+    """the larger-scene configuration at reduced size: the network is trained
+    on the solves of a 9-cube crop (the bottom layer of a 3 x 3 x 3 grid of
+    cubes, each with its position, rotation and modulus in the full scene)
+    and used on the full 27-cube scene.  Over the fall and landing: the
+    learned mode needs fewer PCG iterations than plain MAS-PCG, every warm
+    solve that runs past twice the recent zero-guess count is abandoned and
+    counted, and the trajectory stays within the solver's tolerance scale
+    (1e-3) of the plain GPU run and of the oracle's"""
+    import sgipc
+    from sgipc import scene as S
+
+    def scene(crop=None):
+        return S.cubes_scene(n_side=3, cells=3, size=0.2, spacing=0.55, fall=0.03, seed=5,
+                             crop=crop)
+
+    steps = 10
+    crop = sgipc.Simulation(scene((3, 1, 3)))
+    crop.set_pcg(rel_tol=1e-8, use_rz=True, solver_precision=sgipc.SGIPC_SOLVE_FP64)
+    crop.learned_mode(1)
+    crop.warm_start_adaptive(False)
+    F, T = [], []
+    for k in range(steps):
+        crop.time_step()
+        f, t = crop.learned_samples()
+        F.append(f)
+        T.append(t)
+    F, T = np.concatenate(F), np.concatenate(T)
+    loss = crop.mlp_train(F, T, epochs=600, lr=1e-3)
+    w = crop.mlp_weights()
+    L0 = float((T.astype(np.float64) ** 2).mean())
+    print(f"[crop-train] samples={F.shape[0]} loss untrained={L0:.4g} trained={loss:.4g}",
+          flush=True)
+    assert loss < 0.1 * L0
+
+    runs = {}
+    for mode in (0, 2):
+        sim = sgipc.Simulation(scene())
+        if mode:
+            sim.mlp_weights(w)
+        sim.learned_mode(mode)
+        sim.warm_start_adaptive(False)
+        pcg = fb = warm = 0
+        for k in range(steps):
+            st = sim.time_step()
+            log = sim.step_log()
+            pcg += st["pcg_iterations"]
+            fb += sum(it["warm_fallback"] for it in log)
+            warm += sum(1 for it in log if it["warm_start"] > 0)
+        runs[mode] = (sim, pcg, fb, warm)
+        print(f"[crop-run] mode={mode} pcg={pcg} warm={warm} fallbacks={fb}", flush=True)
+    w_ = oracle.World(scene())
+    for k in range(steps):
+        w_.time_step()
+    xo = w_.positions()
+    x0 = runs[0][0].positions()
+    x2 = runs[2][0].positions()
+    e_plain = np.abs(x2 - x0).max() / np.abs(x0).max()
+    e_oracle = np.abs(x2 - xo).max() / np.abs(xo).max()
+    print(f"[crop-run] dev plain={e_plain:.2e} oracle={e_oracle:.2e}", flush=True)
+    assert runs[2][3] > 0
+    assert runs[2][1] < runs[0][1]
+    assert e_plain < 1e-3 and e_oracle < 1e-3
