@@ -409,6 +409,33 @@ __device__ __forceinline__ void slot_set_add(double* set, double v)
     }
 }
 
+// the total of a set whose nparts workgroup partials have not been
+// finalized, computed by one wave in a fixed order: lane i loads partials
+// i + 64 j, j < 16, all in one round (1024 partials per round), adds them in
+// four interleaved chains (j mod 4), then the butterfly; every lane of the
+// wave receives it.  A consumer launch can finalize a set this way instead of
+// a set_finalize launch before it.
+__device__ __forceinline__ double slot_set_partials_sum(const double* set, int nparts)
+{
+    const double* p    = set + kSetOffBlock;
+    const int     lane = (int)__lane_id();
+    double        c[4] = {0.0, 0.0, 0.0, 0.0};
+    for(int b = 0; b < nparts; b += 1024)
+    {
+        double v[16];
+#pragma unroll
+        for(int q = 0; q < 16; q++)
+        {
+            const int i = b + 64 * q + lane;
+            v[q]        = i < nparts ? p[i] : 0.0;
+        }
+#pragma unroll
+        for(int q = 0; q < 16; q++)
+            c[q & 3] += v[q];
+    }
+    return wave_sum((c[0] + c[1]) + (c[2] + c[3]));
+}
+
 // set_finalize's body: one workgroup of 1024 threads sums the nblocks
 // partials: thread t adds partials t + 1024 j in four interleaved chains
 // (j mod 4), whose loads are all issued before the adds, then the chains,

@@ -170,13 +170,19 @@ class Mas
     // coarse levels; collect with rz accumulated into the slot set rz_out and
     // the two slot sets zero0/zero1 (may be null) cleared for later use.
     // Every kernel returns at once when done (may be null) is non-zero.
+    // defer_rz: rz_out keeps collect's collect_blocks() workgroup partials
+    // without the finalize launch (its reader sums them)
     void apply_fused(double* x, double* r, const double* p, const double* Ap,
                      const double* rz_prev, const double* pAp, double* z, double* rz_out,
-                     double* zero0, double* zero1, const int* done, hipStream_t s);
+                     double* zero0, double* zero1, const int* done, hipStream_t s,
+                     bool defer_rz = false);
     // the same on FP32 PCG vectors (inner solve of the mixed-precision mode)
     void apply_fused(float* x, float* r, const float* p, const float* Ap,
                      const double* rz_prev, const double* pAp, float* z, double* rz_out,
-                     double* zero0, double* zero1, const int* done, hipStream_t s);
+                     double* zero0, double* zero1, const int* done, hipStream_t s,
+                     bool defer_rz = false);
+    // workgroups of the collect launch (partials of its rz set)
+    int collect_blocks() const { return (int)grid_for(nv_, 256); }
 
     int levels() const { return L_; }
     int total_clusters() const { return total_clusters_; }
@@ -195,7 +201,7 @@ class Mas
     template <typename V>
     void apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
                        const double* pap, V* z, double* rz_out, double* zero0, double* zero1,
-                       const int* done, hipStream_t s);
+                       const int* done, hipStream_t s, bool defer_rz);
     void cluster_level(int level, hipStream_t s);
     void assemble(const SysView& S, hipStream_t s);
     void invert(hipStream_t s);

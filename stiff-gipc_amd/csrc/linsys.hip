@@ -365,15 +365,34 @@ __global__ void __launch_bounds__(256)
            const V* __restrict__ vals, const int* __restrict__ rp2,
            const int* __restrict__ cols2, const V* __restrict__ vals2,
            const V* __restrict__ x, const V* __restrict__ pprev, V* __restrict__ pcur,
-           const V* __restrict__ b, V* __restrict__ y, const double* rz_a, const double* rz_b,
-           double* dot, PcgCtl pc)
+           const V* __restrict__ b, V* __restrict__ y, double* rz_a, const double* rz_b,
+           int rz_parts, double* dot, PcgCtl pc)
 {
     if(pc.ctl && pc.ctl[0])
         return;
+    // rz_parts > 0: the set rz_a holds the preconditioner's rz_parts
+    // workgroup partials, not yet finalized.  One wave per workgroup sums them
+    // in a fixed order (every workgroup gets the same bits) and workgroup 0
+    // stores the total in slot 0 for the launches that follow.
+    double rz_new = 0.0;
+    if(MODE == MODE_PCG && rz_parts > 0)
+    {
+        __shared__ double rz_sh;
+        if(threadIdx.x < 64)
+        {
+            double t = slot_set_partials_sum(rz_a, rz_parts);
+            if(threadIdx.x == 0)
+                rz_sh = t;
+        }
+        __syncthreads();
+        rz_new = rz_sh;
+        if(blockIdx.x == 0 && threadIdx.x == 0)
+            rz_a[0] = rz_new;
+    }
     if(MODE == MODE_PCG && pc.q_set)
     {
-        double q   = slot_set_sum(pc.q_set);
-        double rzc = slot_set_sum(pc.rz_check);
+        double q   = (rz_parts > 0 && pc.q_set == rz_a) ? rz_new : slot_set_sum(pc.q_set);
+        double rzc = (rz_parts > 0 && pc.rz_check == rz_a) ? rz_new : slot_set_sum(pc.rz_check);
         bool   bad = !(rzc == rzc) || !(q == q);
         bool   div = pc.div_thr > 0.0 && q > pc.div_thr;
         if(blockIdx.x == 0 && threadIdx.x == 0)
@@ -431,8 +450,10 @@ __global__ void __launch_bounds__(256)
     V beta = 0;
     if(MODE == MODE_PCG && pprev)
     {
-        double sa = rz_a[__lane_id() * kSlotStride], sb = rz_b[__lane_id() * kSlotStride];
-        beta      = (V)(wave_sum(sa) / wave_sum(sb));  // slot_set_sum of both sets
+        double sa = rz_parts > 0 ? 0.0 : rz_a[__lane_id() * kSlotStride];
+        double sb = rz_b[__lane_id() * kSlotStride];
+        sa        = rz_parts > 0 ? rz_new : wave_sum(sa);
+        beta      = (V)(sa / wave_sum(sb));  // slot_set_sum of both sets
     }
     auto fetch = [&](int j, V& x0, V& x1, V& x2) {
         const V* xj = x + 3 * (size_t)j;
@@ -573,8 +594,9 @@ static inline void contact_rows(const SysView& S, const int*& rp2, const int*& c
 
 template <int MODE, typename V>
 static void launch_spmv(const SysView& S, const V* x, const V* pprev, V* pcur, const V* b, V* y,
-                        const double* rz_a, const double* rz_b, double* dot, const PcgCtl& pc,
-                        hipStream_t s, std::pair<hipEvent_t, hipEvent_t> ev = {nullptr, nullptr})
+                        double* rz_a, const double* rz_b, double* dot, const PcgCtl& pc,
+                        hipStream_t s, std::pair<hipEvent_t, hipEvent_t> ev = {nullptr, nullptr},
+                        int rz_parts = 0)
 {
     const Bsr& A = *S.A;
     const int* rp2;
@@ -586,7 +608,7 @@ static void launch_spmv(const SysView& S, const V* x, const V* pprev, V* pcur, c
     hipExtLaunchKernelGGL((k_spmv<MODE, V>), dim3(grid_for((long long)A.nb * kRowLanes, 256)),
                           dim3(256), 0, s, ev.first, ev.second, 0, A.nb, A.rowptr.data(),
                           A.cols.data(), MatVals<V>::a(S), rp2, c2, v2, x, pprev, pcur, b, y, rz_a,
-                          rz_b, dot, pc);
+                          rz_b, rz_parts, dot, pc);
     SG_LAUNCH_CHECK();
     if(dot)
         set_finalize(dot, (int)grid_for((long long)A.nb * kRowLanes, 256), pc.ctl, s);
@@ -1026,7 +1048,10 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
     }
     const int* done = ctl_.data();
 
-    // z = M r with the optional update; rz[k] -> rz_set(k)
+    // z = M r with the optional update; rz[k] -> rz_set(k).  With MAS, the
+    // sets rz_set(k), k >= 1, are left as workgroup partials: the SpMV of
+    // iteration k + 1 (their first reader) sums them (rz_parts).
+    const int rz_parts = pc == Precond::Mas ? mas->collect_blocks() : 0;
     auto precond = [&](bool axpy, int k, const V* p, const int* dn) {
         double* rz_prev = axpy ? rz_set(k - 1) : nullptr;
         double* pap     = axpy ? pap_set(k) : nullptr;
@@ -1034,7 +1059,7 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         double* zero1   = rz_set(k + 2);
         if(pc == Precond::Mas)
             mas->apply_fused(axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(), rz_prev, pap,
-                             W.z.data(), rz_set(k), zero0, zero1, dn, s);
+                             W.z.data(), rz_set(k), zero0, zero1, dn, s, k >= 1);
         else
             apply_block_jacobi<V>(nb, axpy ? x : (V*)nullptr, W.r.data(), p, W.Ap.data(),
                                   rz_prev, pap, W.z.data(), rz_set(k), zero0, zero1, dn, s);
@@ -1090,7 +1115,8 @@ PcgStats Pcg::solve_t(const SysView& S, Precond pc, Mas* mas, const V* b, V* x,
         launch_spmv<MODE_PCG, V>(S, W.z.data(), pprev, pcur, nullptr, W.Ap.data(), rz_set(k - 1),
                                  k >= 2 ? rz_set(k - 2) : rz_set(k - 1), pap_set(k), pcx, s,
                                  prof ? prof->kernel_events("spmv")
-                                      : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr});
+                                      : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr},
+                                 k >= 2 ? rz_parts : 0);
         if(test)
         {
             SG_HIP(hipMemcpyAsync(h_ctl_ + 4 * slot, ctl_.data(), 2 * sizeof(int),

@@ -715,16 +715,29 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
     T         pv[BD];
     bank_load(inv, b0 + (bi < nb ? bi : 0), pv);
     const int nc = lv.count[lt];
+    // goingNext and the member banks over the tail's node span, loaded
+    // before the R_lt gathers (independent of them), stored in LDS after
+    static_assert(kTailNodes <= 2 * 64 * kCoarseWaves, "two span entries per thread");
+    const int above = (lt + 1 < L) ? lv.off[lt + 1] - base : span;
+    int       upv[2];
+    int       cbv[2];
+#pragma unroll
+    for(int h = 0; h < 2; h++)
+    {
+        const int i = threadIdx.x + h * blockDim.x;
+        upv[h]      = i < span ? going_next[base + i] : -1;
+        cbv[h]      = (i < span && i >= above) ? child_bank[base + i] - b0 : 0;
+    }
     if(lb >= 0)
     {
         // R_lt summed from R_lb (level lb's banks run in this launch)
         for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
             if(i >= 3 * nc)
                 Rs[i] = (T)0;
-        // the bank's 16 parents, then its 48 values, each as one batch of
-        // independent loads (addresses of nodes past the level's end fall
-        // back to the bank's first node, which always exists; their values
-        // are not used)
+        // one thread per cluster q of level lt: the cluster's member bank,
+        // then its 16 parents and 48 values as one batch of independent
+        // loads (nodes past the level's end load the bank's first node, which
+        // always exists; their values are not used), summed in node order
         const int e0 = lv.off[lb] + lv.count[lb];
         for(int q = threadIdx.x; q < nc; q += blockDim.x)
         {
@@ -732,19 +745,19 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
             const int gb = child_bank[p];
             int       par[BANK];
 #pragma unroll
-            for(int u = 0; u < BANK; u++)
+            for(int v = 0; v < BANK; v++)
             {
-                const int g = gb * BANK + u;
-                par[u]      = g < e0 ? going_next[g] : -1;
+                const int g = gb * BANK + v;
+                par[v]      = g < e0 ? going_next[g] : -1;
             }
             T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-            for(int u = 0; u < BANK; u++)
+            for(int v = 0; v < BANK; v++)
             {
-                const int    g  = gb * BANK + u;
+                const int    g  = gb * BANK + v;
                 const size_t gg = 3 * (size_t)(g < e0 ? g : gb * BANK);
                 const T      r0 = R[gg], r1 = R[gg + 1], r2 = R[gg + 2];
-                if(par[u] == p)
+                if(par[v] == p)
                 {
                     s0 += r0;
                     s1 += r1;
@@ -761,12 +774,16 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
             Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
     }
-    const int above = (lt + 1 < L) ? lv.off[lt + 1] - base : span;
-    for(int i = threadIdx.x; i < span; i += blockDim.x)
+#pragma unroll
+    for(int h = 0; h < 2; h++)
     {
-        up[i] = going_next[base + i];
-        if(i >= above)
-            cb[i] = (unsigned char)(child_bank[base + i] - b0);
+        const int i = threadIdx.x + h * blockDim.x;
+        if(i < span)
+        {
+            up[i] = upv[h];
+            if(i >= above)
+                cb[i] = (unsigned char)cbv[h];
+        }
     }
     __syncthreads();
     for(int l = lt; l + 1 < L; l++)
@@ -1121,7 +1138,7 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
                         const int* child_bank, const T* inv, T* R, T* Z, V* x, V* r, const V* p,
                         const V* Ap, const double* rz_prev, const double* pap, V* z,
                         double* rz_out, double* zero0, double* zero1, const int* done,
-                        hipStream_t s)
+                        hipStream_t s, bool defer_rz)
 {
     // Plan: separate launches for levels 1 .. lb-1, one k_mas_coarse launch
     // for level lb plus the tail lt .. L-1 (lt = lb + 1).  The kernel of level
@@ -1203,14 +1220,14 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
     default: collect(k_mas_collect<T, V, -1>); break;
     }
     SG_LAUNCH_CHECK();
-    if(rz_out)
+    if(rz_out && !defer_rz)
         set_finalize(rz_out, (int)grid_for(nv, 256), done, s);
 }
 
 template <typename V>
 void Mas::apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_prev,
                         const double* pap, V* z, double* rz_out, double* zero0, double* zero1,
-                        const int* done, hipStream_t s)
+                        const int* done, hipStream_t s, bool defer_rz)
 {
     SG_CHECK(built_, -5, "MAS apply before build");
     if(cfg_.fp32)
@@ -1219,28 +1236,30 @@ void Mas::apply_fused_t(V* x, V* r, const V* p, const V* Ap, const double* rz_pr
                               bank_first_.data(), bank_ncl_.data(), child_bank_.data(),
                               inv32_.data(), R32_.data(),
                               Z32_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
-                              done, s);
+                              done, s, defer_rz);
     else
         mas_apply_t<double, V>(prof, nparts_, L_, nv_, h_lvl_, vert_of_slot_.data(),
                                slot_of_.data(), chain_.data(), going_next_.data(),
                                bank_first_.data(), bank_ncl_.data(), child_bank_.data(),
                                inv64_.data(), R64_.data(),
                                Z64_.data(), x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1,
-                               done, s);
+                               done, s, defer_rz);
 }
 
 void Mas::apply_fused(double* x, double* r, const double* p, const double* Ap,
                       const double* rz_prev, const double* pap, double* z, double* rz_out,
-                      double* zero0, double* zero1, const int* done, hipStream_t s)
+                      double* zero0, double* zero1, const int* done, hipStream_t s,
+                      bool defer_rz)
 {
-    apply_fused_t<double>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
+    apply_fused_t<double>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s, defer_rz);
 }
 
 void Mas::apply_fused(float* x, float* r, const float* p, const float* Ap,
                       const double* rz_prev, const double* pap, float* z, double* rz_out,
-                      double* zero0, double* zero1, const int* done, hipStream_t s)
+                      double* zero0, double* zero1, const int* done, hipStream_t s,
+                      bool defer_rz)
 {
-    apply_fused_t<float>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s);
+    apply_fused_t<float>(x, r, p, Ap, rz_prev, pap, z, rz_out, zero0, zero1, done, s, defer_rz);
 }
 
 void Mas::apply(const double* r, double* z, hipStream_t s)
