@@ -653,19 +653,23 @@ __global__ void __launch_bounds__(64) k_mas_lx(int bank0, int node0, int count,
         restrict_bank(Rc, rv, valid ? going_next[g] : -1, bank_first[gb], bank_ncl[gb], R);
 }
 
-// The coarse levels left after the separate launches, in one launch.
-// Workgroups [0, nwg_b) give one wave to each bank of level lb (R complete
-// from earlier launches, no restriction needed).  The remaining workgroups
-// (when lt < L) own the levels from lt up: each forms R_lt (loaded when the
-// level lt-1 kernel stored it; summed from R_lb when level lb = lt - 1 runs
-// in this launch), sums the higher levels' R in LDS, and gives one wave to
-// each of its share of the banks of those levels.  Every cluster sum adds
-// the cluster's member nodes in node order: the members of cluster p of
-// level l+1 are the nodes of the level-l bank child_bank[p] whose parent is
-// p.
+// The coarse levels from lt up (the tail), in one launch: every workgroup
+// loads R_lt (stored complete by the level lt-1 launch), sums the higher
+// levels' R in LDS and gives one wave to each of its share of the tail banks
+// (each wave at most one bank: the launch has ceil(banks / kCoarseWaves)
+// workgroups, each restricting R redundantly).  Everything it reads from
+// global memory -- this wave's bank inverse (FP32), R_lt, goingNext and the
+// member banks over the tail's node span -- is requested in one batch per
+// thread before the first use, so the chain is one memory round trip, the
+// LDS restriction and the product.  Every cluster sum adds the cluster's
+// member nodes in node order: the members of cluster p of level l+1 are the
+// nodes of the level-l bank cb[p] whose parent is p.
 constexpr int kCoarseWaves = 16;
-constexpr int kTailBanks   = 64;    // a level with at most this many banks may start the tail
-constexpr int kTailNodes   = 2048;  // node span of the tail levels (a multiple of 16)
+constexpr int kTailBanks   = 128;   // a level with at most this many banks may start the tail
+constexpr int kTailNodes   = 4096;  // node span of the tail levels (a multiple of 16)
+constexpr int kTailThreads = 64 * kCoarseWaves;
+constexpr int kTailSpanPer = kTailNodes / kTailThreads;      // span entries per thread
+constexpr int kTailRPer    = 3 * kTailNodes / kTailThreads;  // R values per thread
 struct LevelTable
 {
     int count[24];
@@ -673,111 +677,57 @@ struct LevelTable
 };
 
 template <typename T>
-__global__ void __launch_bounds__(64 * kCoarseWaves)
-    k_mas_coarse(LevelTable lv, int L, int lb, int nwg_b, int lt,
-                 const T* __restrict__ inv, const int* __restrict__ going_next,
-                 const int* __restrict__ child_bank, T* __restrict__ R, T* __restrict__ Z,
-                 const int* __restrict__ done)
+__global__ void __launch_bounds__(kTailThreads)
+    k_mas_coarse(LevelTable lv, int L, int lt, const T* __restrict__ inv,
+                 const int* __restrict__ going_next, const int* __restrict__ child_bank,
+                 const T* __restrict__ R, T* __restrict__ Z, const int* __restrict__ done)
 {
     if(done && *done)
         return;
-    const int w  = threadIdx.x >> 6;
-    const int t  = threadIdx.x & 63;
-    const int cc = t % 3;
-    if((int)blockIdx.x < nwg_b)
-    {
-        const int node0 = lv.off[lb], count = lv.count[lb];
-        const int bk    = blockIdx.x * kCoarseWaves + w;
-        const int nbk   = (count + BANK - 1) / BANK;
-        if(bk >= nbk)
-            return;
-        const int gb = node0 / BANK + bk;
-        const int g  = gb * BANK + t / 3;
-        T         rv = (t < BD && g - node0 < count) ? R[3 * (size_t)g + cc] : (T)0;
-        T         z  = bank_product(inv, gb, rv);
-        Z[(size_t)gb * ZB + t] = z;
-        return;
-    }
-    // ---- tail workgroup
-    // Everything the tail reads from global memory is requested up front:
-    // R_lt, goingNext over the tail's node span (into LDS) and this wave's
-    // bank inverse (each wave owns at most one tail bank: the launch gives
-    // ceil(banks / kCoarseWaves) tail workgroups), so the chain is one
-    // memory round trip, the LDS restriction, and the product.
     __shared__ T             Rs[3 * kTailNodes];
     __shared__ int           up[kTailNodes];
     __shared__ unsigned char cb[kTailNodes];  // member bank of a node above lt, minus b0
+    const int w    = threadIdx.x >> 6;
+    const int t    = threadIdx.x & 63;
     const int base = lv.off[lt];
     const int span = lv.off[L - 1] + lv.count[L - 1] - base;
     const int nb   = (span + BANK - 1) / BANK;  // banks of the tail levels
     const int b0   = base / BANK;
-    const int bi   = (blockIdx.x - nwg_b) * kCoarseWaves + w;
-    T         pv[BD];
-    bank_load(inv, b0 + (bi < nb ? bi : 0), pv);
-    const int nc = lv.count[lt];
-    // goingNext and the member banks over the tail's node span, loaded
-    // before the R_lt gathers (independent of them), stored in LDS after
-    static_assert(kTailNodes <= 2 * 64 * kCoarseWaves, "two span entries per thread");
+    const int bi   = blockIdx.x * kCoarseWaves + w;
+    const int nr   = 3 * lv.count[lt];          // R_lt values
+    const int nz   = 3 * BANK * nb;             // R entries of the tail in LDS
+    // FP64 inverses would spill the registers here: loaded after the restriction
+    T pv[BD];
+    if constexpr(sizeof(T) == 4)
+        bank_load(inv, b0 + (bi < nb ? bi : 0), pv);
     const int above = (lt + 1 < L) ? lv.off[lt + 1] - base : span;
-    int       upv[2];
-    int       cbv[2];
+    T         rl[kTailRPer];
+    int       upv[kTailSpanPer];
+    int       cbv[kTailSpanPer];
 #pragma unroll
-    for(int h = 0; h < 2; h++)
+    for(int h = 0; h < kTailRPer; h++)
     {
-        const int i = threadIdx.x + h * blockDim.x;
+        const int i = threadIdx.x + h * kTailThreads;
+        rl[h]       = i < nr ? R[3 * (size_t)base + i] : (T)0;
+    }
+#pragma unroll
+    for(int h = 0; h < kTailSpanPer; h++)
+    {
+        const int i = threadIdx.x + h * kTailThreads;
         upv[h]      = i < span ? going_next[base + i] : -1;
         cbv[h]      = (i < span && i >= above) ? child_bank[base + i] - b0 : 0;
     }
-    if(lb >= 0)
-    {
-        // R_lt summed from R_lb (level lb's banks run in this launch)
-        for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
-            if(i >= 3 * nc)
-                Rs[i] = (T)0;
-        // one thread per cluster q of level lt: the cluster's member bank,
-        // then its 16 parents and 48 values as one batch of independent
-        // loads (nodes past the level's end load the bank's first node, which
-        // always exists; their values are not used), summed in node order
-        const int e0 = lv.off[lb] + lv.count[lb];
-        for(int q = threadIdx.x; q < nc; q += blockDim.x)
-        {
-            const int p  = base + q;
-            const int gb = child_bank[p];
-            int       par[BANK];
 #pragma unroll
-            for(int v = 0; v < BANK; v++)
-            {
-                const int g = gb * BANK + v;
-                par[v]      = g < e0 ? going_next[g] : -1;
-            }
-            T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-            for(int v = 0; v < BANK; v++)
-            {
-                const int    g  = gb * BANK + v;
-                const size_t gg = 3 * (size_t)(g < e0 ? g : gb * BANK);
-                const T      r0 = R[gg], r1 = R[gg + 1], r2 = R[gg + 2];
-                if(par[v] == p)
-                {
-                    s0 += r0;
-                    s1 += r1;
-                    s2 += r2;
-                }
-            }
-            Rs[3 * q]     = s0;
-            Rs[3 * q + 1] = s1;
-            Rs[3 * q + 2] = s2;
-        }
-    }
-    else
+    for(int h = 0; h < kTailRPer; h++)
     {
-        for(int i = threadIdx.x; i < 3 * BANK * nb; i += blockDim.x)
-            Rs[i] = i < 3 * nc ? R[3 * (size_t)base + i] : (T)0;
+        const int i = threadIdx.x + h * kTailThreads;
+        if(i < nz)
+            Rs[i] = rl[h];
     }
 #pragma unroll
-    for(int h = 0; h < 2; h++)
+    for(int h = 0; h < kTailSpanPer; h++)
     {
-        const int i = threadIdx.x + h * blockDim.x;
+        const int i = threadIdx.x + h * kTailThreads;
         if(i < span)
         {
             up[i] = upv[h];
@@ -794,15 +744,31 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
             const int p  = n1 + q;
             const int gb = b0 + cb[p - base];
             T         s0 = 0, s1 = 0, s2 = 0;
-            for(int u = 0; u < BANK; u++)
+            // the bank's parents and values read as batches of 8 members
+            // (reads past the level's end stay inside the tail's LDS rows;
+            // their values are not used), summed in node order
+#pragma unroll
+            for(int h = 0; h < BANK; h += 8)
             {
-                const int g = gb * BANK + u;
-                if(g < e0 && up[g - base] == p)
+                int par[8];
+                T   v[24];
+#pragma unroll
+                for(int u = 0; u < 8; u++)
                 {
-                    s0 += Rs[3 * (g - base)];
-                    s1 += Rs[3 * (g - base) + 1];
-                    s2 += Rs[3 * (g - base) + 2];
+                    const int g = gb * BANK + h + u;
+                    par[u]      = g < e0 ? up[g - base] : -1;
+                    v[3 * u]     = Rs[3 * (g - base)];
+                    v[3 * u + 1] = Rs[3 * (g - base) + 1];
+                    v[3 * u + 2] = Rs[3 * (g - base) + 2];
                 }
+#pragma unroll
+                for(int u = 0; u < 8; u++)
+                    if(par[u] == p)
+                    {
+                        s0 += v[3 * u];
+                        s1 += v[3 * u + 1];
+                        s2 += v[3 * u + 2];
+                    }
             }
             Rs[3 * (p - base)]     = s0;
             Rs[3 * (p - base) + 1] = s1;
@@ -810,12 +776,13 @@ __global__ void __launch_bounds__(64 * kCoarseWaves)
         }
         __syncthreads();
     }
-    // the tail workgroups split the tail banks (each restricted R redundantly)
     if(bi < nb)
     {
         const int gb = b0 + bi;
         const T*  rs = Rs + 3 * (gb * BANK - base);
-        T         z  = 0;
+        if constexpr(sizeof(T) == 8)
+            bank_load(inv, gb, pv);
+        T z = 0;
 #pragma unroll
         for(int u = 0; u < BD; u++)
             z += pv[u] * rs[u];
@@ -1140,10 +1107,12 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
                         double* rz_out, double* zero0, double* zero1, const int* done,
                         hipStream_t s, bool defer_rz)
 {
-    // Plan: separate launches for levels 1 .. lb-1, one k_mas_coarse launch
-    // for level lb plus the tail lt .. L-1 (lt = lb + 1).  The kernel of level
-    // lt - 2 also accumulates R_lt so the tail can start at once.
-    int lt = L, lb = -1;
+    // Plan: separate launches for levels 1 .. lt-1 (each stores the next
+    // level's R), then one k_mas_coarse launch for the tail lt .. L-1, whose
+    // workgroups load R_lt and restrict the levels above in LDS.  (Summing R_lt
+    // from level lt-1 inside the tail launch costs two dependent gathers,
+    // more than the extra launch.)
+    int lt = L;
     if(L > 1)
     {
         for(int l = 1; l < L; l++)
@@ -1156,7 +1125,6 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
                 break;
             }
         }
-        lb = (lt >= 2) ? lt - 1 : -1;
     }
     auto ev = prof ? prof->kernel_events("mas_apply_l0")
                    : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
@@ -1166,12 +1134,12 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
     SG_LAUNCH_CHECK();
     if(L > 1)
     {
-        for(int l = 1; l < lb; l++)
+        for(int l = 1; l < lt; l++)
         {
             int count = lvl[2 * l], off = lvl[2 * l + 1];
             int nbk   = (count + BANK - 1) / BANK;
             hipLaunchKernelGGL(k_mas_lx<T>, dim3(nbk), dim3(64), 0, s, off / BANK, off, count, inv,
-                               going_next, bank_first, bank_ncl, R, Z, 1, done);
+                               going_next, bank_first, bank_ncl, R, Z, l + 1 < L ? 1 : 0, done);
             SG_LAUNCH_CHECK();
         }
         SG_CHECK(L <= 24, -1, "MAS: more than 24 levels");
@@ -1181,21 +1149,13 @@ static void mas_apply_t(Profiler* prof, int nparts, int L, int nv, const std::ve
             lv.count[q] = lvl[2 * q];
             lv.off[q]   = lvl[2 * q + 1];
         }
-        int nwg_b = 0;
-        if(lb >= 1)
-            nwg_b = ((lvl[2 * lb] + BANK - 1) / BANK + kCoarseWaves - 1) / kCoarseWaves;
-        int ntw = 0;
         if(lt < L)
         {
             int span = lvl[2 * (L - 1) + 1] + lvl[2 * (L - 1)] - lvl[2 * lt + 1];
             int nbt  = (span + BANK - 1) / BANK;
-            ntw      = (nbt + kCoarseWaves - 1) / kCoarseWaves;
-        }
-        int grid = nwg_b + ntw;
-        if(grid > 0)
-        {
-            hipLaunchKernelGGL(k_mas_coarse<T>, dim3(grid), dim3(64 * kCoarseWaves), 0, s, lv, L,
-                               lb, nwg_b, lt, inv, going_next, child_bank, R, Z, done);
+            int ntw  = (nbt + kCoarseWaves - 1) / kCoarseWaves;
+            hipLaunchKernelGGL(k_mas_coarse<T>, dim3(ntw), dim3(kTailThreads), 0, s, lv, L, lt,
+                               inv, going_next, child_bank, R, Z, done);
             SG_LAUNCH_CHECK();
         }
     }
