@@ -182,11 +182,16 @@ __device__ __forceinline__ bool ee_ok(const int* body, const int* bt, uint2 a, u
     return true;
 }
 
-// generic stack traversal: visit(leaf_element) for every overlapping leaf
+// generic stack traversal: visit(leaf_element) for every overlapping leaf.
+// With rng (each internal node's sorted leaf range) only leaves at sorted
+// positions above `after` are visited: a self query of the tree from its
+// sorted leaf `after` then meets every unordered pair of leaves once, from the
+// earlier one, and skips the subtrees that lie wholly at or before it.
 template <typename Visit>
 __device__ __forceinline__ void traverse(const uint32_t* __restrict__ nodes,
                                          const double* __restrict__ bvs, int n, const double* q,
-                                         double gap, uint32_t (*stk)[64], Visit&& visit)
+                                         double gap, uint32_t (*stk)[64], Visit&& visit,
+                                         const int* __restrict__ rng = nullptr, int after = -1)
 {
     const int lane = threadIdx.x;
     if(n == 1)
@@ -204,6 +209,12 @@ __device__ __forceinline__ void traverse(const uint32_t* __restrict__ nodes,
         for(int c = 0; c < 2; c++)
         {
             uint32_t C = ch[c];
+            if(rng)
+            {
+                const int last = (int)C >= n - 1 ? (int)C - (n - 1) : rng[2 * (size_t)C + 1];
+                if(last <= after)
+                    continue;
+            }
             if(!overlap(q, bvs + 6 * (size_t)C, gap))
                 continue;
             uint32_t obj = nodes[4 * (size_t)C + 3];
@@ -238,13 +249,17 @@ __global__ void __launch_bounds__(64) k_query_vf(int ns, const uint32_t* __restr
     });
 }
 
+// Edge-edge pairs: each edge queries the edge tree from its own sorted leaf
+// for the leaves after it, and a pair is emitted with its lower edge index
+// first, as the reference emits it from that edge's query (obj > self).
 __global__ void __launch_bounds__(64) k_query_ee(int ne, const uint2* __restrict__ edges,
                                                  const int* __restrict__ body,
                                                  const int* __restrict__ bt,
                                                  const double* __restrict__ x,
                                                  const double* __restrict__ xr,
                                                  const uint32_t* __restrict__ nodes,
-                                                 const double* __restrict__ bvs, double dHat,
+                                                 const double* __restrict__ bvs,
+                                                 const int* __restrict__ rng, double dHat,
                                                  PairSink sk)
 {
     __shared__ uint32_t stk[kStack][64];
@@ -256,13 +271,19 @@ __global__ void __launch_bounds__(64) k_query_ee(int ne, const uint2* __restrict
     double q[6];
     for(int c = 0; c < 6; c++)
         q[c] = bvs[6 * (size_t)leaf + c];
-    uint2  a   = edges[self];
+    uint2  e   = edges[self];
     double gap = sqrt(dHat);
-    traverse(nodes, bvs, ne, q, gap, stk, [&](uint32_t obj) {
-        uint2 b = edges[obj];
-        if(ee_ok(body, bt, a, b, self, (int)obj))
-            check_ee(x, xr, a.x, a.y, b.x, b.y, (int)obj, dHat, sk);
-    });
+    traverse(
+        nodes, bvs, ne, q, gap, stk,
+        [&](uint32_t obj) {
+            const int lo = self < (int)obj ? self : (int)obj;
+            const int hi = self < (int)obj ? (int)obj : self;
+            uint2     o  = edges[obj];
+            uint2     a  = lo == self ? e : o, b = lo == self ? o : e;
+            if(ee_ok(body, bt, a, b, lo, hi))
+                check_ee(x, xr, a.x, a.y, b.x, b.y, hi, dHat, sk);
+        },
+        rng, i);
 }
 
 struct CandSink
@@ -319,7 +340,8 @@ __global__ void __launch_bounds__(64) k_ccd_ee(int ne, const uint2* __restrict__
                                                const int* __restrict__ body,
                                                const int* __restrict__ bt,
                                                const uint32_t* __restrict__ nodes,
-                                               const double* __restrict__ bvs, double dHat,
+                                               const double* __restrict__ bvs,
+                                               const int* __restrict__ rng, double dHat,
                                                CandSink sk)
 {
     __shared__ uint32_t stk[kStack][64];
@@ -331,13 +353,19 @@ __global__ void __launch_bounds__(64) k_ccd_ee(int ne, const uint2* __restrict__
     double q[6];
     for(int c = 0; c < 6; c++)
         q[c] = bvs[6 * (size_t)leaf + c];
-    uint2  a   = edges[self];
+    uint2  e   = edges[self];
     double gap = sqrt(dHat);
-    traverse(nodes, bvs, ne, q, gap, stk, [&](uint32_t obj) {
-        uint2 b = edges[obj];
-        if(ee_ok(body, bt, a, b, self, (int)obj))
-            sk.emit(make_int4(a.x, a.y, b.x, b.y));
-    });
+    traverse(
+        nodes, bvs, ne, q, gap, stk,
+        [&](uint32_t obj) {
+            const int lo = self < (int)obj ? self : (int)obj;
+            const int hi = self < (int)obj ? (int)obj : self;
+            uint2     o  = edges[obj];
+            uint2     a  = lo == self ? e : o, b = lo == self ? o : e;
+            if(ee_ok(body, bt, a, b, lo, hi))
+                sk.emit(make_int4(a.x, a.y, b.x, b.y));
+        },
+        rng, i);
 }
 
 __device__ bool seg_tri(V3 e0, V3 e1, V3 t0, V3 t1, V3 t2)
@@ -423,7 +451,7 @@ void launch_query_proximity(const SimDev& d, const Lbvh& bf, const Lbvh& be, con
     if(d.ne > 1)
         hipLaunchKernelGGL(k_query_ee, dim3(grid_for(d.ne, 64)), dim3(64), 0, s, d.ne,
                            (const uint2*)d.edges, d.body, d.btype, d.x, xr, be.nodes.data(),
-                           be.bvs.data(), d.dHat, sk);
+                           be.bvs.data(), be.rng.data(), d.dHat, sk);
     SG_LAUNCH_CHECK();
 }
 
@@ -440,7 +468,7 @@ void launch_query_ccd(const SimDev& d, const Lbvh& bf, const Lbvh& be, const dou
     if(d.ne > 1)
         hipLaunchKernelGGL(k_ccd_ee, dim3(grid_for(d.ne, 64)), dim3(64), 0, s, d.ne,
                            (const uint2*)d.edges, d.body, d.btype, be.nodes.data(), be.bvs.data(),
-                           d.dHat, sk);
+                           be.rng.data(), d.dHat, sk);
     SG_LAUNCH_CHECK();
 }
 
