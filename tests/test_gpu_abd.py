@@ -172,8 +172,8 @@ def test_hybrid_impact_newton_iterations_match(oracle):
 def test_hybrid_impact_steps(oracle):
     """time steps of the impact (20-40 Newton iterations each): each step
     starts the oracle from the GPU's state.  Newton converges slowly on the
-    soft bed, so when rounding (GPU atomic order, bank inverses eliminated in
-    another order) splits the Newton paths, the two end points differ by more
+    soft bed, so when rounding (element and contact sums in another order on
+    each side) splits the Newton paths, the two end points differ by more
     than the last Newton step: the steps are held to 5e-3 of the scene scale
     (1e-4 when Newton and PCG counts agree); the boxes stay rigid and the fixed
     bed layer does not move"""
@@ -203,6 +203,31 @@ def test_hybrid_impact_steps(oracle):
         d0 = np.linalg.norm(x0[abd == b][:, None] - x0[abd == b][None], axis=2)
         d = np.linalg.norm(xg[abd == b][:, None] - xg[abd == b][None], axis=2)
         assert np.abs(d - d0).max() < 1e-6
+
+
+def test_hybrid_impact_trajectory_to_the_step_minimiser(oracle):
+    """the same impact as one trajectory (no restarts): with PCG stopped on
+    the FP64 true residual 1e-6 and Newton on |dx|_inf < 1e-6 of the scene
+    diagonal, each step ends within that distance of the incremental
+    potential's minimiser on both sides, whatever path rounding takes; four
+    steps (70 Newton iterations each) held to 1e-5 of the scene scale"""
+    import sgipc
+    sc = _hybrid()
+    sc.pcg_rel_tol = 1e-6
+    sc.pcg_use_preconditioned_norm = 0
+    sc.abs_xdelta_tol = 1e-6
+    sim = sgipc.Simulation(sc)
+    w = oracle.World(sc)
+    for step in range(4):
+        sg = sim.time_step()
+        so = w.time_step()
+        xg, xo = sim.positions(), w.positions()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"[hybrid-tight] step={step} newton={sg['newton_iterations']}/"
+              f"{so['newton_iterations']} pcg={sg['pcg_iterations']}/{so['pcg_iterations']} "
+              f"err={err:.2e}", flush=True)
+        assert sg["capped"] == 0
+        assert err < 1e-5
 
 
 def test_contact_table_growth(oracle, monkeypatch):
