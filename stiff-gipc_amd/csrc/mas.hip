@@ -564,7 +564,9 @@ __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_s
     if(done && *done)
         return;
     __shared__ T Rc[BD];
-    const int b  = blockIdx.x;
+    // banks in contiguous ranges per XCD, as the SpMV's block rows: a bank's
+    // vertices are then mostly rows the same XCD's SpMV workgroups wrote
+    const int b  = (int)xcd_block(blockIdx.x, gridDim.x);
     const int t  = threadIdx.x;
     const int s  = b * BANK + t / 3;
     const int cc = t % 3;
@@ -806,7 +808,7 @@ __global__ void __launch_bounds__(256)
 {
     if(done && *done)
         return;
-    int    v = blockIdx.x * blockDim.x + threadIdx.x;
+    int    v = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double d = 0.0;
     if(v < nv)
     {
