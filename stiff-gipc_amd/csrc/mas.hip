@@ -544,9 +544,9 @@ __device__ __forceinline__ void restrict_bank(T (&Rc)[BD], T rv, int node_up, in
 }
 
 // Level-0 banks (one wave each): optional PCG update of x and r
-// (a = rz[k-1] / pAp[k] from slot sets), z0 = Binv r, and the level-1 (and,
-// with up2, level-2) cluster sums of r.  T: preconditioner precision, V:
-// precision of the PCG vectors.
+// (a = rz[k-1] / pAp[k] from slot sets), z0 = Binv r, and the level-1
+// cluster sums of r.  T: preconditioner precision, V: precision of the PCG
+// vectors.
 template <typename T, typename V>
 __global__ void __launch_bounds__(64) k_mas_l0(const int* __restrict__ vert_of_slot,
                                                V* __restrict__ x, V* __restrict__ r,
