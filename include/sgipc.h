@@ -361,8 +361,9 @@ int sgipc_learned_mode(sgipc_t ctx, int32_t mode);
 /* on (the default): each warm start's net gain is its predicted saving (from
  * the guess's preconditioned residual and PCG's linear convergence) minus its
  * own device time; while the running average (weight 1/2) of the net gains is
- * negative, the guesses are skipped for the next 2 solves, the pause doubling
- * up to 32; off: every solve computes and judges the guesses */
+ * negative, the guesses are skipped for the next 2 solves (8 when the probe
+ * saved less than a quarter of its cost), the pause doubling up to 32; off:
+ * every solve computes and judges the guesses */
 int sgipc_warm_start_adaptive(sgipc_t ctx, int32_t on);
 /* training samples from the last solving Newton iteration: features
  * [n][SGIPC_MLP_IN] and targets [n][SGIPC_MLP_OUT] ((q* - q_c) / |q_c| with q*

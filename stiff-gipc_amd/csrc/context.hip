@@ -1302,7 +1302,8 @@ void Context::newton_iteration(sgipc_iter_stats* st)
 // f / (1 - f) times the solve that ran (none when the zero guess won).  The
 // net gain (saving minus cost) is averaged over the probes with weight 1/2;
 // while that average is negative the warm start is paused for the next 2
-// solves, the pause doubling (up to 32) while it stays negative.  Occasional
+// solves (8 when the probe saved less than a quarter of its cost), the pause
+// doubling (up to 32) while it stays negative.  Occasional
 // rejected guesses among paying ones therefore do not pause it.
 PcgStats Context::solve_plain(const SysView& S, Precond pc, const PcgConfig& cfg, double* sol)
 {
@@ -1385,12 +1386,14 @@ PcgStats Context::solve_linear(const SysView& S, Precond pc, const PcgConfig& cf
                 (void)hipEventDestroy(e);
             if(!warm_adaptive_)
                 return;
-            double net = solve * f / (1.0 - f) - cost;
-            warm_gain_ = warm_probes_++ ? 0.5 * (warm_gain_ + net) : net;
+            double saving = solve * f / (1.0 - f);
+            double net    = saving - cost;
+            warm_gain_    = warm_probes_++ ? 0.5 * (warm_gain_ + net) : net;
             if(warm_gain_ < 0.0)
             {
-                warm_pause_ = warm_pause_ ? std::min(2 * warm_pause_, 32) : 2;
-                warm_skip_  = warm_pause_;
+                const int first = saving < 0.25 * cost ? 8 : 2;
+                warm_pause_     = warm_pause_ ? std::min(2 * warm_pause_, 32) : first;
+                warm_skip_      = warm_pause_;
             }
             else
                 warm_pause_ = 0;
