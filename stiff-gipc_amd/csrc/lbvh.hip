@@ -503,15 +503,27 @@ void Lbvh::init(int n_, int arity_, const uint32_t* prims_)
     int nblk_agg = (n + kAgg - 1) / kAgg;
     agg_blk.resize(6 * (size_t)nblk_agg);
     agg_sup.resize(6 * (size_t)((nblk_agg + kAgg - 1) / kAgg));
-    int nblk = (n + kSortTile - 1) / kSortTile;
-    hist.resize((size_t)256 * nblk);
+    hist.resize(radix_sort_hist_size(n));
     scene.resize(6);
     partial.resize(6 * (size_t)grid_for(n, 256));
 }
 
+// exclusive scan of the digit-major histogram (256 digits x nblk tiles), in
+// place: one workgroup for short histograms, the three-launch tiled scan
+// (tile sums, their scan, apply; its tile sums after the histogram) for long
+// ones, where one workgroup's sequential rounds take longer than two launches
+static void sort_scan(unsigned* hist, int nblk, hipStream_t s)
+{
+    const int total = 256 * nblk;
+    if(nblk < 128)
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, total, hist);
+    else
+        scan_exclusive_u32(total, hist, hist, hist + total, s);
+}
+
 // stable LSD radix sort of n 64-bit keys on the 8-bit digits from bit lo to
 // bit hi (exclusive); the result ends in keys (alt: scratch of n keys, hist:
-// 256 * ceil(n / kSortTile) counters)
+// radix_sort_hist_size(n) counters)
 void radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int lo, int hi,
                      hipStream_t s)
 {
@@ -523,7 +535,7 @@ void radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int l
     for(int sh = lo; sh < hi; sh += 8)
     {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, hist, nblk);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist);
+        sort_scan(hist, nblk, s);
         hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
                            hist, nblk, nullptr, nullptr);
         SG_LAUNCH_CHECK();
@@ -546,7 +558,7 @@ void radix_sort_pairs(uint64_t* keys, uint64_t* kalt, uint32_t* vals, uint32_t* 
     for(int sh = lo; sh < hi; sh += 8)
     {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, hist, nblk);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, 256 * nblk, hist);
+        sort_scan(hist, nblk, s);
         hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nblk), dim3(kSortThreads), 0, s, n, sh, a, b,
                            hist, nblk, va, vb);
         SG_LAUNCH_CHECK();
@@ -568,7 +580,11 @@ int radix_key_bits(uint64_t max_key)
     return (b + 7) / 8 * 8;
 }
 
-size_t radix_sort_hist_size(int n) { return (size_t)256 * ((n + kSortTile - 1) / kSortTile); }
+size_t radix_sort_hist_size(int n)
+{
+    const int total = 256 * ((n + kSortTile - 1) / kSortTile);
+    return (size_t)total + scan_tmp_size(total);
+}
 
 void Lbvh::build(const double* x, const double* dx, double alpha, hipStream_t s)
 {

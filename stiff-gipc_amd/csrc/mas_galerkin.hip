@@ -127,9 +127,11 @@ __global__ void __launch_bounds__(1024) k_scan_top(int nt, unsigned* __restrict_
         tsum[nt] = carry;
 }
 
-__global__ void __launch_bounds__(256) k_scan_apply(int n, const unsigned* __restrict__ in,
+// in and out may be the same array: each thread reads its entries before the
+// scan and writes them after it
+__global__ void __launch_bounds__(256) k_scan_apply(int n, const unsigned* in,
                                                     const unsigned* __restrict__ tsum,
-                                                    unsigned* __restrict__ out)
+                                                    unsigned* out)
 {
     __shared__ unsigned ws[4];
     const int base = blockIdx.x * kScanTile + 4 * threadIdx.x;

@@ -163,7 +163,8 @@ void launch_step_forward(const SimDev& d, const double* xs, const double* dx, do
                          hipStream_t s);
 void launch_update_velocity(const SimDev& d, double* xprev, double* v, hipStream_t s);
 
-// lbvh.hip: stable radix sort of 64-bit keys on bits [lo, hi) (8-bit digits)
+// lbvh.hip: stable radix sort of 64-bit keys on bits [lo, hi) (8-bit digits);
+// hist holds radix_sort_hist_size(n) counters
 void   radix_sort_keys(uint64_t* keys, uint64_t* alt, unsigned* hist, int n, int lo, int hi,
                        hipStream_t s);
 size_t radix_sort_hist_size(int n);
@@ -173,8 +174,9 @@ void radix_sort_pairs(uint64_t* keys, uint64_t* kalt, uint32_t* vals, uint32_t* 
 // the digit-rounded bit count that covers keys up to max_key
 int radix_key_bits(uint64_t max_key);
 
-// mas_galerkin.hip: exclusive scan of n unsigned values into out; tsum holds
-// scan_tmp_size(n) entries, the total ends in tsum[scan_tmp_size(n) - 1]
+// mas_galerkin.hip: exclusive scan of n unsigned values into out (out may be
+// in); tsum holds scan_tmp_size(n) entries, the total ends in
+// tsum[scan_tmp_size(n) - 1]
 void   scan_exclusive_u32(int n, const unsigned* in, unsigned* out, unsigned* tsum, hipStream_t s);
 size_t scan_tmp_size(int n);
 
