@@ -33,7 +33,11 @@ enum sgipc_mas_precision {
     SGIPC_MAS_FP64 = 1  /* all-FP64 MAS                                           */
 };
 
-/* linear-solver precision mode */
+/* linear-solver precision mode.  MIXED: when the FP32 iteration stalls or
+ * breaks down, FP64 PCG continues from its iterate, and the next 2 mixed
+ * solves of the same context or system run in FP64 directly (the pause
+ * doubling up to 32 while FP32 attempts keep failing; restarted when the PCG
+ * settings change) */
 enum sgipc_solver_precision {
     SGIPC_SOLVE_FP64  = 0, /* FP64 SpMV / PCG vectors (reference default)          */
     SGIPC_SOLVE_MIXED = 1  /* FP32 SpMV+MAS inner PCG, FP64 residual refinement    */

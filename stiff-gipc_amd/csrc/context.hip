@@ -896,6 +896,7 @@ void Context::set_pcg(const sgipc_pcg_cfg& c)
     desc_.pcg_use_preconditioned_norm = c.use_preconditioned_norm;
     desc_.solver_precision            = c.solver_precision;
     plain_iters_                      = 0.0;  // counts of the former settings
+    pcg_.reset_precision();
 }
 
 void Context::get(const DBuf<double>& b, double* out)

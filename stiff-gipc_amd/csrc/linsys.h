@@ -332,6 +332,15 @@ class Pcg
     static size_t workspace_doubles(int n);
     void          bind_workspace(double* base, int n);
 
+    // mixed precision: after an FP32 solve that stalls or breaks down, the
+    // next 2 mixed solves run in FP64 directly, the pause doubling (up to 32)
+    // while the FP32 attempts keep failing; restarted by reset_precision()
+    void reset_precision()
+    {
+        fp32_pause_ = 0;
+        fp32_skip_  = 0;
+    }
+
   private:
     // b64/x64 non-null: FP64 residual replacement at every check (mixed mode)
     template <typename V>
@@ -373,6 +382,7 @@ class Pcg
     // stream synchronisation in place of the pinned copy and its event
     bool         external_ = false;
     int          h_ctl_local_[8] = {};
+    int          fp32_pause_ = 0, fp32_skip_ = 0;
 };
 
 }  // namespace sgipc

@@ -1293,6 +1293,15 @@ PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* 
 {
     const int n = 3 * S.A->nb;
     ensure(n);
+    if(fp32_skip_ > 0)
+    {
+        // the recent FP32 attempts stalled: this solve runs in FP64
+        fp32_skip_--;
+        PcgStats f         = solve_t<double>(S, pc, mas, b, x, cfg, s, nullptr, nullptr);
+        f.outer_iterations = 1;
+        f.rel_residual     = true_residual(S, b, x, s);
+        return f;
+    }
     refresh_fp32(S, s);
     v64_.ensure(n);
     v32_.ensure(n);
@@ -1308,6 +1317,13 @@ PcgStats Pcg::solve_mixed(const SysView& S, Precond pc, Mas* mas, const double* 
     st.rel_residual     = true_residual(S, b, x, s);
     bool ok = st.breakdown == 0 && st.rel_residual == st.rel_residual
               && (st.converged || cfg.max_iter > 0);
+    if(ok)
+        fp32_pause_ = 0;
+    else
+    {
+        fp32_pause_ = fp32_pause_ ? std::min(2 * fp32_pause_, 32) : 2;
+        fp32_skip_  = fp32_pause_;
+    }
     if(!ok)
     {
         // The FP32 iteration broke down, diverged or stalled (FP32 rounding
