@@ -149,6 +149,7 @@ class MasGalerkin
     DBuf<uint64_t> keys_, kalt_;
     DBuf<uint32_t> vals_, valt_;
     DBuf<unsigned> flag_, pos_, hist_, tsum_;
+    DBuf<int>      rstart_;  // start of each run of equal keys
     DBuf<int>      er_[2], ec_[2];
     DBuf<double>   eb_[2];
 };

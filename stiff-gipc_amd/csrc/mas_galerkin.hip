@@ -68,6 +68,15 @@ __global__ void k_gal_heads(int n, const uint64_t* __restrict__ keys, uint64_t k
     flag[k]            = (key != kdead && (k == 0 || keys[k - 1] != key)) ? 1u : 0u;
 }
 
+// run starts in run order: rstart[pos[k]] = k for every run head k
+__global__ void k_gal_starts(int n, const unsigned* __restrict__ flag,
+                             const unsigned* __restrict__ pos, int* __restrict__ rstart)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < n && flag[k])
+        rstart[pos[k]] = k;
+}
+
 // exclusive scan of n unsigned values: tile sums, a one-workgroup scan of the
 // tile sums (total in tsum[nt]), then each tile rescanned with its offset
 __device__ __forceinline__ unsigned block_scan_excl(unsigned v, unsigned* ws, unsigned& total)
@@ -151,36 +160,68 @@ __global__ void __launch_bounds__(256) k_scan_apply(int n, const unsigned* in,
     }
 }
 
-// one thread per run head: the run's blocks in sorted order (ties in entry
-// order), stored as entry pos[k] of A_l and, within a bank, into B
-__global__ void k_gal_reduce(int n, const uint64_t* __restrict__ keys,
-                             const uint32_t* __restrict__ vals, const unsigned* __restrict__ flag,
-                             const unsigned* __restrict__ pos, int nsrc0,
+// one thread per run (runs packed by their start, the count in *nruns): the
+// run's blocks in sorted order (ties in entry order), four entries' indices
+// and blocks requested at a time, stored as entry r of A_l and, within a
+// bank, into B
+__global__ void k_gal_reduce(int n, const unsigned* __restrict__ nruns,
+                             const int* __restrict__ rstart, const uint64_t* __restrict__ keys,
+                             const uint32_t* __restrict__ vals, int nsrc0,
                              const double* __restrict__ src0, const double* __restrict__ src1,
                              int offl, long long nl, int* __restrict__ er, int* __restrict__ ec,
                              double* __restrict__ eb, double* __restrict__ B)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if(k >= n || !flag[k])
+    const int      r   = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned cnt = *nruns;
+    if(r >= (int)cnt)
         return;
+    const int      k   = rstart[r];
     const uint64_t key = keys[k];
-    double         s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for(int m = k; m < n && keys[m] == key; m++)
+    int            e   = (r + 1 < (int)cnt) ? rstart[r + 1] : n;
+    if(r + 1 == (int)cnt)  // the last run ends where the dead keys start
     {
-        const uint32_t v   = vals[m];
-        const double*  blk = (int)v < nsrc0 ? src0 + 9 * (size_t)v : src1 + 9 * (size_t)(v - nsrc0);
-#pragma unroll
-        for(int e = 0; e < 9; e++)
-            s[e] += blk[e];
+        int m = k + 1;
+        while(m < n && keys[m] == key)
+            m++;
+        e = m;
     }
-    const int      ci = offl + (int)(key / (uint64_t)nl);
-    const int      cj = offl + (int)(key % (uint64_t)nl);
-    const unsigned p  = pos[k];
-    er[p]             = ci;
-    ec[p]             = cj;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    auto   src  = [&](uint32_t v) {
+        return (int)v < nsrc0 ? src0 + 9 * (size_t)v : src1 + 9 * (size_t)(v - nsrc0);
+    };
+    int m = k;
+    for(; m + 4 <= e; m += 4)
+    {
+        const double* blk[4];
+        double        val[4][9];
 #pragma unroll
-    for(int e = 0; e < 9; e++)
-        eb[9 * (size_t)p + e] = s[e];
+        for(int q = 0; q < 4; q++)
+            blk[q] = src(vals[m + q]);
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+#pragma unroll
+            for(int c = 0; c < 9; c++)
+                val[q][c] = blk[q][c];
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+#pragma unroll
+            for(int c = 0; c < 9; c++)
+                s[c] += val[q][c];
+    }
+    for(; m < e; m++)
+    {
+        const double* blk = src(vals[m]);
+#pragma unroll
+        for(int c = 0; c < 9; c++)
+            s[c] += blk[c];
+    }
+    const int ci = offl + (int)(key / (uint64_t)nl);
+    const int cj = offl + (int)(key % (uint64_t)nl);
+    er[r]        = ci;
+    ec[r]        = cj;
+#pragma unroll
+    for(int c = 0; c < 9; c++)
+        eb[9 * (size_t)r + c] = s[c];
     if((ci >> 4) == (cj >> 4))
     {
         double* M = B + (size_t)(ci >> 4) * 48 * 48 + (3 * (ci & 15)) * 48 + 3 * (cj & 15);
@@ -217,6 +258,7 @@ void MasGalerkin::ensure(int n)
     valt_.resize(n);
     flag_.resize(n);
     pos_.resize(n);
+    rstart_.resize(n);
     hist_.resize(radix_sort_hist_size(n));
     tsum_.resize((n + kScanTile - 1) / kScanTile + 1);
     for(int q = 0; q < 2; q++)
@@ -244,9 +286,11 @@ int MasGalerkin::reduce(int n, uint64_t kdead, int nsrc0, const double* src0, co
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, nt, tsum_.data());
     hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, n, flag_.data(), tsum_.data(),
                        pos_.data());
-    hipLaunchKernelGGL(k_gal_reduce, dim3(g), dim3(256), 0, s, n, keys_.data(), vals_.data(),
-                       flag_.data(), pos_.data(), nsrc0, src0, src1, offl, nl, er_[out].data(),
-                       ec_[out].data(), eb_[out].data(), B);
+    hipLaunchKernelGGL(k_gal_starts, dim3(g), dim3(256), 0, s, n, flag_.data(), pos_.data(),
+                       rstart_.data());
+    hipLaunchKernelGGL(k_gal_reduce, dim3(g), dim3(256), 0, s, n, tsum_.data() + nt,
+                       rstart_.data(), keys_.data(), vals_.data(), nsrc0, src0, src1, offl, nl,
+                       er_[out].data(), ec_[out].data(), eb_[out].data(), B);
     SG_LAUNCH_CHECK();
     unsigned cnt = 0;
     SG_HIP(hipMemcpyAsync(&cnt, tsum_.data() + nt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
