@@ -257,8 +257,8 @@ Context::Context(const sgipc_scene_desc& d)
                 {
                     M[12 * k + 3 + 3 * k + l] += mass[v] * xb[l];
                     M[12 * (3 + 3 * k + l) + k] += mass[v] * xb[l];
-                    for(int l2 = 0; l2 < 3; l2++)
-                        M[12 * (3 + 3 * k + l) + 3 + 3 * k + l2] += mass[v] * xb[l] * xb[l2];
+                    for(int l2 = 0; l2 < 3; l2++)  // xb_l xb_l2 first: M exactly symmetric
+                        M[12 * (3 + 3 * k + l) + 3 + 3 * k + l2] += mass[v] * (xb[l] * xb[l2]);
                 }
             }
         }
