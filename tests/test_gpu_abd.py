@@ -294,3 +294,25 @@ def test_motor_body_trajectory(oracle):
     x0, x = sc.x[abd == 0], sim.positions()[abd == 0]
     A, *_ = np.linalg.lstsq(x0 - x0.mean(0), x - x.mean(0), rcond=None)
     assert np.arctan2(A.T[2, 1], A.T[1, 1]) < -0.01
+
+
+def test_contact_free_system_is_exactly_symmetric():
+    """before any contact the Newton matrix comes from the element gathers
+    (each lower block the exact transpose of its upper partner) and the
+    affine-body node blocks (mass matrix accumulated symmetrically, packed
+    shape Hessian): A equals its transpose bit for bit, FEM and affine rows
+    alike"""
+    import sgipc
+    from sgipc import scene as S
+    sc = S.hybrid_scene(n_side=2, box_cells=2, bed_cells=(8, 4, 8), bed_size=0.4,
+                        box_size=0.08, speed=2.0, gap=0.05)
+    sim = sgipc.Simulation(sc)
+    sim.newton_iteration()
+    assert int(sim.sizes()["pairs"]) == 0
+    _, rowptr, cols, vals = sim.assemble()
+    A = _dense(sim.nodes, rowptr, cols, vals)
+    nfem = 3 * (sim.nodes - 4 * 8)
+    print(f"[symmetric] n={A.shape[0]} max|A-A^T|={np.abs(A - A.T).max():.3e}", flush=True)
+    assert np.abs(A[nfem:, nfem:]).max() > 0  # affine node blocks present
+    assert np.array_equal(A, A.T)
+    sim.close()
